@@ -1,0 +1,119 @@
+/*
+ * mcx.h -- C ABI of the gfx950 TSDF -> mesh library (libmcx.so).
+ *
+ * Plain C: pointers, sizes and status codes only.  Device pointers are HIP
+ * device addresses; `stream` is a hipStream_t passed as void* (NULL = the
+ * null stream).  Every call is stream-ordered and re-entrant: all mutable
+ * state lives in caller-owned workspace memory, never in device globals
+ * (unlike global_count/output_count/blocks_done in the reference,
+ * cu_marching_cubes_v1.cu:49-51).  Calls that share one workspace must be
+ * ordered on one stream.
+ *
+ * Status codes replace the reference's abort-on-error (checkCudaErrors,
+ * cu_device.h:45-151) and its silent overflow clamp (:158-161, :176).
+ */
+#ifndef MCX_H
+#define MCX_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+#define MCX_OK 0
+#define MCX_ERR_ARG 1         /* bad argument (null pointer, size, kind) */
+#define MCX_ERR_CAPACITY 2    /* output capacity too small; *count holds the
+                                 required number of vertices */
+#define MCX_ERR_WORKSPACE 3   /* workspace too small or not initialised */
+#define MCX_ERR_HIP 4         /* a HIP runtime call failed */
+#define MCX_ERR_INTERNAL 5    /* device-side consistency check tripped */
+#define MCX_ERR_UNSUPPORTED 6 /* combination not supported (e.g. too wide) */
+
+/* ---- voxel kinds ------------------------------------------------------- */
+/* cu_voxel_t / short2 {int16 tsdf; uint16 w}, [z][y][x], x fastest
+ * (structure.h:14-18, 37-39); f = tsdf / 32766 (cu_kinect_fusion_v1.h:253-256),
+ * a cube with any zero weight is empty (cu_marching_cubes_v1.cu:70-94). */
+#define MCX_ELEM_SHORT2 0
+/* plain fp32 field, same layout, no weights (differentiable form). */
+#define MCX_ELEM_F32 1
+
+/* vertex_voxel (structure.h:29-35), 28 bytes: ver_pos, vox_pos, tsdf, w. */
+typedef struct {
+  float x, y, z;
+  int32_t vx, vy, vz;
+  int16_t tsdf;
+  int16_t w;
+} mcx_vertex_voxel;
+
+/* ---- library ----------------------------------------------------------- */
+int mcx_version(void);
+const char* mcx_strerror(int status);
+/* Host copy of the packed case table used by the kernels (host-only call):
+ * case_table[c] = 15 x 4-bit edge ids | (#triangles << 60); num_verts[c] =
+ * 3 * #triangles.  Re-encodes cu_marching_cubes_v1.cu:509-1028. */
+int mcx_get_tables(uint64_t* case_table, uint8_t* num_verts);
+
+/* ---- workspace --------------------------------------------------------- */
+/* Bytes of workspace that mcx_extract needs for an X*Y*Z volume. */
+size_t mcx_extract_workspace_size(int X, int Y, int Z);
+/* Zero a workspace once before its first use (stream-ordered fill kernel). */
+int mcx_workspace_init(void* d_ws, size_t ws_bytes, void* stream);
+
+/* ---- extraction (replaces MarchingCubes::run, cu_marching_cubes_v1.cu:417-467:
+ *      getOccupiedVoxels :185-206, computeOffsetsAndTotalVertexes :208-225,
+ *      generateTriangles :372-396, in one fused kernel) --------------------
+ * Outputs (each optional, NULL = not written):
+ *   d_vv        vertex_voxel records (SHORT2 only)
+ *   d_pos       float3 positions
+ *   d_v2vox     int3 vertex->voxel map (vertexindex_to_voxel)
+ *   d_ver_voxel int16 per voxel, zeroed then scattered (SHORT2 only)
+ * Vertices are in canonical order (ascending cube linear id, then the
+ * reference's reversed-winding slot order); entries with index >= cap are
+ * not written.  d_count (device int64, optional) receives the total.
+ * h_count (host, optional): the call synchronises the stream, stores the
+ * total and returns MCX_ERR_CAPACITY if it exceeds cap. */
+int mcx_extract(const void* d_vol, int elem, int X, int Y, int Z, float h,
+                mcx_vertex_voxel* d_vv, float* d_pos, int32_t* d_v2vox,
+                int16_t* d_ver_voxel, int64_t cap, int64_t* d_count,
+                int64_t* h_count, void* d_ws, size_t ws_bytes, void* stream);
+
+/* Per-cube classification (the per-cube part of OccupiedVoxels,
+ * cu_marching_cubes_v1.cu:120-131): d_ci / d_nv (X*Y*Z bytes each, indexed by
+ * the cube's min-corner voxel; 0 for cubes that do not exist). */
+int mcx_classify(const void* d_vol, int elem, int X, int Y, int Z,
+                 uint8_t* d_ci, uint8_t* d_nv, void* stream);
+
+/* ---- TSDF volume (replaces cu_voxel_new / d_voxel_initialize,
+ *      cu_kinect_fusion_v1.cu:364-398): every voxel = (tsdf -32766, w 0). */
+int mcx_volume_init(void* d_vol, int X, int Y, int Z, void* stream);
+
+/* ---- projective integration (replaces _cu_voxel_volumetric_integration,
+ *      cu_kinect_fusion_v1.cu:534-549, and its kernels :416-532).
+ * d_vol: packed short2 volume, voxel i at ((i+0.5)h) per axis.
+ * intr (host): {fx, fy, cx, cy}.  d_poses: K x 12 floats, camera-to-world
+ * {R row-major, t}.  d_depth: K x H x W uint16, metres = value * depth_scale,
+ * 0 = no measurement.  Frames are fused in order, all K in one sweep of the
+ * volume.  Workspace: mcx_integrate_workspace_size(W, H) bytes (per-pixel ray
+ * factors, recomputed by every call). */
+size_t mcx_integrate_workspace_size(int W, int H);
+int mcx_integrate(void* d_vol, int X, int Y, int Z, float h, float trunc, int max_w,
+                  const float* intr, const float* d_poses, const uint16_t* d_depth,
+                  float depth_scale, int W, int H, int K, void* d_ws, size_t ws_bytes,
+                  void* stream);
+
+/* ---- synthetic depth frames (test/benchmark input; no reference
+ *      counterpart: the reference reads OpenNI frames).  Ray-casts nobj
+ *      objects (8 floats each: kind 0 sphere {c, r} / 1 box {c, half sizes})
+ *      from each pose; z-depth plus Gaussian noise (sigma, metres), rounded
+ *      to depth_unit_m; a pixel is dropped (0) with probability dropout. */
+int mcx_synth_depth(uint16_t* d_depth, int W, int H, int K, const float* intr,
+                    const float* d_poses, const float* d_objs, int nobj, float noise_sigma_m,
+                    float dropout, uint32_t seed, float depth_unit_m, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MCX_H */

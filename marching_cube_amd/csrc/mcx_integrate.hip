@@ -1,0 +1,278 @@
+// mcx_integrate.hip -- TSDF volume init and projective integration for gfx950.
+//
+// Replaces the reference's dead KinectFusion kernels
+//   d_voxel_initialize / cu_voxel_new      cu_kinect_fusion_v1.cu:364-398
+//   d_voxel_volumetric_depth_scaling       :416-425
+//   d_voxel_volumetric_integration         :427-532
+//   _cu_voxel_volumetric_integration       :534-549
+// The reference gives each thread one (y, z) row and walks x inside the
+// thread, so a wave strides the volume by 4*X bytes (uncoalesced), and it
+// sweeps the whole volume once per frame.  Here a thread owns one voxel, a
+// workgroup owns 256 consecutive voxels of a z-slab (x fastest, coalesced
+// 1 KiB per workgroup), and K frames are fused per sweep: the voxel is read
+// once, updated for every frame in order in registers, and written once, so a
+// sweep costs 8 B/voxel plus the depth reads instead of 8 B/voxel/frame.
+//
+// Per voxel and frame (bit-for-bit the oracle's update_voxel):
+//   p = ((i + 0.5) h) per axis; q = R^T (p - t)          (camera-to-world pose)
+//   skip q.z < FLT_MIN; u = rint(fx q.x / q.z + cx), v likewise (round half even)
+//   skip u,v outside [0, W-2) x [0, H-2)                 (reference :449)
+//   zval = depth * scale * sqrt(((u-cx)/fx)^2 + ((v-cy)/fy)^2 + 1)  (ray distance)
+//   skip zval <= FLT_MIN (dropout); sdf = zval - |q|; skip sdf < -trunc
+//   tsdf = clamp(sdf / trunc, -1, 1); running average with integer weights
+//   capped at max_w; tsdf stored as trunc(t * 32767) clamped to +-32767.
+#include <float.h>
+#include <hip/hip_runtime.h>
+
+#include "mcx_internal.h"
+
+namespace mcx {
+namespace {
+
+constexpr int kFramesPerPass = 512;  // poses staged in LDS per launch
+
+__global__ __launch_bounds__(256) void init_kernel(uint32_t* vol, int64_t n) {
+  const uint32_t v = (uint32_t)(uint16_t)(int16_t)-32766;  // (-CU_VOXEL_MAX_TSDF, 0)
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) vol[i] = v;
+}
+
+// Per-pixel ray-length factor sqrt(((u-cx)/fx)^2 + ((v-cy)/fy)^2 + 1)
+// (d_voxel_volumetric_depth_scaling, :416-425), shared by all frames.
+__global__ __launch_bounds__(256) void rayfac_kernel(float* out, int W, int H, float fx, float fy,
+                                                      float cx, float cy) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= W * H) return;
+  const int u = i % W, v = i / W;
+  const float ax = ((float)u - cx) / fx, ay = ((float)v - cy) / fy;
+  out[i] = sqrtf(__fmaf_rn(ay, ay, ax * ax) + 1.0f);
+}
+
+struct IntegrateParams {
+  uint32_t* vol;
+  int X, Y, Z;
+  float h, trunc;
+  int max_w;
+  float fx, fy, cx, cy;
+  const float* poses;      // [K][12] device
+  const uint16_t* depth;   // [K][H][W] device
+  const float* rayfac;     // [H][W]
+  float depth_scale;
+  int W, H, K;
+  FastDiv divX, divY;
+};
+
+__global__ __launch_bounds__(256) void integrate_kernel(IntegrateParams P) {
+  __shared__ float s_pose[kFramesPerPass * 12];
+  for (int i = threadIdx.x; i < P.K * 12; i += blockDim.x) s_pose[i] = P.poses[i];
+  __syncthreads();
+  const int64_t n = (int64_t)P.X * P.Y * P.Z;
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t xy = (uint32_t)((int64_t)P.X * P.Y);
+  const int z = (int)(i / xy);
+  const uint32_t rem = (uint32_t)(i - (int64_t)z * xy);
+  const uint32_t y = P.divX.div(rem);
+  const int x = (int)(rem - y * (uint32_t)P.X);
+  const float px = ((float)x + 0.5f) * P.h;
+  const float py = ((float)(int)y + 0.5f) * P.h;
+  const float pz = ((float)z + 0.5f) * P.h;
+  uint32_t cur = P.vol[i];
+  const int64_t frame = (int64_t)P.W * P.H;
+  for (int k = 0; k < P.K; ++k) {
+    const float* R = s_pose + 12 * k;
+    const float dx = px - R[9], dy = py - R[10], dz = pz - R[11];
+    const float qx = __fmaf_rn(R[6], dz, __fmaf_rn(R[3], dy, R[0] * dx));
+    const float qy = __fmaf_rn(R[7], dz, __fmaf_rn(R[4], dy, R[1] * dx));
+    const float qz = __fmaf_rn(R[8], dz, __fmaf_rn(R[5], dy, R[2] * dx));
+    if (qz < FLT_MIN) continue;
+    const float u = __fmaf_rn(P.fx, qx / qz, P.cx);
+    const float v = __fmaf_rn(P.fy, qy / qz, P.cy);
+    if (!(fabsf(u) < 1e9f) || !(fabsf(v) < 1e9f)) continue;
+    const int ui = (int)rintf(u), vi = (int)rintf(v);
+    if (ui < 0 || vi < 0 || ui >= P.W - 2 || vi >= P.H - 2) continue;
+    const int pix = vi * P.W + ui;
+    const uint16_t dm = P.depth[(int64_t)k * frame + pix];
+    const float d = (float)dm * P.depth_scale;
+    const float zval = d * P.rayfac[pix];
+    if (!(zval > FLT_MIN)) continue;
+    const float qn = sqrtf(__fmaf_rn(qz, qz, __fmaf_rn(qy, qy, qx * qx)));
+    const float sdf = zval - qn;
+    if (!(sdf >= -P.trunc)) continue;
+    float tsdf = sdf / P.trunc;
+    tsdf = fminf(fmaxf(tsdf, -1.0f), 1.0f);
+    const int16_t ti = (int16_t)(cur & 0xffffu);
+    const int wi = (int)(cur >> 16);
+    const float told = (float)ti / 32767.0f;
+    const float tnew = __fmaf_rn(told, (float)wi, tsdf) / (float)(wi + 1);
+    const int wnew = wi + 1 < P.max_w ? wi + 1 : P.max_w;
+    int fixedp = (int)(tnew * 32767.0f);
+    fixedp = fixedp > 32767 ? 32767 : (fixedp < -32767 ? -32767 : fixedp);
+    cur = (uint32_t)(uint16_t)(int16_t)fixedp | ((uint32_t)wnew << 16);
+  }
+  P.vol[i] = cur;
+}
+
+// ---- synthetic depth frames: ray-cast spheres and axis-aligned boxes -------
+__device__ __forceinline__ uint32_t hash32(uint32_t x) {
+  x ^= x >> 16;
+  x *= 0x7feb352dU;
+  x ^= x >> 15;
+  x *= 0x846ca68bU;
+  x ^= x >> 16;
+  return x;
+}
+__device__ __forceinline__ float u01(uint32_t h) { return ((h >> 8) + 0.5f) * (1.0f / 16777216.0f); }
+
+__global__ __launch_bounds__(256) void synth_kernel(uint16_t* depth, int W, int H, int K, float fx,
+                                                     float fy, float cx, float cy,
+                                                     const float* poses, const float* objs,
+                                                     int nobj, float sigma_m, float dropout,
+                                                     uint32_t seed, float depth_unit) {
+  const int64_t n = (int64_t)W * H * K;
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int k = (int)(i / ((int64_t)W * H));
+  const int pix = (int)(i - (int64_t)k * W * H);
+  const int u = pix % W, v = pix / W;
+  const float* R = poses + 12 * k;
+  const float a = ((float)u - cx) / fx, b = ((float)v - cy) / fy;
+  // world direction of the pixel ray with unit camera-z component
+  const float dx = R[0] * a + R[1] * b + R[2];
+  const float dy = R[3] * a + R[4] * b + R[5];
+  const float dz = R[6] * a + R[7] * b + R[8];
+  const float ox = R[9], oy = R[10], oz = R[11];
+  float best = 1e30f;
+  for (int j = 0; j < nobj; ++j) {
+    const float* o = objs + 8 * j;
+    if (o[0] < 0.5f) {  // sphere
+      const float lx = ox - o[1], ly = oy - o[2], lz = oz - o[3];
+      const float A = dx * dx + dy * dy + dz * dz;
+      const float B = lx * dx + ly * dy + lz * dz;
+      const float C = lx * lx + ly * ly + lz * lz - o[4] * o[4];
+      const float disc = B * B - A * C;
+      if (disc >= 0.f) {
+        const float sq = sqrtf(disc);
+        float s = (-B - sq) / A;
+        if (s <= 1e-4f) s = (-B + sq) / A;
+        if (s > 1e-4f && s < best) best = s;
+      }
+    } else {  // axis-aligned box
+      float tmin = -1e30f, tmax = 1e30f;
+      const float org[3] = {ox, oy, oz}, dir[3] = {dx, dy, dz};
+      bool miss = false;
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        const float lo = o[1 + c] - o[4 + c], hi = o[1 + c] + o[4 + c];
+        if (fabsf(dir[c]) < 1e-12f) {
+          if (org[c] < lo || org[c] > hi) miss = true;
+        } else {
+          float t0 = (lo - org[c]) / dir[c], t1 = (hi - org[c]) / dir[c];
+          if (t0 > t1) {
+            const float tt = t0;
+            t0 = t1;
+            t1 = tt;
+          }
+          tmin = fmaxf(tmin, t0);
+          tmax = fminf(tmax, t1);
+        }
+      }
+      if (!miss && tmax >= tmin) {
+        const float s = tmin > 1e-4f ? tmin : tmax;
+        if (s > 1e-4f && s < best) best = s;
+      }
+    }
+  }
+  uint16_t out = 0;
+  if (best < 1e29f) {
+    const uint32_t h0 = hash32(seed ^ hash32((uint32_t)i * 2u + 1u));
+    const uint32_t h1 = hash32(h0 + 0x9e3779b9u);
+    const uint32_t h2 = hash32(h1 + 0x9e3779b9u);
+    const float r = sqrtf(-2.0f * logf(u01(h0)));
+    const float gauss = r * cosf(6.28318530718f * u01(h1));
+    const float mm = rintf((best + sigma_m * gauss) / depth_unit);
+    if (u01(h2) >= dropout && mm >= 1.f) out = (uint16_t)fminf(mm, 65535.f);
+  }
+  depth[i] = out;
+}
+
+unsigned blocks_for(int64_t n) { return (unsigned)((n + 255) / 256); }
+
+}  // namespace
+}  // namespace mcx
+
+using namespace mcx;
+
+extern "C" int mcx_volume_init(void* d_vol, int X, int Y, int Z, void* stream) {
+  if (!d_vol || X < 1 || Y < 1 || Z < 1) return MCX_ERR_ARG;
+  const int64_t n = (int64_t)X * Y * Z;
+  int64_t b = (n + 255) / 256;
+  if (b > 65536) b = 65536;
+  hipLaunchKernelGGL(init_kernel, dim3((unsigned)b), dim3(256), 0, as_stream(stream),
+                     static_cast<uint32_t*>(d_vol), n);
+  MCX_HIP(hipGetLastError());
+  return MCX_OK;
+}
+
+extern "C" size_t mcx_integrate_workspace_size(int W, int H) {
+  return (size_t)(W > 0 ? W : 0) * (size_t)(H > 0 ? H : 0) * sizeof(float);
+}
+
+extern "C" int mcx_integrate(void* d_vol, int X, int Y, int Z, float h, float trunc, int max_w,
+                             const float* intr, const float* d_poses, const uint16_t* d_depth,
+                             float depth_scale, int W, int H, int K, void* d_ws, size_t ws_bytes,
+                             void* stream) {
+  if (!d_vol || !intr || X < 1 || Y < 1 || Z < 1 || W < 3 || H < 3 || K < 0) return MCX_ERR_ARG;
+  if (!(h > 0.f) || !(trunc > 0.f) || max_w < 1 || max_w > 65535) return MCX_ERR_ARG;
+  if (K == 0) return MCX_OK;
+  if (!d_poses || !d_depth) return MCX_ERR_ARG;
+  if ((int64_t)X * Y > 0xffffffffll) return MCX_ERR_UNSUPPORTED;
+  if (!d_ws || ws_bytes < mcx_integrate_workspace_size(W, H)) return MCX_ERR_WORKSPACE;
+  hipStream_t s = as_stream(stream);
+  float* rayfac = static_cast<float*>(d_ws);
+  hipLaunchKernelGGL(rayfac_kernel, dim3(blocks_for((int64_t)W * H)), dim3(256), 0, s, rayfac, W,
+                     H, intr[0], intr[1], intr[2], intr[3]);
+  MCX_HIP(hipGetLastError());
+  IntegrateParams P;
+  P.vol = static_cast<uint32_t*>(d_vol);
+  P.X = X;
+  P.Y = Y;
+  P.Z = Z;
+  P.h = h;
+  P.trunc = trunc;
+  P.max_w = max_w;
+  P.fx = intr[0];
+  P.fy = intr[1];
+  P.cx = intr[2];
+  P.cy = intr[3];
+  P.rayfac = rayfac;
+  P.depth_scale = depth_scale;
+  P.W = W;
+  P.H = H;
+  P.divX = FastDiv((uint32_t)X);
+  P.divY = FastDiv((uint32_t)Y);
+  const int64_t n = (int64_t)X * Y * Z;
+  for (int k0 = 0; k0 < K; k0 += kFramesPerPass) {
+    P.K = K - k0 < kFramesPerPass ? K - k0 : kFramesPerPass;
+    P.poses = d_poses + (int64_t)12 * k0;
+    P.depth = d_depth + (int64_t)k0 * W * H;
+    hipLaunchKernelGGL(integrate_kernel, dim3(blocks_for(n)), dim3(256), 0, s, P);
+    MCX_HIP(hipGetLastError());
+  }
+  return MCX_OK;
+}
+
+extern "C" int mcx_synth_depth(uint16_t* d_depth, int W, int H, int K, const float* intr,
+                               const float* d_poses, const float* d_objs, int nobj,
+                               float noise_sigma_m, float dropout, uint32_t seed,
+                               float depth_unit_m, void* stream) {
+  if (!d_depth || !intr || !d_poses || (nobj > 0 && !d_objs) || W < 1 || H < 1 || K < 1 ||
+      !(depth_unit_m > 0.f))
+    return MCX_ERR_ARG;
+  const int64_t n = (int64_t)W * H * K;
+  hipLaunchKernelGGL(synth_kernel, dim3(blocks_for(n)), dim3(256), 0, as_stream(stream), d_depth,
+                     W, H, K, intr[0], intr[1], intr[2], intr[3], d_poses, d_objs, nobj,
+                     noise_sigma_m, dropout, seed, depth_unit_m);
+  MCX_HIP(hipGetLastError());
+  return MCX_OK;
+}

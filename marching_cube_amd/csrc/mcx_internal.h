@@ -1,0 +1,167 @@
+// mcx_internal.h -- shared device helpers and host plumbing for libmcx.
+//
+// gfx950 only (wave64).  Nothing here keeps device globals that hold state:
+// look-back descriptors, tickets and counters live in caller workspace memory.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "mcx.h"
+
+namespace mcx {
+
+constexpr int kWave = 64;
+
+// ---------------------------------------------------------------- host side
+// Map a HIP error to a status code; keep the first HIP error for strerror.
+int hip_status(hipError_t e);
+void set_last_hip_error(hipError_t e);
+
+#define MCX_HIP(expr)                               \
+  do {                                              \
+    hipError_t _e = (expr);                         \
+    if (_e != hipSuccess) return ::mcx::hip_status(_e); \
+  } while (0)
+
+inline hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
+
+// Stream-ordered byte fill (a kernel, so it is safe inside captured graphs).
+int fill_u32(void* d_ptr, uint32_t value, size_t n_words, hipStream_t s);
+int fill_bytes_zero(void* d_ptr, size_t n_bytes, hipStream_t s);
+
+// ------------------------------------------------------ workspace layout
+// [0, 256)   header: u32 ticket, u32 done, u32 parity, u32 error, i64 count
+// [256, ...) u64 descriptors, two arrays of cap_tiles entries (by parity)
+constexpr size_t kWsHeader = 256;
+struct WsHeader {
+  uint32_t ticket;
+  uint32_t done;
+  uint32_t parity;
+  uint32_t error;
+  int64_t count;
+};
+inline size_t ws_bytes_for_tiles(int64_t tiles) {
+  return kWsHeader + 2 * sizeof(uint64_t) * (size_t)(tiles > 0 ? tiles : 1);
+}
+inline int64_t ws_cap_tiles(size_t bytes) {
+  return bytes < kWsHeader + 16 ? 0 : (int64_t)((bytes - kWsHeader) / 16);
+}
+
+// Look-back descriptor: status in the top two bits, value below.
+constexpr uint64_t kStatAgg = 1ull << 62;
+constexpr uint64_t kStatInc = 2ull << 62;
+constexpr uint64_t kValMask = (1ull << 62) - 1;
+constexpr int kSpinLimit = 1 << 22;
+
+// ------------------------------------------------- division by invariants
+// n / d for 32-bit unsigned n via one mulhi (Granlund-Montgomery).
+struct FastDiv {
+  uint32_t d, m;
+  int s;  // shift
+  FastDiv() = default;
+  __host__ __device__ explicit FastDiv(uint32_t div) {
+    d = div;
+    if (div <= 1) {
+      m = 0;
+      s = 0;
+      return;
+    }
+    int l = 0;
+    while ((1ull << l) < div) ++l;
+    m = (uint32_t)(((1ull << 32) * ((1ull << l) - div)) / div + 1);
+    s = l;
+  }
+  __host__ __device__ __forceinline__ uint32_t div(uint32_t n) const {
+    if (d <= 1) return n;
+    uint32_t t = (uint32_t)(((uint64_t)m * n) >> 32);
+    return (t + ((n - t) >> 1)) >> (s - 1);
+  }
+};
+
+// ----------------------------------------------------------- wave helpers
+__device__ __forceinline__ int lane_id() { return __lane_id(); }
+
+__device__ __forceinline__ uint64_t shfl_up_u64(uint64_t v, int d) {
+  uint32_t lo = __shfl_up((uint32_t)v, d, kWave);
+  uint32_t hi = __shfl_up((uint32_t)(v >> 32), d, kWave);
+  return ((uint64_t)hi << 32) | lo;
+}
+__device__ __forceinline__ uint64_t shfl_xor_u64(uint64_t v, int m) {
+  uint32_t lo = __shfl_xor((uint32_t)v, m, kWave);
+  uint32_t hi = __shfl_xor((uint32_t)(v >> 32), m, kWave);
+  return ((uint64_t)hi << 32) | lo;
+}
+
+// Inclusive prefix sum over the 64 lanes of a wave.
+__device__ __forceinline__ uint64_t wave_incl_scan(uint64_t v) {
+  const int lane = lane_id();
+#pragma unroll
+  for (int d = 1; d < kWave; d <<= 1) {
+    uint64_t n = shfl_up_u64(v, d);
+    if (lane >= d) v += n;
+  }
+  return v;
+}
+__device__ __forceinline__ uint32_t wave_incl_scan32(uint32_t v) {
+  const int lane = lane_id();
+#pragma unroll
+  for (int d = 1; d < kWave; d <<= 1) {
+    uint32_t n = __shfl_up(v, d, kWave);
+    if (lane >= d) v += n;
+  }
+  return v;
+}
+__device__ __forceinline__ uint64_t wave_sum(uint64_t v) {
+#pragma unroll
+  for (int m = 1; m < kWave; m <<= 1) v += shfl_xor_u64(v, m);
+  return v;
+}
+
+__device__ __forceinline__ uint64_t ld_agent(const uint64_t* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_agent(uint64_t* p, uint64_t v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Decoupled look-back executed by one full wave for tile t (t > 0):
+// returns the sum of all tiles before t.  Predecessors publish their
+// aggregate before they look back themselves, so every wait ends; a wait
+// beyond kSpinLimit sets *err and counts the tile as 0 so the wave exits.
+__device__ __forceinline__ int64_t lookback(const uint64_t* desc, int64_t t,
+                                            uint32_t* err) {
+  const int lane = lane_id();
+  int64_t acc = 0;
+  int64_t j = t - 1;
+  while (true) {
+    const int64_t idx = j - lane;
+    uint64_t d = kStatInc;  // virtual inclusive 0 before tile 0
+    if (idx >= 0) {
+      int spins = 0;
+      d = ld_agent(desc + idx);
+      while ((d >> 62) == 0) {
+        if (++spins >= kSpinLimit) {
+          atomicOr(err, 1u);
+          d = kStatInc;
+          break;
+        }
+        __builtin_amdgcn_s_sleep(2);
+        d = ld_agent(desc + idx);
+      }
+    }
+    const uint64_t incmask = __ballot((d >> 62) == 2);
+    uint64_t v = d & kValMask;
+    if (incmask) {
+      const int first = __ffsll((unsigned long long)incmask) - 1;
+      if (lane > first) v = 0;
+      acc += (int64_t)wave_sum(v);
+      break;
+    }
+    acc += (int64_t)wave_sum(v);
+    j -= kWave;
+  }
+  return acc;
+}
+
+}  // namespace mcx

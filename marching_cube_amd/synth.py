@@ -1,0 +1,186 @@
+"""Synthetic TSDF inputs of the benchmark shapes (numpy, host side).
+
+These generate the inputs the extraction and integration entry points are
+measured and tested on; nothing here is on the device path.
+
+* sphere_volume   -- 128^3 short2 TSDF of a sphere r=0.3 m in a 1 m cube,
+                     trunc = 3 voxels, w=1 inside the band, (+32767, 0) outside.
+* noise_volume    -- dense random short2 volume (every cube case appears).
+* bumps_volume    -- fp32 field: sum of seeded Gaussian bumps + small noise.
+* batch_shapes    -- uneven short2 volumes holding a sphere / box / torus SDF
+                     at a random pose (sides log-uniform per axis).
+* kinfu_scene / orbit_poses -- seeded spheres/boxes scene and an orbiting
+                     camera for the depth-frame synthesis kernel.
+
+Packing of a short2 voxel into one uint32: low 16 bits = int16 tsdf, high 16
+bits = uint16 weight (cu_voxel_t, structure.h:14-18).
+"""
+import math
+
+import numpy as np
+
+TSDF_SCALE = 32767.0
+
+
+def pack_short2(tsdf, w):
+    t = np.asarray(tsdf).astype(np.int16).view(np.uint16).astype(np.uint32)
+    return t | (np.asarray(w).astype(np.uint32) << 16)
+
+
+def unpack_short2(vol):
+    v = np.asarray(vol, dtype=np.uint32)
+    return (v & 0xFFFF).astype(np.uint16).view(np.int16), (v >> 16).astype(np.uint16)
+
+
+def _node_coords(n, h):
+    return (np.arange(n, dtype=np.float64) + 0.5) * h
+
+
+def sdf_to_short2(sdf, trunc):
+    """Band |sdf| < trunc: w=1, tsdf = lrint(clamp(sdf/trunc,-1,1)*32767) in
+    FP64; outside: tsdf=+32767, w=0."""
+    band = np.abs(sdf) < trunc
+    t = np.rint(np.clip(sdf / trunc, -1.0, 1.0) * TSDF_SCALE)
+    tsdf = np.where(band, t, TSDF_SCALE).astype(np.int16)
+    w = band.astype(np.uint16)
+    return pack_short2(tsdf, w)
+
+
+def sphere_volume(n=128, radius=0.3, extent=1.0, trunc_vox=3.0, center=None):
+    h = extent / n
+    c = (0.5 * extent,) * 3 if center is None else center
+    x = _node_coords(n, h)
+    zz, yy, xx = np.meshgrid(x, x, x, indexing="ij")
+    sdf = np.sqrt((xx - c[0]) ** 2 + (yy - c[1]) ** 2 + (zz - c[2]) ** 2) - radius
+    return sdf_to_short2(sdf, trunc_vox * h), h
+
+
+def noise_volume(shape, seed=0, p_zero_w=0.0, max_w=128):
+    """Dense random short2 volume; tsdf uniform in [-32767, 32767]."""
+    rng = np.random.default_rng(seed)
+    Z, Y, X = shape
+    t = rng.integers(-32767, 32768, size=(Z, Y, X), dtype=np.int32)
+    w = rng.integers(1, max_w + 1, size=(Z, Y, X), dtype=np.int32)
+    if p_zero_w > 0:
+        w[rng.random((Z, Y, X)) < p_zero_w] = 0
+    return pack_short2(t, w)
+
+
+def bumps_volume(n=256, nbumps=64, seed=20261202, noise_vox=0.1, dtype=np.float32):
+    """fp32 implicit field on [0,1]^3 (h = 1/n): f = 0.5 - sum_k a_k exp(-|p-c_k|^2/(2 s_k^2))
+    plus band-limited noise of amplitude noise_vox voxels (in distance units)."""
+    rng = np.random.default_rng(seed)
+    h = 1.0 / n
+    x = _node_coords(n, h).astype(np.float32)
+    c = rng.uniform(0.2, 0.8, size=(nbumps, 3)).astype(np.float32)
+    s = rng.uniform(0.05, 0.12, size=nbumps).astype(np.float32)
+    a = rng.uniform(0.4, 1.0, size=nbumps).astype(np.float32)
+    f = np.full((n, n, n), 0.5, np.float32)
+    for k in range(nbumps):
+        gx = np.exp(-((x - c[k, 0]) ** 2) / (2 * s[k] ** 2))
+        gy = np.exp(-((x - c[k, 1]) ** 2) / (2 * s[k] ** 2))
+        gz = np.exp(-((x - c[k, 2]) ** 2) / (2 * s[k] ** 2))
+        f -= a[k] * gz[:, None, None] * gy[None, :, None] * gx[None, None, :]
+    # band-limited noise: a few low-frequency cosines, amplitude noise_vox*h
+    k = rng.integers(1, 6, size=(8, 3))
+    ph = rng.uniform(0, 2 * math.pi, size=8)
+    noise = np.zeros_like(f)
+    for i in range(8):
+        noise += (np.cos(2 * math.pi * k[i, 2] * x + ph[i])[:, None, None]
+                  * np.cos(2 * math.pi * k[i, 1] * x)[None, :, None]
+                  * np.cos(2 * math.pi * k[i, 0] * x)[None, None, :])
+    f += (noise_vox * h / 8.0) * noise
+    return f.astype(dtype), h
+
+
+def _random_rotation(rng):
+    q = rng.normal(size=4)
+    q /= np.linalg.norm(q)
+    w, x, y, z = q
+    return np.array([[1 - 2 * (y * y + z * z), 2 * (x * y - z * w), 2 * (x * z + y * w)],
+                     [2 * (x * y + z * w), 1 - 2 * (x * x + z * z), 2 * (y * z - x * w)],
+                     [2 * (x * z - y * w), 2 * (y * z + x * w), 1 - 2 * (x * x + y * y)]])
+
+
+def shape_volume(dims, kind, rng, trunc_vox=3.0):
+    """One (Z,Y,X) short2 volume with h = 1/max(dims) holding a shape SDF."""
+    Z, Y, X = dims
+    h = 1.0 / max(dims)
+    ext = np.array([X, Y, Z], np.float64) * h
+    zz, yy, xx = np.meshgrid(_node_coords(Z, h), _node_coords(Y, h), _node_coords(X, h),
+                             indexing="ij")
+    c = ext * rng.uniform(0.35, 0.65, size=3)
+    R = _random_rotation(rng)
+    p = np.stack([xx - c[0], yy - c[1], zz - c[2]], axis=-1) @ R  # into shape frame
+    m = 0.5 * ext.min()
+    if kind == "sphere":
+        sdf = np.linalg.norm(p, axis=-1) - m * rng.uniform(0.4, 0.8)
+    elif kind == "box":
+        b = m * rng.uniform(0.3, 0.7, size=3)
+        q = np.abs(p) - b
+        sdf = np.linalg.norm(np.maximum(q, 0), axis=-1) + np.minimum(q.max(axis=-1), 0)
+    else:  # torus
+        R0 = m * rng.uniform(0.4, 0.6)
+        r0 = R0 * rng.uniform(0.25, 0.5)
+        qx = np.sqrt(p[..., 0] ** 2 + p[..., 1] ** 2) - R0
+        sdf = np.sqrt(qx ** 2 + p[..., 2] ** 2) - r0
+    return sdf_to_short2(sdf, trunc_vox * h), h
+
+
+def batch_dims(count, lo=16, hi=128, seed=20261202):
+    rng = np.random.default_rng(seed)
+    d = np.exp(rng.uniform(math.log(lo), math.log(hi), size=(count, 3)))
+    return np.clip(np.rint(d), lo, hi).astype(np.int64)  # rows are (Z, Y, X)
+
+
+def batch_shapes(count, lo=16, hi=128, seed=20261202):
+    dims = batch_dims(count, lo, hi, seed)
+    rng = np.random.default_rng(seed + 1)
+    kinds = ("sphere", "box", "torus")
+    vols, hs = [], []
+    for i in range(count):
+        v, h = shape_volume(tuple(int(a) for a in dims[i]), kinds[rng.integers(0, 3)], rng)
+        vols.append(v)
+        hs.append(h)
+    return vols, np.array(hs, np.float32)
+
+
+def kinfu_scene(nobj=40, extent=3.0, seed=20261202):
+    """Scene of random spheres and axis-aligned boxes inside [0, extent]^3.
+    Returns float32 array (nobj, 8): kind (0 sphere, 1 box), cx, cy, cz,
+    r (sphere radius) or half sizes hx, hy, hz (box), padding."""
+    rng = np.random.default_rng(seed)
+    objs = np.zeros((nobj, 8), np.float32)
+    for i in range(nobj):
+        kind = rng.integers(0, 2)
+        c = rng.uniform(0.6, extent - 0.6, size=3)
+        objs[i, 0] = kind
+        objs[i, 1:4] = c
+        if kind == 0:
+            objs[i, 4] = rng.uniform(0.08, 0.3)
+        else:
+            objs[i, 4:7] = rng.uniform(0.06, 0.25, size=3)
+    return objs
+
+
+def orbit_poses(nframes, extent=3.0, radius=2.6, seed=20261202):
+    """Camera-to-world poses (nframes, 12): R row-major (columns = camera x, y,
+    z axes in world) and t, orbiting the cube centre and looking at it."""
+    rng = np.random.default_rng(seed + 7)
+    c = np.full(3, extent / 2.0)
+    poses = np.zeros((nframes, 12), np.float32)
+    for k in range(nframes):
+        th = 2 * math.pi * k / max(nframes, 1)
+        el = 0.35 * math.sin(3 * th) + rng.normal(0, 0.01)
+        eye = c + radius * np.array([math.cos(th) * math.cos(el), math.sin(el),
+                                     math.sin(th) * math.cos(el)])
+        fwd = c - eye
+        fwd /= np.linalg.norm(fwd)
+        up = np.array([0.0, 1.0, 0.0])
+        right = np.cross(fwd, up)
+        right /= np.linalg.norm(right)
+        down = np.cross(fwd, right)
+        R = np.stack([right, down, fwd], axis=1)  # columns: camera x, y, z in world
+        poses[k, :9] = R.reshape(-1)
+        poses[k, 9:] = eye
+    return poses

@@ -1,0 +1,67 @@
+"""C-ABI surface checks that need no GPU: the library loads, exports every
+function include/mcx.h declares, and rejects bad arguments before touching
+the device."""
+import ctypes
+import subprocess
+
+import pytest
+
+from marching_cube_amd import _capi
+
+
+def test_library_exports_every_declared_function():
+    lib = _capi.load()
+    declared = _capi.declared_functions()
+    assert "mcx_extract" in declared
+    missing = [n for n in declared if not hasattr(lib, n)]
+    assert not missing, missing
+    # the Python binding knows the signature of every declared function
+    assert sorted(_capi.SIGNATURES) == declared
+
+
+def test_exported_symbols_are_c_linkage():
+    out = subprocess.run(["nm", "-D", "--defined-only", _capi.LIB_PATH], capture_output=True,
+                         text=True, check=True).stdout
+    names = {line.split()[-1] for line in out.splitlines() if line.strip()}
+    for n in _capi.declared_functions():
+        assert n in names, n
+
+
+def test_version_and_strerror():
+    lib = _capi.load()
+    assert lib.mcx_version() >= 10000
+    for st in range(7):
+        assert lib.mcx_strerror(st)
+    assert b"capacity" in lib.mcx_strerror(_capi.ERR_CAPACITY)
+
+
+def test_argument_errors_without_device():
+    lib = _capi.load()
+    hc = ctypes.c_int64(0)
+    # null volume, bad dims, bad kind, f32 with vertex_voxel output
+    assert lib.mcx_extract(None, 0, 4, 4, 4, 1.0, None, None, None, None, 0, None,
+                           ctypes.byref(hc), None, 0, None) == _capi.ERR_ARG
+    fake = ctypes.c_void_p(0x1000)
+    assert lib.mcx_extract(fake, 0, 0, 4, 4, 1.0, None, None, None, None, 0, None,
+                           None, None, 0, None) == _capi.ERR_ARG
+    assert lib.mcx_extract(fake, 7, 4, 4, 4, 1.0, None, None, None, None, 0, None,
+                           None, None, 0, None) == _capi.ERR_ARG
+    assert lib.mcx_extract(fake, 1, 4, 4, 4, 1.0, fake, None, None, None, 10, None,
+                           None, None, 0, None) == _capi.ERR_ARG
+    assert lib.mcx_classify(None, 0, 4, 4, 4, None, None, None) == _capi.ERR_ARG
+
+
+def test_workspace_size_grows_with_volume():
+    lib = _capi.load()
+    a = lib.mcx_extract_workspace_size(16, 16, 16)
+    b = lib.mcx_extract_workspace_size(512, 512, 512)
+    assert 256 < a < b
+    # one u64 descriptor per tile and parity
+    assert b >= 256 + 16 * (511 * 512 * 512 // 1024)
+
+
+def test_missing_library_fails_loudly(monkeypatch, tmp_path):
+    monkeypatch.setattr(_capi, "_lib", None)
+    monkeypatch.setattr(_capi, "LIB_PATH", str(tmp_path / "libmcx.so"))
+    with pytest.raises(RuntimeError, match="no CPU fallback"):
+        _capi.load()

@@ -1,0 +1,204 @@
+"""Pin the CPU oracle (oracle/mc_oracle.c) with checks that do not restate
+its formulas: brute force over all cube cases, closed forms, mesh invariants,
+an FP64 cross-check and finite differences."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from marching_cube_amd import synth
+from oracle import oracle as O
+
+T = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "mc_tables.json")))
+CX = np.array([0, 1, 1, 0, 0, 1, 1, 0])
+CY = np.array([0, 0, 1, 1, 0, 0, 1, 1])
+CZ = np.array([0, 0, 0, 0, 1, 1, 1, 1])
+EA = np.array([0, 1, 2, 3, 4, 5, 6, 7, 0, 1, 2, 3])
+EB = np.array([1, 2, 3, 0, 5, 6, 7, 4, 4, 5, 6, 7])
+
+
+def cube_volume(values, weights=None):
+    """2x2x2 short2 volume with corner k holding values[k] (reference order)."""
+    t = np.zeros((2, 2, 2), np.int32)
+    w = np.ones((2, 2, 2), np.int32)
+    for k in range(8):
+        t[CZ[k], CY[k], CX[k]] = values[k]
+        if weights is not None:
+            w[CZ[k], CY[k], CX[k]] = weights[k]
+    return synth.pack_short2(t, w)
+
+
+def test_all_256_cases_classify():
+    nvt = T["numVertsTable"]
+    for c in range(256):
+        vals = [-100 if (c >> k) & 1 else 100 for k in range(8)]
+        ci, nv, act = O.classify(cube_volume(vals))
+        assert ci[0, 0, 0] == c
+        assert nv[0, 0, 0] == (0 if c in (0, 255) else nvt[c])
+        # nothing is reported for the non-existent cubes of the last planes
+        assert ci.sum() == c and act == (nv[0, 0, 0] > 0)
+
+
+def test_zero_weight_kills_cube():
+    for c in (1, 37, 128, 254):
+        vals = [-100 if (c >> k) & 1 else 100 for k in range(8)]
+        for k in range(8):
+            w = [1] * 8
+            w[k] = 0
+            ci, nv, _ = O.classify(cube_volume(vals, w))
+            assert ci[0, 0, 0] == 0 and nv[0, 0, 0] == 0
+
+
+def test_zero_tsdf_is_outside():
+    vals = [0, 5, 5, 5, -5, 5, 5, 5]  # f0 == 0 is not below the iso value
+    ci, _, _ = O.classify(cube_volume(vals))
+    assert ci[0, 0, 0] == 16
+
+
+@pytest.mark.parametrize("f0,fo", [(-1000, 3000), (-30000, 12), (-7, 32767)])
+def test_single_corner_closed_form(f0, fo):
+    # case 1 = corner 0 inside -> triTable[1] = {0, 8, 3}, emitted reversed
+    vals = [f0] + [fo] * 7
+    h = 0.25
+    r = O.extract(cube_volume(vals), h, want=("vv", "pos"))
+    assert r["n"] == 3
+    p = r["pos"].astype(np.float64)
+    t = f0 / (f0 - fo)  # zero crossing measured from corner 0
+    base = np.array([0.5, 0.5, 0.5]) * h
+    expect = {0: base + [t * h, 0, 0], 8: base + [0, 0, t * h], 3: base + [0, t * h, 0]}
+    for slot, e in enumerate([3, 8, 0]):
+        np.testing.assert_allclose(p[slot], expect[e], rtol=0, atol=1e-6 * h)
+
+
+def test_planar_field_lies_on_plane():
+    n = 12
+    h = 1.0 / n
+    x = (np.arange(n) + 0.5) * h
+    c = 0.4321
+    f = np.broadcast_to((x - c)[None, None, :], (n, n, n)).astype(np.float32).copy()
+    r = O.extract(f, h, want=("pos",))
+    assert r["n"] == (n - 1) * (n - 1) * 6  # two triangles per crossed cube
+    np.testing.assert_allclose(r["pos"][:, 0], c, atol=2e-7)
+
+
+def weld_topological(vol, h):
+    X = vol.shape[2]
+    Y = vol.shape[1]
+    pos, cube, edge = O.extract_f64(vol, h)
+    x = cube % X
+    y = (cube // X) % Y
+    z = cube // (X * Y)
+    a, b = EA[edge], EB[edge]
+    va = ((z + CZ[a]) * Y + (y + CY[a])) * X + (x + CX[a])
+    vb = ((z + CZ[b]) * Y + (y + CY[b])) * X + (x + CX[b])
+    axis = np.where(CX[a] != CX[b], 0, np.where(CY[a] != CY[b], 1, 2))
+    key = np.minimum(va, vb) * 3 + axis
+    _, inv = np.unique(key, return_inverse=True)
+    return pos, inv.reshape(-1, 3)
+
+
+def test_sphere_closed_oriented_manifold():
+    vol, h = synth.sphere_volume(64)
+    pos, tris = weld_topological(vol, h)
+    nV = tris.max() + 1
+    D = np.concatenate([tris[:, [0, 1]], tris[:, [1, 2]], tris[:, [2, 0]]])
+    ud, dc = np.unique(D, axis=0, return_counts=True)
+    assert dc.max() == 1  # each directed edge once: consistent orientation
+    U = np.unique(np.sort(D, axis=1), axis=0, return_counts=True)[1]
+    assert (U == 2).all()  # closed: every edge shared by exactly two triangles
+    assert nV - len(U) + len(tris) == 2  # Euler characteristic of a sphere
+    # reversed winding makes the triangle normals point out of the sphere
+    v0, v1, v2 = pos[0::3], pos[1::3], pos[2::3]
+    nrm = np.cross(v1 - v0, v2 - v0)
+    outward = np.einsum("ij,ij->i", nrm, (v0 + v1 + v2) / 3 - 0.5)
+    assert (outward > 0).mean() > 0.99
+
+
+def test_sphere_distance_error_is_small():
+    vol, h = synth.sphere_volume(128)
+    r = O.extract(vol, h, want=("pos",))
+    d = np.linalg.norm(r["pos"].astype(np.float64) - 0.5, axis=1) - 0.3
+    assert np.abs(d).max() < 0.01 * h
+
+
+def test_fp32_matches_fp64_path():
+    vol = synth.noise_volume((9, 10, 11), seed=3, p_zero_w=0.1)
+    h = 0.01
+    r = O.extract(vol, h, want=("pos",))
+    p64, _, _ = O.extract_f64(vol, h)
+    assert r["n"] == len(p64)
+    err = np.abs(r["pos"].astype(np.float64) - p64).max() / h
+    assert err < 2e-6  # voxel units
+
+
+def test_record_fields_closed_form():
+    vol = synth.noise_volume((7, 8, 13), seed=5, p_zero_w=0.05)
+    h = np.float32(0.003)
+    r = O.extract(vol, float(h), want=("vv", "v2vox", "ver_voxel"))
+    vv = r["vv"]
+    p = vv["pos"]
+    vox = np.floor(p / h - np.float32(0.5)).astype(np.int32)
+    np.testing.assert_array_equal(vv["vox"], vox)
+    np.testing.assert_array_equal(r["v2vox"], vox)
+    t, w = synth.unpack_short2(vol)
+    Z, Y, X = vol.shape
+    inb = ((vox >= 0) & (vox < [X, Y, Z])).all(axis=1)
+    assert inb.all()
+    np.testing.assert_array_equal(vv["tsdf"], t[vox[:, 2], vox[:, 1], vox[:, 0]])
+    np.testing.assert_array_equal(vv["w"].view(np.uint16), w[vox[:, 2], vox[:, 1], vox[:, 0]])
+    ref = np.zeros((Z, Y, X), np.int16)
+    ref[vox[:, 2], vox[:, 1], vox[:, 0]] = t[vox[:, 2], vox[:, 1], vox[:, 0]]
+    np.testing.assert_array_equal(r["ver_voxel"], ref)
+
+
+def test_canonical_order_and_capacity():
+    vol = synth.noise_volume((6, 7, 8), seed=9)
+    h = 0.5
+    full = O.extract(vol, h, want=("vv",))
+    part = O.extract(vol, h, want=("vv",), cap=full["n"] // 2)
+    assert part["n"] == full["n"]
+    assert part["vv"].tobytes() == full["vv"][: full["n"] // 2].tobytes()
+    # counts per cube in linear order reproduce the output offsets
+    _, nv, _ = O.classify(vol)
+    assert nv.sum() == full["n"]
+
+
+def test_openmp_matches_serial():
+    vol = synth.noise_volume((17, 9, 31), seed=11, p_zero_w=0.2)
+    a = O.extract(vol, 0.1, want=("vv", "ver_voxel"), nthreads=1)
+    b = O.extract(vol, 0.1, want=("vv", "ver_voxel"), nthreads=8)
+    assert a["vv"].tobytes() == b["vv"].tobytes()
+    assert a["ver_voxel"].tobytes() == b["ver_voxel"].tobytes()
+
+
+def test_backward_f64_matches_central_differences():
+    # delta = 2^-12 and |f| >= 0.25 make f +- delta and f +- delta/2 exact in
+    # fp32, so Richardson extrapolation of two central differences is O(delta^4)
+    rng = np.random.default_rng(1)
+    n = 6
+    f = rng.uniform(-1, 1, size=(n, n, n)).astype(np.float32)
+    h = 0.1
+    pos, _, _ = O.extract_f64(f, h)
+    g = rng.normal(size=pos.shape)
+    grad = O.extract_backward_f64(f, h, g)
+    delta = 2.0 ** -12
+
+    def cd(idx, d):
+        fp, fm = f.copy(), f.copy()
+        fp[idx] += np.float32(d)
+        fm[idx] -= np.float32(d)
+        assert float(fp[idx]) - float(fm[idx]) == 2 * d
+        pp, _, _ = O.extract_f64(fp, h)
+        pm, _, _ = O.extract_f64(fm, h)
+        assert pp.shape == pos.shape and pm.shape == pos.shape
+        return ((pp - pm) * g).sum() / (2 * d)
+
+    checked = 0
+    for idx in zip(*np.nonzero(np.abs(f) >= 0.25)):
+        fd = (4 * cd(idx, delta / 2) - cd(idx, delta)) / 3
+        assert abs(fd - grad[idx]) <= 1e-7 * max(1.0, abs(fd)), (idx, fd, grad[idx])
+        checked += 1
+        if checked >= 60:
+            break
+    assert checked >= 30
