@@ -57,8 +57,12 @@ const char* mcx_strerror(int status);
 int mcx_get_tables(uint64_t* case_table, uint8_t* num_verts);
 
 /* ---- workspace --------------------------------------------------------- */
-/* Bytes of workspace that mcx_extract needs for an X*Y*Z volume. */
+/* Bytes of workspace that mcx_extract needs for an X*Y*Z volume: the tile
+ * tables plus scratch for one 36-byte record per active cube, sized for one
+ * active cube in 16.  _ex sizes the record scratch explicitly; tiles whose
+ * records do not fit are re-read from the volume instead (same output). */
 size_t mcx_extract_workspace_size(int X, int Y, int Z);
+size_t mcx_extract_workspace_size_ex(int X, int Y, int Z, int64_t records);
 /* Zero a workspace once before its first use (stream-ordered fill kernel). */
 int mcx_workspace_init(void* d_ws, size_t ws_bytes, void* stream);
 
@@ -85,6 +89,26 @@ int mcx_extract(const void* d_vol, int elem, int X, int Y, int Z, float h,
  * the cube's min-corner voxel; 0 for cubes that do not exist). */
 int mcx_classify(const void* d_vol, int elem, int X, int Y, int Z,
                  uint8_t* d_ci, uint8_t* d_nv, void* stream);
+
+/* ---- differentiable extraction (no counterpart in the reference) -------
+ * mcx_extract_diff: extraction of an fp32 field (d_pos: float3 positions, same
+ * order and values as mcx_extract with d_pos) that also keeps in the
+ * workspace what the backward needs (active-cube bits and records).  With
+ * h_count it returns MCX_ERR_WORKSPACE when the record scratch was too small
+ * for some column (grow the workspace and call again).
+ * mcx_extract_backward: d_grad_vol[i] = sum over the vertices on edges at
+ * voxel i of dL/dp . dp/df_i, from d_grad_pos (float3 per vertex), using the
+ * workspace of the preceding mcx_extract_diff call on the same volume
+ * (unchanged) and d_scratch of mcx_backward_scratch_size() bytes.
+ * Deterministic: fixed-order sums, no atomics. */
+int mcx_extract_diff(const float* d_vol, int X, int Y, int Z, float h, float* d_pos,
+                     int64_t cap, int64_t* d_count, int64_t* h_count, void* d_ws,
+                     size_t ws_bytes, void* stream);
+size_t mcx_backward_scratch_size(int X, int Y, int Z, size_t ws_bytes);
+int mcx_extract_backward(const float* d_vol, int X, int Y, int Z, float h,
+                         const float* d_grad_pos, float* d_grad_vol, void* d_ws,
+                         size_t ws_bytes, void* d_scratch, size_t scratch_bytes,
+                         void* stream);
 
 /* ---- TSDF volume (replaces cu_voxel_new / d_voxel_initialize,
  *      cu_kinect_fusion_v1.cu:364-398): every voxel = (tsdf -32766, w 0). */

@@ -35,10 +35,15 @@ SIGNATURES = {
     "mcx_strerror": (ctypes.c_char_p, [_i32]),
     "mcx_get_tables": (_i32, [_vp, _vp]),
     "mcx_extract_workspace_size": (_sz, [_i32, _i32, _i32]),
+    "mcx_extract_workspace_size_ex": (_sz, [_i32, _i32, _i32, _i64]),
     "mcx_workspace_init": (_i32, [_vp, _sz, _vp]),
     "mcx_extract": (_i32, [_vp, _i32, _i32, _i32, _i32, _f32, _vp, _vp, _vp, _vp, _i64, _vp,
                            _vp, _vp, _sz, _vp]),
     "mcx_classify": (_i32, [_vp, _i32, _i32, _i32, _i32, _vp, _vp, _vp]),
+    "mcx_extract_diff": (_i32, [_vp, _i32, _i32, _i32, _f32, _vp, _i64, _vp, _vp, _vp, _sz, _vp]),
+    "mcx_backward_scratch_size": (_sz, [_i32, _i32, _i32, _sz]),
+    "mcx_extract_backward": (_i32, [_vp, _i32, _i32, _i32, _f32, _vp, _vp, _vp, _sz, _vp, _sz,
+                                    _vp]),
     "mcx_volume_init": (_i32, [_vp, _i32, _i32, _i32, _vp]),
     "mcx_integrate_workspace_size": (_sz, [_i32, _i32]),
     "mcx_integrate": (_i32, [_vp, _i32, _i32, _i32, _f32, _f32, _i32, _vp, _vp, _vp, _f32, _i32,

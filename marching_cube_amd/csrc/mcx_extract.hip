@@ -1,19 +1,30 @@
-// mcx_extract.hip -- fused single-pass marching-cubes extraction for gfx950.
+// mcx_extract.hip -- marching-cubes extraction for gfx950 in three kernels.
 //
 // Replaces the reference's three synchronous stages in MarchingCubes::run
 // (cu_marching_cubes_v1.cu:417-467):
 //   getOccupiedVoxels              :97-206  classify + atomic compaction
 //   computeOffsetsAndTotalVertexes :208-225  thrust exclusive scan + 2 D2H reads
 //   generateTriangles              :228-396  per-active-cube triangle emission
-// with one kernel that reads the volume once.  Each workgroup owns a tile of
-// L = 256 * 4 * G consecutive cube linear ids (z*X*Y + y*X + x), stages the two
-// voxel ranges it needs (plane z and plane z+1, each L + X + 1 voxels) in LDS,
-// classifies and counts, obtains its output offset by a decoupled look-back
-// over the tiles before it, and writes its vertices.  The tile order is the
-// ticket order taken at workgroup start (forward progress), so the output is
-// in canonical order (ascending cube id, then the reference's triangle slot
-// order with reversed winding, :319-332) and is deterministic.
 //
+// K1 count   Each wave owns a column: a range of kTile consecutive in-plane
+//            cube ids and a slab of consecutive z planes.  It marches z with
+//            the two current planes in registers and the next plane's loads in
+//            flight, so the volume is read once.  Classification is bit
+//            parallel: per 64 cubes, eight 64-bit ballot masks (corner signs)
+//            and eight more (zero weights) give the active cubes with a few
+//            scalar ops.  Active cubes get their case and a record (offset,
+//            case, 8 corner words) in the column's scratch quota; every tile
+//            (kTile cubes of one plane) gets its vertex count.
+// K2 scan    Exclusive scan of the tile counts (single pass, look-back over
+//            blocks of 4096 tiles) -> tile vertex offsets and the total.
+// K3 emit    Vertex-parallel emission: per tile, the records are staged in
+//            LDS, their vertex counts scanned, and each thread produces one
+//            output vertex, so neighbouring threads write neighbouring
+//            records.  A tile whose records did not fit in its column's quota
+//            re-stages its two voxel ranges from the volume instead.
+//
+// The output is in canonical order: ascending cube linear id (z*X*Y + y*X + x),
+// then the reference's triangle slot order with reversed winding (:319-332).
 // Per-cube semantics follow the reference exactly:
 //   cube index   computeCubeIndex :70-94 (any zero weight -> 0; bit k = f_k < 0)
 //   numVerts     0 for cases 0 and 255, else numVertsTable (:127-131)
@@ -35,6 +46,17 @@ namespace {
 __constant__ unsigned long long dCase[256] = MCX_CASE_TABLE_INIT;
 __constant__ unsigned char dNumVerts[256] = MCX_NUM_VERTS_INIT;
 
+constexpr int kCPW = 8;                // 64-cube chunks per wave and plane
+constexpr int kTile = 64 * kCPW;       // cubes per tile (one plane of a column)
+constexpr int kRecWords = 9;           // record: (offset << 8 | case), 8 corner words
+constexpr int kScanItems = 4096;       // tiles per scan block (256 threads x 16)
+constexpr int kEmitThreads = 256;
+constexpr int kTilesPerEmitGroup = 4096 / kTile;
+static_assert(kTile <= 4 * kEmitThreads, "staged path covers 4 cubes per thread");
+constexpr int kMaxLds = 160 * 1024;
+// third word of a gfx9 raw buffer resource (stride 0, 32-bit data format)
+constexpr int kBufferWord3 = 0x00020000;
+
 // Edge e runs from corner kEa[e] to kEb[e] (:291-302), 3 bits per edge.
 constexpr uint64_t kEdgeA = (0ull << 0) | (1ull << 3) | (2ull << 6) | (3ull << 9) | (4ull << 12) |
                             (5ull << 15) | (6ull << 18) | (7ull << 21) | (0ull << 24) |
@@ -43,41 +65,57 @@ constexpr uint64_t kEdgeB = (1ull << 0) | (2ull << 3) | (3ull << 6) | (0ull << 9
                             (6ull << 15) | (7ull << 18) | (4ull << 21) | (4ull << 24) |
                             (5ull << 27) | (6ull << 30) | (7ull << 33);
 
+__device__ __forceinline__ int corner_dx(int c) { return ((c + 1) >> 1) & 1; }
+__device__ __forceinline__ int corner_dy(int c) { return (c >> 1) & 1; }
+__device__ __forceinline__ int corner_dz(int c) { return c >> 2; }
+// corner id of offset (dx, dy, dz)
+__device__ __forceinline__ int corner_of(int dx, int dy, int dz) {
+  return 4 * dz + (dy ? (dx ? 2 : 3) : dx);
+}
+
 struct VolDesc {
-  const uint32_t* vol;  // X*Y*Z raw voxels (short2 bits or fp32 bits)
+  const uint32_t* vol;   // X*Y*Z raw voxels (short2 bits or fp32 bits), [z][y][x]
   int X, Y, Z;
+  int R;                 // tiles per plane = ceil(X*Y / kTile)
   float h;
-  int64_t tile_begin;   // first tile of this volume in the launch
-  int16_t* ver_voxel;   // optional, per volume
-  FastDiv divX, divY;
+  int64_t tile_begin;    // first tile of this volume in the launch
+  int16_t* ver_voxel;    // optional
+  FastDiv divX, divR;
+};
+
+struct Column {          // one K1 wave: volume, in-plane tile range, planes [z0, z1)
+  int vol, r, z0, z1;
 };
 
 struct ExtractParams {
-  VolDesc single;
-  const VolDesc* descs;  // batched launches: [nvol], sorted by tile_begin
+  VolDesc vol0;          // single-volume launches
+  const VolDesc* vols;   // batched launches: [nvol] in device memory
+  const Column* cols;    // batched launches: [ncols]
   int nvol;
+  int64_t ncols;
+  int slab;              // single volume: planes per column
   int64_t n_tiles;
-  int L;                 // cubes per tile
-  int lds_plane;         // words per staged plane range
+  int64_t quota;         // records per column
+  int emit_plane_words;  // staged path: words per voxel range
   mcx_vertex_voxel* vv;
   float* pos;
   int32_t* v2vox;
   int64_t cap;
   int64_t* d_count;
-  int64_t* vbase;        // batched: per-volume first vertex, [nvol + 1]
-  int64_t* tile_excl;    // optional: exclusive vertex offset of every tile
+  int64_t* vbase;
+  // workspace
   WsHeader* hdr;
-  uint64_t* desc;        // [2][cap_tiles]
-  int64_t cap_tiles;
+  uint64_t* desc;
+  int64_t cap_desc;
+  int32_t* tileT;        // vertex count per tile
+  int32_t* tile_nrec;    // records per tile, -1 = staged path
+  int64_t* tile_rec;     // first record of the tile
+  int64_t* voff;         // exclusive vertex offset per tile
+  uint64_t* masks;       // active-cube bits, kCPW words per tile
+  bool write_masks;
+  uint32_t* rec;
+  int64_t rec_cap;
 };
-
-constexpr int kThreads = 256;
-// LDS word offsets of the fixed part
-constexpr int kLdsCase = 0;      // 256 x u64
-constexpr int kLdsNv = 512;      // 256 x u8
-constexpr int kLdsRed = 576;     // 8 x u64 wave partials
-constexpr int kLdsBc = 592;      // 8 x i64 broadcast slots
-constexpr int kLdsPlanes = 608;  // two staged voxel ranges
 
 template <int ELEM>
 __device__ __forceinline__ float unpack_f(uint32_t v) {
@@ -89,47 +127,316 @@ __device__ __forceinline__ float unpack_f(uint32_t v) {
 }
 
 template <int ELEM>
-__device__ __forceinline__ uint32_t neg_bit(uint32_t v) {
+__device__ __forceinline__ bool is_neg(uint32_t v) {
   if constexpr (ELEM == MCX_ELEM_SHORT2) {
-    return (v >> 15) & 1u;  // tsdf < 0  <=>  tsdf / 32766 < 0
+    return (v & 0x8000u) != 0;  // tsdf < 0  <=>  tsdf / 32766 < 0
   } else {
-    return __uint_as_float(v) < 0.0f ? 1u : 0u;
+    return __uint_as_float(v) < 0.0f;
   }
 }
 
-// Cube index from the 8 corner words c[0..7] in the reference corner order.
+template <int ELEM>
+__device__ __forceinline__ bool is_w0(uint32_t v) {
+  if constexpr (ELEM == MCX_ELEM_SHORT2) return v < 0x10000u;  // weight == 0
+  return false;
+}
+
+// Cube index from the 8 corner words in the reference corner order.
 template <int ELEM>
 __device__ __forceinline__ uint32_t cube_index(const uint32_t c[8]) {
   uint32_t ci = 0;
+  bool w0 = false;
 #pragma unroll
-  for (int k = 0; k < 8; ++k) ci |= neg_bit<ELEM>(c[k]) << k;
-  if constexpr (ELEM == MCX_ELEM_SHORT2) {
-    uint32_t mn = c[0];
-#pragma unroll
-    for (int k = 1; k < 8; ++k) mn = min(mn, c[k]);
-    if (mn < 0x10000u) ci = 0;  // some weight == 0
+  for (int k = 0; k < 8; ++k) {
+    ci |= (is_neg<ELEM>(c[k]) ? 1u : 0u) << k;
+    w0 = w0 || is_w0<ELEM>(c[k]);
   }
-  return ci;
+  return w0 ? 0u : ci;
 }
 
-// Advance (x, y, z) by k cubes along the linear order (k < 2X).
-__device__ __forceinline__ void step_coords(int X, int Y, int& x, int& y, int& z, int k) {
-  x += k;
-  while (x >= X) {
-    x -= X;
-    ++y;
+template <bool BATCH>
+__device__ __forceinline__ const VolDesc& vol_at(const ExtractParams& P, int vi) {
+  if constexpr (BATCH) return P.vols[vi];
+  return P.vol0;
+}
+
+// Volume of global tile t (batched: last volume with tile_begin <= t).
+__device__ __forceinline__ int volume_of_tile(const ExtractParams& P, int64_t t) {
+  int lo = 0, hi = P.nvol - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (P.vols[mid].tile_begin <= t) lo = mid; else hi = mid - 1;
   }
-  while (y >= Y) {
-    y -= Y;
-    ++z;
+  return lo;
+}
+
+// ============================================================ K1: count
+template <int ELEM, bool BATCH>
+__global__ __launch_bounds__(256) void count_kernel(ExtractParams P) {
+  __shared__ uint8_t s_nv[256];
+  s_nv[threadIdx.x] = dNumVerts[threadIdx.x];
+  __syncthreads();
+  const int lane = threadIdx.x & 63;
+  // wave index made wave-uniform so that everything derived from the column
+  // (plane bases, loop bounds) lives in scalar registers
+  const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  const int64_t col = (int64_t)blockIdx.x * 4 + wave;
+  if (col >= P.ncols) return;  // whole waves exit; no barrier follows
+
+  int vi, r, z0, z1;
+  if constexpr (BATCH) {
+    const Column c = P.cols[col];
+    vi = c.vol;
+    r = c.r;
+    z0 = c.z0;
+    z1 = c.z1;
+  } else {
+    vi = 0;
+    r = (int)(col % P.vol0.R);
+    const int s = (int)(col / P.vol0.R);
+    z0 = s * P.slab;
+    z1 = min(z0 + P.slab, P.vol0.Z - 1);
+  }
+  const VolDesc& V = vol_at<BATCH>(P, vi);
+  const int X = V.X;
+  const int64_t XY = (int64_t)X * V.Y;
+  const int64_t p0 = (int64_t)r * kTile;
+  const int64_t rec_base = col * P.quota;
+
+  // validity of the lane's cube in chunk j (identical for every plane): bit j
+  uint32_t vbits = 0;
+#pragma unroll
+  for (int j = 0; j < kCPW; ++j) {
+    const int64_t p = p0 + 64 * j + lane;
+    if (p < XY) {
+      const uint32_t y = V.divX.div((uint32_t)p);
+      const uint32_t x = (uint32_t)p - y * (uint32_t)X;
+      if (x + 1 < (uint32_t)X && y + 1 < (uint32_t)V.Y) vbits |= 1u << j;
+    }
+  }
+
+  // plane q values at in-plane p (U) and p + X (W); chunk kCPW only on lane 0.
+  // Raw buffer loads: the resource spans exactly plane q (X*Y words), so a
+  // load past the plane returns 0, and every load shares one VGPR offset.
+  uint32_t A[kCPW + 1], B[kCPW + 1], C[kCPW + 1], D[kCPW + 1];
+  uint32_t E[kCPW + 1] = {}, F[kCPW + 1] = {};
+  const uint32_t voff = (uint32_t)(p0 + lane) * 4u;
+  // chunk kCPW is only needed on lane 0; other lanes use an offset past any
+  // plane, which returns 0 without touching memory
+  const uint32_t voff_x = lane == 0 ? voff + 256u * kCPW : 0xfffffff0u;
+  const uint32_t plane_bytes = (uint32_t)(XY * 4);
+  // Every load is unconditional (no branch around it), so the compiler can
+  // keep the next plane's loads in flight while the current planes are
+  // classified.  A plane that is not needed gets an empty resource.
+  auto load_plane = [&](int q, bool need, uint32_t* U, uint32_t* W) {
+    const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
+        const_cast<uint32_t*>(V.vol + (need ? (int64_t)q * XY : 0)), (short)0,
+        need ? (int)plane_bytes : 0, kBufferWord3);
+#pragma unroll
+    for (int j = 0; j < kCPW; ++j) {
+      U[j] = __builtin_amdgcn_raw_buffer_load_b32(rs, voff, 256 * j, 0);
+      W[j] = __builtin_amdgcn_raw_buffer_load_b32(rs, voff, 256 * j + 4 * X, 0);
+    }
+    U[kCPW] = __builtin_amdgcn_raw_buffer_load_b32(rs, voff_x, 0, 0);
+    W[kCPW] = __builtin_amdgcn_raw_buffer_load_b32(rs, voff_x, 4 * X, 0);
+  };
+  load_plane(z0, true, A, B);
+  load_plane(z0 + 1, true, C, D);
+
+  // sign and zero-weight masks of one chunk (wave-uniform)
+  struct Masks {
+    uint64_t n[4], w[4];
+  };
+  auto masks_of = [&](int j) {
+    Masks M;
+    M.n[0] = __ballot(is_neg<ELEM>(A[j]));
+    M.n[1] = __ballot(is_neg<ELEM>(B[j]));
+    M.n[2] = __ballot(is_neg<ELEM>(C[j]));
+    M.n[3] = __ballot(is_neg<ELEM>(D[j]));
+    if constexpr (ELEM == MCX_ELEM_SHORT2) {
+      M.w[0] = __ballot(is_w0<ELEM>(A[j]));
+      M.w[1] = __ballot(is_w0<ELEM>(B[j]));
+      M.w[2] = __ballot(is_w0<ELEM>(C[j]));
+      M.w[3] = __ballot(is_w0<ELEM>(D[j]));
+    } else {
+      M.w[0] = M.w[1] = M.w[2] = M.w[3] = 0;
+    }
+    return M;
+  };
+
+  int64_t used = 0;
+  for (int z = z0; z < z1; ++z) {
+    load_plane(z + 2, z + 1 < z1, E, F);
+    const int64_t t = V.tile_begin + (int64_t)z * V.R + r;
+    const int64_t room = P.quota - used;  // records this tile may still write
+
+    uint32_t vsum = 0;
+    int64_t nrec = 0;
+    Masks M0 = masks_of(0);
+#pragma unroll
+    for (int j = 0; j < kCPW; ++j) {
+      const Masks M1 = masks_of(j + 1);
+      // x+1 neighbour masks: shift down one lane, bit 63 from the next chunk
+      uint64_t all = ~0ull, any = 0, w = 0;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const uint64_t n1 = (M0.n[q] >> 1) | ((M1.n[q] & 1ull) << 63);
+        all &= M0.n[q] & n1;
+        any |= M0.n[q] | n1;
+        w |= M0.w[q] | (M0.w[q] >> 1) | ((M1.w[q] & 1ull) << 63);
+      }
+      const uint64_t m = any & ~all & ~w & __ballot((vbits >> j) & 1u);
+      M0 = M1;
+      if (P.write_masks && lane == 0) P.masks[t * kCPW + j] = m;
+      if (m == 0) continue;
+      uint32_t a1 = __shfl_down(A[j], 1), b1 = __shfl_down(B[j], 1);
+      uint32_t c1 = __shfl_down(C[j], 1), d1 = __shfl_down(D[j], 1);
+      const uint32_t na = __shfl(A[j + 1], 0), nb = __shfl(B[j + 1], 0);
+      const uint32_t nc = __shfl(C[j + 1], 0), nd = __shfl(D[j + 1], 0);
+      if (lane == 63) {
+        a1 = na;
+        b1 = nb;
+        c1 = nc;
+        d1 = nd;
+      }
+      if ((m >> lane) & 1ull) {
+        const uint32_t cw[8] = {A[j], a1, b1, B[j], C[j], c1, d1, D[j]};
+        const uint32_t ci = cube_index<ELEM>(cw);
+        vsum += s_nv[ci];
+        const int64_t k = nrec + __popcll(m & ((1ull << lane) - 1ull));
+        if (k < room) {
+          uint32_t* rp = P.rec + (rec_base + used + k) * kRecWords;
+          rp[0] = ((uint32_t)(64 * j + lane) << 8) | ci;
+#pragma unroll
+          for (int q = 0; q < 8; ++q) rp[1 + q] = cw[q];
+        }
+      }
+      nrec += __popcll(m);
+    }
+    const bool dense = nrec > room;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) vsum += __shfl_xor(vsum, o);
+    if (lane == 0) {
+      P.tileT[t] = (int32_t)vsum;
+      P.tile_nrec[t] = dense ? -1 : (int32_t)nrec;
+      P.tile_rec[t] = rec_base + used;
+      if (dense) atomicAdd(&P.hdr->dense_acc, 1u);
+    }
+    if (!dense) used += nrec;
+
+#pragma unroll
+    for (int j = 0; j <= kCPW; ++j) {
+      A[j] = C[j];
+      B[j] = D[j];
+      C[j] = E[j];
+      D[j] = F[j];
+    }
   }
 }
 
+// ============================================================ K2: scan
+__global__ __launch_bounds__(256) void scan_kernel(ExtractParams P) {
+  __shared__ int64_t s_wave[4];
+  __shared__ int64_t s_excl;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int64_t blk = blockIdx.x;
+  const int64_t i0 = blk * kScanItems + (int64_t)tid * 16;
+  int32_t v[16];
+#pragma unroll
+  for (int k = 0; k < 16; ++k) v[k] = i0 + k < P.n_tiles ? P.tileT[i0 + k] : 0;
+  int64_t s = 0;
+#pragma unroll
+  for (int k = 0; k < 16; ++k) s += v[k];
+  const uint64_t incl = wave_incl_scan((uint64_t)s);
+  if (lane == 63) s_wave[wave] = (int64_t)incl;
+  __syncthreads();
+  int64_t wpre = 0, total = 0;
+#pragma unroll
+  for (int w = 0; w < 4; ++w) {
+    if (w < wave) wpre += s_wave[w];
+    total += s_wave[w];
+  }
+  const int64_t texcl = (int64_t)incl - s + wpre;
+  const uint32_t parity =
+      __hip_atomic_load(&P.hdr->parity, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & 1u;
+  uint64_t* desc = P.desc + (int64_t)parity * P.cap_desc;
+  uint64_t* desc_other = P.desc + (int64_t)(parity ^ 1u) * P.cap_desc;
+  if (wave == 0) {
+    int64_t excl = 0;
+    if (blk == 0) {
+      if (lane == 0) st_agent(desc, kStatInc | (uint64_t)total);
+    } else {
+      if (lane == 0) st_agent(desc + blk, kStatAgg | (uint64_t)total);
+      excl = lookback<1>(desc, blk, &P.hdr->error);
+      if (lane == 0) st_agent(desc + blk, kStatInc | (uint64_t)(excl + total));
+    }
+    if (lane == 0) {
+      s_excl = excl;
+      if (blk == (int64_t)gridDim.x - 1) {
+        P.hdr->count = excl + total;
+        if (P.d_count) *P.d_count = excl + total;
+        P.hdr->dense_last = P.hdr->dense_acc;  // K1 has finished: hand the count over
+        P.hdr->dense_acc = 0;
+        // every block has read the parity (the look-back saw all their
+        // aggregates): flip it for the next call
+        __hip_atomic_store(&P.hdr->parity, parity ^ 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      }
+    }
+  } else {
+    const int64_t k = (P.cap_desc + gridDim.x - 1) / gridDim.x;
+    const int64_t beg = blk * k;
+    const int64_t end = min(beg + k, P.cap_desc);
+    for (int64_t q = beg + (tid - 64); q < end; q += 192) desc_other[q] = 0;
+  }
+  __syncthreads();
+  int64_t run = s_excl + texcl;
+#pragma unroll
+  for (int k = 0; k < 16; ++k) {
+    if (i0 + k < P.n_tiles) P.voff[i0 + k] = run;
+    run += v[k];
+  }
+}
+
+// per-volume first vertex (batched launches), vbase[nvol] = total
+__global__ __launch_bounds__(256) void vbase_kernel(ExtractParams P) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b > P.nvol) return;
+  const int64_t total = P.hdr->count;
+  if (b == P.nvol) {
+    P.vbase[b] = total;
+    return;
+  }
+  const int64_t t = P.vols[b].tile_begin;
+  P.vbase[b] = t < P.n_tiles ? P.voff[t] : total;
+}
+
+// ============================================================ K3: emit
+// Output vertex `idx`: slot q of cube (x, y, z) with case ci and corner words
+// cw[8] in the reference corner order.
 template <int ELEM>
-__device__ __forceinline__ void store_point(const ExtractParams& P, const VolDesc& V,
-                                            int64_t idx, float px, float py, float pz) {
+__device__ __forceinline__ void emit_vertex(const ExtractParams& P, const VolDesc& V,
+                                            const uint64_t* s_case, const uint32_t cw[8], int x,
+                                            int y, int z, uint32_t ci, int q, int64_t idx) {
   if (idx >= P.cap) return;
+  // output slot q = 3*tri + r holds table corner 2 - r of triangle tri (:324-332)
+  const int m = 3 * (q / 3) + 2 - (q % 3);
+  const int e = (int)((s_case[ci] >> (4 * m)) & 0xfu);
+  const int ca = (int)((kEdgeA >> (3 * e)) & 7u);
+  const int cb = (int)((kEdgeB >> (3 * e)) & 7u);
   const float h = V.h;
+  const float fa = unpack_f<ELEM>(cw[ca]);
+  const float fb = unpack_f<ELEM>(cw[cb]);
+  const float pax = ((float)(x + corner_dx(ca)) + 0.5f) * h;
+  const float pay = ((float)(y + corner_dy(ca)) + 0.5f) * h;
+  const float paz = ((float)(z + corner_dz(ca)) + 0.5f) * h;
+  const float pbx = ((float)(x + corner_dx(cb)) + 0.5f) * h;
+  const float pby = ((float)(y + corner_dy(cb)) + 0.5f) * h;
+  const float pbz = ((float)(z + corner_dz(cb)) + 0.5f) * h;
+  const float tt = (0.0f - fa) / (fb - fa + 1e-15f);
+  const float px = __fmaf_rn(tt, pbx - pax, pax);
+  const float py = __fmaf_rn(tt, pby - pay, pay);
+  const float pz = __fmaf_rn(tt, pbz - paz, paz);
+  // record fields (store_point :336-367)
   float cx = px / h, cy = py / h, cz = pz / h;
   cx = cx - 0.5f;
   cy = cy - 0.5f;
@@ -140,289 +447,395 @@ __device__ __forceinline__ void store_point(const ExtractParams& P, const VolDes
   int64_t li = 0;
   if (inb) {
     li = ((int64_t)vz * V.Y + vy) * V.X + vx;
-    if constexpr (ELEM == MCX_ELEM_SHORT2) tw = V.vol[li];
+    if constexpr (ELEM == MCX_ELEM_SHORT2) {
+      const unsigned dx = (unsigned)(vx - x), dy = (unsigned)(vy - y), dz = (unsigned)(vz - z);
+      tw = (dx | dy | dz) <= 1u ? cw[corner_of((int)dx, (int)dy, (int)dz)] : V.vol[li];
+    }
   }
   if (P.vv) {
-    uint32_t* r = reinterpret_cast<uint32_t*>(P.vv) + 7 * idx;
-    r[0] = __float_as_uint(px);
-    r[1] = __float_as_uint(py);
-    r[2] = __float_as_uint(pz);
-    r[3] = (uint32_t)vx;
-    r[4] = (uint32_t)vy;
-    r[5] = (uint32_t)vz;
-    r[6] = tw;
+    uint32_t* rr = reinterpret_cast<uint32_t*>(P.vv) + 7 * idx;
+    rr[0] = __float_as_uint(px);
+    rr[1] = __float_as_uint(py);
+    rr[2] = __float_as_uint(pz);
+    rr[3] = (uint32_t)vx;
+    rr[4] = (uint32_t)vy;
+    rr[5] = (uint32_t)vz;
+    rr[6] = tw;
   }
   if (P.pos) {
-    float* q = P.pos + 3 * idx;
-    q[0] = px;
-    q[1] = py;
-    q[2] = pz;
+    float* o = P.pos + 3 * idx;
+    o[0] = px;
+    o[1] = py;
+    o[2] = pz;
   }
   if (P.v2vox) {
-    int32_t* q = P.v2vox + 3 * idx;
-    q[0] = vx;
-    q[1] = vy;
-    q[2] = vz;
+    int32_t* o = P.v2vox + 3 * idx;
+    o[0] = vx;
+    o[1] = vy;
+    o[2] = vz;
   }
   if (V.ver_voxel && inb) V.ver_voxel[li] = (int16_t)(tw & 0xffffu);
 }
 
-template <int G, bool BATCH, int ELEM>
-__global__ __launch_bounds__(kThreads) void extract_kernel(ExtractParams P) {
-  extern __shared__ uint32_t smem[];
-  uint64_t* s_case = reinterpret_cast<uint64_t*>(smem + kLdsCase);
-  uint8_t* s_nv = reinterpret_cast<uint8_t*>(smem + kLdsNv);
-  uint64_t* s_red = reinterpret_cast<uint64_t*>(smem + kLdsRed);
-  int64_t* s_bc = reinterpret_cast<int64_t*>(smem + kLdsBc);
-  uint32_t* s_p0 = smem + kLdsPlanes;
-  uint32_t* s_p1 = s_p0 + P.lds_plane;
+// Exclusive scan of one int per thread over a 256-thread workgroup; returns
+// the thread's exclusive prefix and the total through *tot (s_tmp: 4 ints).
+__device__ __forceinline__ int block_excl_scan(int v, int* s_tmp, int* tot) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const uint32_t incl = wave_incl_scan32((uint32_t)v);
+  if (lane == 63) s_tmp[wave] = (int)incl;
+  __syncthreads();
+  int pre = 0, all = 0;
+#pragma unroll
+  for (int w = 0; w < 4; ++w) {
+    if (w < wave) pre += s_tmp[w];
+    all += s_tmp[w];
+  }
+  __syncthreads();
+  *tot = all;
+  return (int)incl - v + pre;
+}
 
+// LDS word layout of the emit kernel
+constexpr int kEmCase = 0;    // 256 x u64
+constexpr int kEmNv = 512;    // 256 x u8
+constexpr int kEmTmp = 576;   // scan scratch (8 words)
+constexpr int kEmTile = 584;  // per-tile info, kTilesPerEmitGroup x 8 words
+constexpr int kEmMap = kEmTile + 8 * kTilesPerEmitGroup;  // kEmitThreads map entries
+constexpr int kEmDat = kEmMap + kEmitThreads;              // records or staged voxel ranges
+
+struct TileInfo {      // 8 words in LDS
+  int64_t delta;       // output index of the tile's first record vertex minus
+                       // the group-list vertex prefix at that record
+  int64_t rstart;      // first record in scratch
+  int32_t nrec;        // records (-1: staged path)
+  int32_t T;           // vertices
+  int32_t lbeg;        // first position in the group's record list
+  int32_t vi;          // volume
+};
+static_assert(sizeof(TileInfo) == 32, "tile info is 8 words");
+
+// Emit the vertices of one tile from its two staged voxel ranges (the path
+// for tiles whose records did not fit in the scratch quota).
+template <int ELEM>
+__device__ void emit_staged(const ExtractParams& P, const VolDesc& V, int64_t t, int64_t vbeg,
+                            const uint64_t* s_case, const uint8_t* s_nv, int* s_tmp,
+                            uint32_t* s_map, uint32_t* s_dat) {
   const int tid = threadIdx.x;
-  const int wave = tid >> 6;
-  const int lane = tid & 63;
-  constexpr int L = kThreads * 4 * G;
-
-  s_case[tid] = dCase[tid];
-  s_nv[tid] = dNumVerts[tid];
-  if (tid == 0) {
-    const uint32_t tk =
-        __hip_atomic_fetch_add(&P.hdr->ticket, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    const uint32_t par = __hip_atomic_load(&P.hdr->parity, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    int vi = 0;
-    if constexpr (BATCH) {
-      int lo = 0, hi = P.nvol - 1;  // last volume with tile_begin <= tk
-      while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (P.descs[mid].tile_begin <= (int64_t)tk) lo = mid; else hi = mid - 1;
-      }
-      vi = lo;
-    }
-    s_bc[0] = tk;
-    s_bc[1] = par & 1u;
-    s_bc[3] = vi;
+  const int X = V.X;
+  const int64_t XY = (int64_t)X * V.Y;
+  const int64_t local = t - V.tile_begin;
+  const int z = (int)V.divR.div((uint32_t)local);
+  const int64_t p0 = (local - (int64_t)z * V.R) * kTile;
+  const int plane_words = P.emit_plane_words;
+  const int64_t N = XY * V.Z;
+  for (int pz = 0; pz < 2; ++pz) {
+    const int64_t start = (int64_t)(z + pz) * XY + p0;
+    int64_t len = N - start;
+    if (len > (int64_t)kTile + X + 1) len = (int64_t)kTile + X + 1;
+    uint32_t* dst = s_dat + pz * plane_words;
+    for (int64_t k = tid; k < len; k += kEmitThreads) dst[k] = V.vol[start + k];
   }
   __syncthreads();
-  const int64_t t = s_bc[0];
-  const uint32_t parity = (uint32_t)s_bc[1];
-  const int vi = (int)s_bc[3];
-  const VolDesc V = BATCH ? P.descs[vi] : P.single;
-  uint64_t* desc = P.desc + (int64_t)parity * P.cap_tiles;
-  uint64_t* desc_other = P.desc + (int64_t)(parity ^ 1u) * P.cap_tiles;
-
-  const int X = V.X, Y = V.Y, Z = V.Z;
-  const int64_t XY = (int64_t)X * Y;
-  const int64_t N = XY * Z;
-  const int64_t ncube_lin = XY * (Z - 1);
-  const int64_t i0 = (t - V.tile_begin) * (int64_t)L;
-
-  // tile start coordinates (x0, y0, z0)
-  int x0, y0, z0;
-  {
-    const int64_t r = i0 / X;
-    x0 = (int)(i0 - r * X);
-    z0 = (int)(r / Y);
-    y0 = (int)(r - (int64_t)z0 * Y);
-  }
-
-  // ---- stage voxel ranges [i0, i0+L+X+1) of plane z and z+1 in LDS
-  int shift[2];
+  // thread owns cubes 4*tid .. 4*tid+3 of the tile (those below kTile)
+  uint32_t ciw = 0;
+  int cnt = 0;
 #pragma unroll
-  for (int p = 0; p < 2; ++p) {
-    const int64_t start = i0 + p * XY;
-    int64_t len64 = N - start;
-    const int64_t want = (int64_t)L + X + 1;
-    if (len64 > want) len64 = want;
-    if (len64 < 0) len64 = 0;
-    const int len = (int)len64;
-    const uint32_t* g = V.vol + (start < N ? start : 0);
-    const int sh = (int)((reinterpret_cast<uintptr_t>(g) >> 2) & 3u);
-    shift[p] = sh;
-    const uint4* ga = reinterpret_cast<const uint4*>(g - sh);
-    uint32_t* dst = p == 0 ? s_p0 : s_p1;
-    const int nchunks = (len + sh + 3) >> 2;
-    for (int c = tid; c < nchunks; c += kThreads) {
-      const int j0 = 4 * c - sh;
-      if (j0 >= 0 && j0 + 4 <= len) {
-        *reinterpret_cast<uint4*>(dst + 4 * c) = ga[c];
-      } else {
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-          const int jj = j0 + q;
-          dst[4 * c + q] = (jj >= 0 && jj < len) ? g[jj] : 0u;
-        }
+  for (int k = 0; k < 4; ++k) {
+    const int o = 4 * tid + k;
+    const int64_t p = p0 + o;
+    uint32_t ci = 0;
+    if (o < kTile && p < XY) {
+      const uint32_t y = V.divX.div((uint32_t)p);
+      const uint32_t x = (uint32_t)p - y * (uint32_t)X;
+      if (x + 1 < (uint32_t)X && y + 1 < (uint32_t)V.Y) {
+        const uint32_t* a = s_dat + o;
+        const uint32_t* b = s_dat + plane_words + o;
+        const uint32_t cw[8] = {a[0], a[1], a[X + 1], a[X], b[0], b[1], b[X + 1], b[X]};
+        ci = cube_index<ELEM>(cw);
       }
     }
+    ciw |= ci << (8 * k);
+    cnt += s_nv[ci];
   }
-  __syncthreads();
-
-  // ---- classify + count: thread owns cubes o = g*1024 + 4*tid + k
-  uint32_t cis[G];
-  uint64_t packed = 0;  // 16-bit vertex count per group
+  int tot;
+  const int ex = block_excl_scan(cnt, s_tmp, &tot);
+  for (int vb = 0; vb < tot; vb += kEmitThreads) {
+    int off = ex;
 #pragma unroll
-  for (int g = 0; g < G; ++g) {
-    const int o = g * (kThreads * 4) + 4 * tid;
-    uint32_t ciw = 0, cnt = 0;
-    if (i0 + o < ncube_lin) {
-      int x = x0, y = y0, z = z0;
-      {
-        const uint32_t xs = (uint32_t)(x0 + o);
-        const uint32_t q = V.divX.div(xs);
-        x = (int)(xs - q * (uint32_t)X);
-        const uint32_t ys = (uint32_t)y0 + q;
-        const uint32_t q2 = V.divY.div(ys);
-        y = (int)(ys - q2 * (uint32_t)Y);
-        z = z0 + (int)q2;
-      }
-      const uint32_t* a = s_p0 + shift[0] + o;
-      const uint32_t* b = s_p1 + shift[1] + o;
-      uint32_t r0[5], r1[5], r2[5], r3[5];
-#pragma unroll
-      for (int k = 0; k < 5; ++k) {
-        r0[k] = a[k];
-        r1[k] = a[X + k];
-        r2[k] = b[k];
-        r3[k] = b[X + k];
-      }
-#pragma unroll
-      for (int k = 0; k < 4; ++k) {
-        int xk = x, yk = y, zk = z;
-        if (k) step_coords(X, Y, xk, yk, zk, k);
-        const bool valid = xk < X - 1 && yk < Y - 1 && zk < Z - 1;
-        const uint32_t c[8] = {r0[k], r0[k + 1], r1[k + 1], r1[k],
-                               r2[k], r2[k + 1], r3[k + 1], r3[k]};
-        uint32_t ci = valid ? cube_index<ELEM>(c) : 0u;
-        const uint32_t nv = (ci - 1u) < 254u ? (uint32_t)s_nv[ci] : 0u;
-        ci = nv ? ci : 0u;
-        ciw |= ci << (8 * k);
-        cnt += nv;
-      }
-    }
-    cis[g] = ciw;
-    packed |= (uint64_t)cnt << (16 * g);
-  }
-
-  // ---- workgroup exclusive scan of the packed group counts
-  const uint64_t incl = wave_incl_scan(packed);
-  if (lane == 63) s_red[wave] = incl;
-  __syncthreads();
-  uint64_t wprefix = 0, total = 0;
-#pragma unroll
-  for (int w = 0; w < kThreads / kWave; ++w) {
-    const uint64_t v = s_red[w];
-    if (w < wave) wprefix += v;
-    total += v;
-  }
-  const uint64_t texcl = incl - packed + wprefix;
-  int64_t gbase[G];
-  int64_t T = 0;
-#pragma unroll
-  for (int g = 0; g < G; ++g) {
-    gbase[g] = T;
-    T += (int64_t)((total >> (16 * g)) & 0xffffu);
-  }
-
-  // ---- decoupled look-back (wave 0); waves 1..3 clear the other parity
-  if (wave == 0) {
-    int64_t excl = 0;
-    if (t == 0) {
-      if (lane == 0) st_agent(desc + t, kStatInc | (uint64_t)T);
-    } else {
-      if (lane == 0) st_agent(desc + t, kStatAgg | (uint64_t)T);
-      excl = lookback(desc, t, &P.hdr->error);
-      if (lane == 0) st_agent(desc + t, kStatInc | (uint64_t)(excl + T));
-    }
-    if (lane == 0) {
-      s_bc[2] = excl;
-      if (P.tile_excl) P.tile_excl[t] = excl;
-      if constexpr (BATCH) {
-        if (t == V.tile_begin) {
-          P.vbase[vi] = excl;
-          for (int b = vi - 1; b >= 0 && P.descs[b].tile_begin == V.tile_begin; --b) P.vbase[b] = excl;
-        }
-      }
-      if (t == P.n_tiles - 1) {
-        P.hdr->count = excl + T;
-        if (P.d_count) *P.d_count = excl + T;
-        if constexpr (BATCH) {
-          for (int b = P.nvol; b > vi; --b) P.vbase[b] = excl + T;
-        }
-      }
-    }
-  } else {
-    const int64_t k = (P.cap_tiles + P.n_tiles - 1) / P.n_tiles;
-    const int64_t beg = t * k;
-    int64_t end = beg + k;
-    if (end > P.cap_tiles) end = P.cap_tiles;
-    for (int64_t s = beg + (tid - kWave); s < end; s += kThreads - kWave) desc_other[s] = 0;
-  }
-  __syncthreads();
-  const int64_t excl = s_bc[2];
-
-  // ---- emit
-  const float h = V.h;
-#pragma unroll
-  for (int g = 0; g < G; ++g) {
-    const uint32_t ciw = cis[g];
-    if (ciw == 0) continue;
-    int64_t off = excl + gbase[g] + (int64_t)((texcl >> (16 * g)) & 0xffffu);
-    const int o = g * (kThreads * 4) + 4 * tid;
-    int x, y, z;
-    {
-      const uint32_t xs = (uint32_t)(x0 + o);
-      const uint32_t q = V.divX.div(xs);
-      x = (int)(xs - q * (uint32_t)X);
-      const uint32_t ys = (uint32_t)y0 + q;
-      const uint32_t q2 = V.divY.div(ys);
-      y = (int)(ys - q2 * (uint32_t)Y);
-      z = z0 + (int)q2;
-    }
     for (int k = 0; k < 4; ++k) {
       const uint32_t ci = (ciw >> (8 * k)) & 0xffu;
-      if (ci == 0) continue;
-      int xk = x, yk = y, zk = z;
-      if (k) step_coords(X, Y, xk, yk, zk, k);
-      const uint64_t cs = s_case[ci];
-      const int nverts = 3 * (int)(cs >> 60);
-      const uint32_t* pl[2] = {s_p0 + shift[0] + o + k, s_p1 + shift[1] + o + k};
-      for (int m = 0; m < nverts; ++m) {
-        const int e = (int)((cs >> (4 * m)) & 0xfu);
-        const int ca = (int)((kEdgeA >> (3 * e)) & 7u);
-        const int cb = (int)((kEdgeB >> (3 * e)) & 7u);
-        const int ax = ((ca + 1) >> 1) & 1, ay = (ca >> 1) & 1, az = ca >> 2;
-        const int bx = ((cb + 1) >> 1) & 1, by = (cb >> 1) & 1, bz = cb >> 2;
-        const float fa = unpack_f<ELEM>(pl[az][ax + ay * X]);
-        const float fb = unpack_f<ELEM>(pl[bz][bx + by * X]);
-        const float pax = ((float)(xk + ax) + 0.5f) * h;
-        const float pay = ((float)(yk + ay) + 0.5f) * h;
-        const float paz = ((float)(zk + az) + 0.5f) * h;
-        const float pbx = ((float)(xk + bx) + 0.5f) * h;
-        const float pby = ((float)(yk + by) + 0.5f) * h;
-        const float pbz = ((float)(zk + bz) + 0.5f) * h;
-        const float tt = (0.0f - fa) / (fb - fa + 1e-15f);
-        const float px = __fmaf_rn(tt, pbx - pax, pax);
-        const float py = __fmaf_rn(tt, pby - pay, pay);
-        const float pz = __fmaf_rn(tt, pbz - paz, paz);
-        // triangle corner m%3 of triangle m/3 goes to slot 2 - m%3 (:324-332)
-        const int tri = m / 3, corner = m - 3 * tri;
-        store_point<ELEM>(P, V, off + 3 * tri + (2 - corner), px, py, pz);
+      const int nv = s_nv[ci];
+      for (int q = 0; q < nv; ++q) {
+        const int j = off + q - vb;
+        if (j >= 0 && j < kEmitThreads)
+          s_map[j] = ((uint32_t)(4 * tid + k) << 12) | (ci << 4) | (uint32_t)q;
       }
-      off += nverts;
+      off += nv;
     }
+    __syncthreads();
+    if (vb + tid < tot) {
+      const uint32_t e = s_map[tid];
+      const int o = (int)(e >> 12);
+      const uint32_t* a = s_dat + o;
+      const uint32_t* b = s_dat + plane_words + o;
+      const uint32_t cw[8] = {a[0], a[1], a[X + 1], a[X], b[0], b[1], b[X + 1], b[X]};
+      const int64_t p = p0 + o;
+      const uint32_t y = V.divX.div((uint32_t)p);
+      const int x = (int)((uint32_t)p - y * (uint32_t)X);
+      emit_vertex<ELEM>(P, V, s_case, cw, x, (int)y, z, (e >> 4) & 0xffu, (int)(e & 15u),
+                        vbeg + vb + tid);
+    }
+    __syncthreads();
+  }
+}
+
+template <int ELEM, bool BATCH>
+__global__ __launch_bounds__(kEmitThreads) void emit_kernel(ExtractParams P) {
+  extern __shared__ uint32_t smem[];
+  uint64_t* s_case = reinterpret_cast<uint64_t*>(smem + kEmCase);
+  uint8_t* s_nv = reinterpret_cast<uint8_t*>(smem + kEmNv);
+  int* s_tmp = reinterpret_cast<int*>(smem + kEmTmp);
+  TileInfo* s_tile = reinterpret_cast<TileInfo*>(smem + kEmTile);
+  uint32_t* s_map = smem + kEmMap;
+  uint32_t* s_dat = smem + kEmDat;
+  const int tid = threadIdx.x;
+  const int64_t t0 = (int64_t)blockIdx.x * kTilesPerEmitGroup;
+  s_case[tid] = dCase[tid];
+  s_nv[tid] = dNumVerts[tid];
+  if (tid < kTilesPerEmitGroup) {
+    const int64_t t = t0 + tid;
+    TileInfo ti{};
+    if (t < P.n_tiles) {
+      ti.T = P.tileT[t];
+      ti.nrec = P.tile_nrec[t];
+      ti.rstart = P.tile_rec[t];
+      ti.delta = P.voff[t];
+      ti.vi = BATCH ? volume_of_tile(P, t) : 0;
+    }
+    s_tile[tid] = ti;
+  }
+  __syncthreads();
+  if (tid == 0) {
+    // group record list: records of the tiles in order; the vertices of a
+    // tile's records are the tile's vertices, so a record's output index is
+    // delta[tile] + (list prefix of vertex counts before the record)
+    int lpos = 0;
+    int64_t vpre = 0;
+    for (int u = 0; u < kTilesPerEmitGroup; ++u) {
+      TileInfo& ti = s_tile[u];
+      ti.lbeg = lpos;
+      ti.delta -= vpre;
+      if (ti.T > 0 && ti.nrec >= 0) {
+        lpos += ti.nrec;
+        vpre += ti.T;
+      }
+    }
+    s_tmp[7] = lpos;
+  }
+  __syncthreads();
+  const int nlist = s_tmp[7];
+
+  int64_t carry = 0;  // vertex prefix of the list chunks before this one
+  for (int rb = 0; rb < nlist; rb += kEmitThreads) {
+    const int k = rb + tid;
+    int nv = 0, u = 0;
+    uint32_t* my = s_dat + tid * kRecWords;
+    if (k < nlist) {
+      while (u + 1 < kTilesPerEmitGroup &&
+             !(s_tile[u].T > 0 && s_tile[u].nrec >= 0 && k < s_tile[u].lbeg + s_tile[u].nrec))
+        ++u;
+      const uint32_t* src = P.rec + (s_tile[u].rstart + (k - s_tile[u].lbeg)) * kRecWords;
+#pragma unroll
+      for (int q = 0; q < kRecWords; ++q) my[q] = src[q];
+      nv = s_nv[my[0] & 0xffu];
+    }
+    int tot;
+    const int ex = block_excl_scan(nv, s_tmp, &tot);
+    // record tile and start offset for the vertex pass
+    const uint32_t rec_u = (uint32_t)u;
+    for (int vb = 0; vb < tot; vb += kEmitThreads) {
+      for (int q = 0; q < nv; ++q) {
+        const int j = ex + q - vb;
+        if (j >= 0 && j < kEmitThreads)
+          s_map[j] = ((uint32_t)tid << 8) | (rec_u << 4) | (uint32_t)q;
+      }
+      __syncthreads();
+      if (vb + tid < tot) {
+        const uint32_t e = s_map[tid];
+        const int rk = (int)(e >> 8);
+        const int ut = (int)((e >> 4) & 15u);
+        const int q = (int)(e & 15u);
+        const uint32_t* rc = s_dat + rk * kRecWords;
+        const TileInfo& ti = s_tile[ut];
+        const VolDesc& V = vol_at<BATCH>(P, ti.vi);
+        const int64_t local = t0 + ut - V.tile_begin;
+        const int z = (int)V.divR.div((uint32_t)local);
+        const int64_t p = (local - (int64_t)z * V.R) * kTile + (rc[0] >> 8);
+        const uint32_t y = V.divX.div((uint32_t)p);
+        const int x = (int)((uint32_t)p - y * (uint32_t)V.X);
+        const uint32_t cw[8] = {rc[1], rc[2], rc[3], rc[4], rc[5], rc[6], rc[7], rc[8]};
+        emit_vertex<ELEM>(P, V, s_case, cw, x, (int)y, z, rc[0] & 0xffu, q,
+                          ti.delta + carry + vb + tid);
+      }
+      __syncthreads();
+    }
+    carry += tot;
   }
 
-  // ---- last workgroup resets the ticket/done counters and flips parity
-  if (tid == 0) {
-    const uint32_t prev =
-        __hip_atomic_fetch_add(&P.hdr->done, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    if ((int64_t)prev == P.n_tiles - 1) {
-      __hip_atomic_store(&P.hdr->ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      __hip_atomic_store(&P.hdr->done, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      __hip_atomic_store(&P.hdr->parity, parity ^ 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  // tiles whose records overflowed the quota
+  for (int u = 0; u < kTilesPerEmitGroup; ++u) {
+    const TileInfo ti = s_tile[u];
+    if (ti.T > 0 && ti.nrec < 0) {
+      const int64_t t = t0 + u;
+      const VolDesc& V = vol_at<BATCH>(P, ti.vi);
+      emit_staged<ELEM>(P, V, t, P.voff[t], s_case, s_nv, s_tmp, s_map, s_dat);
+      __syncthreads();
     }
   }
 }
 
-// ---- per-cube classification (independent of the fused kernel) -----------
+// ============================================================ backward
+// Differentiable form (no counterpart in the reference).  For a vertex on
+// edge (a -> b), p = p_a + t (p_b - p_a) with t = -f_a / d, d = f_b - f_a + eps,
+// so dp/df_a = (p_b - p_a) * (-(f_b + eps) / d^2) and dp/df_b = (p_b - p_a) *
+// (f_a / d^2).  BW1: one thread per active cube reduces g . dp/df over the
+// cube's vertices, in slot order, into 8 per-corner sums.  BW2: every voxel
+// sums the corner sums of the (at most 8) active cubes it is a corner of, in
+// corner order.  No atomics, so the result is bitwise reproducible.
+template <bool BATCH>
+__global__ __launch_bounds__(kEmitThreads) void contrib_kernel(ExtractParams P, const float* gpos,
+                                                               float* contrib) {
+  __shared__ uint64_t s_case[256];
+  __shared__ uint8_t s_nv[256];
+  __shared__ int s_tmp[8];
+  __shared__ TileInfo s_tile[kTilesPerEmitGroup];
+  const int tid = threadIdx.x;
+  const int64_t t0 = (int64_t)blockIdx.x * kTilesPerEmitGroup;
+  s_case[tid] = dCase[tid];
+  s_nv[tid] = dNumVerts[tid];
+  if (tid < kTilesPerEmitGroup) {
+    const int64_t t = t0 + tid;
+    TileInfo ti{};
+    if (t < P.n_tiles) {
+      ti.T = P.tileT[t];
+      ti.nrec = P.tile_nrec[t];
+      ti.rstart = P.tile_rec[t];
+      ti.delta = P.voff[t];
+      ti.vi = BATCH ? volume_of_tile(P, t) : 0;
+    }
+    s_tile[tid] = ti;
+  }
+  __syncthreads();
+  if (tid == 0) {
+    int lpos = 0;
+    int64_t vpre = 0;
+    for (int u = 0; u < kTilesPerEmitGroup; ++u) {
+      TileInfo& ti = s_tile[u];
+      ti.lbeg = lpos;
+      ti.delta -= vpre;
+      if (ti.T > 0 && ti.nrec >= 0) {
+        lpos += ti.nrec;
+        vpre += ti.T;
+      }
+    }
+    s_tmp[7] = lpos;
+  }
+  __syncthreads();
+  const int nlist = s_tmp[7];
+  int64_t carry = 0;
+  for (int rb = 0; rb < nlist; rb += kEmitThreads) {
+    const int k = rb + tid;
+    int nv = 0, u = 0;
+    uint32_t rc[kRecWords] = {};
+    int64_t ri = 0;
+    if (k < nlist) {
+      while (u + 1 < kTilesPerEmitGroup &&
+             !(s_tile[u].T > 0 && s_tile[u].nrec >= 0 && k < s_tile[u].lbeg + s_tile[u].nrec))
+        ++u;
+      ri = s_tile[u].rstart + (k - s_tile[u].lbeg);
+      const uint32_t* src = P.rec + ri * kRecWords;
+#pragma unroll
+      for (int q = 0; q < kRecWords; ++q) rc[q] = src[q];
+      nv = s_nv[rc[0] & 0xffu];
+    }
+    int tot;
+    const int ex = block_excl_scan(nv, s_tmp, &tot);
+    if (k < nlist) {
+      const TileInfo& ti = s_tile[u];
+      const VolDesc& V = vol_at<BATCH>(P, ti.vi);
+      const float h = V.h;
+      const uint32_t ci = rc[0] & 0xffu;
+      const int64_t first = ti.delta + carry + ex;
+      float c[8];
+#pragma unroll
+      for (int q = 0; q < 8; ++q) c[q] = 0.0f;
+      for (int q = 0; q < nv; ++q) {
+        const int m = 3 * (q / 3) + 2 - (q % 3);
+        const int e = (int)((s_case[ci] >> (4 * m)) & 0xfu);
+        const int ca = (int)((kEdgeA >> (3 * e)) & 7u);
+        const int cb = (int)((kEdgeB >> (3 * e)) & 7u);
+        float fa = 0.f, fb = 0.f;
+#pragma unroll
+        for (int w = 0; w < 8; ++w) {
+          fa = ca == w ? __uint_as_float(rc[1 + w]) : fa;
+          fb = cb == w ? __uint_as_float(rc[1 + w]) : fb;
+        }
+        // p_b - p_a is one signed grid step along one axis (same fp32 values
+        // as the forward's node coordinates)
+        const float dpx = (float)(corner_dx(cb) - corner_dx(ca)) * h;
+        const float dpy = (float)(corner_dy(cb) - corner_dy(ca)) * h;
+        const float dpz = (float)(corner_dz(cb) - corner_dz(ca)) * h;
+        const float* g = gpos + 3 * (first + q);
+        const float sdot = g[0] * dpx + g[1] * dpy + g[2] * dpz;
+        const float d = fb - fa + 1e-15f;
+        const float inv = 1.0f / (d * d);
+        const float ga = sdot * (-(fb + 1e-15f) * inv);
+        const float gb = sdot * (fa * inv);
+#pragma unroll
+        for (int w = 0; w < 8; ++w) c[w] += (ca == w ? ga : 0.0f) + (cb == w ? gb : 0.0f);
+      }
+      float4* dst = reinterpret_cast<float4*>(contrib + ri * 8);
+      dst[0] = make_float4(c[0], c[1], c[2], c[3]);
+      dst[1] = make_float4(c[4], c[5], c[6], c[7]);
+    }
+    carry += tot;
+  }
+}
+
+template <bool BATCH>
+__global__ __launch_bounds__(256) void gather_kernel(ExtractParams P, const float* contrib,
+                                                     float* gvol, int vi) {
+  const VolDesc& V = vol_at<BATCH>(P, vi);
+  const int X = V.X, Y = V.Y, Z = V.Z;
+  const int64_t XY = (int64_t)X * Y;
+  const int64_t N = XY * Z;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < N;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const int z = (int)(i / XY);
+    const uint32_t rem = (uint32_t)(i - (int64_t)z * XY);
+    const uint32_t y = V.divX.div(rem);
+    const int x = (int)(rem - y * (uint32_t)X);
+    float g = 0.0f;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      // cube whose corner k is this voxel
+      const int cx = x - corner_dx(k), cy = (int)y - corner_dy(k), cz = z - corner_dz(k);
+      if (cx < 0 || cy < 0 || cz < 0 || cx > X - 2 || cy > Y - 2 || cz > Z - 2) continue;
+      const int64_t p = (int64_t)cy * X + cx;
+      const int64_t t = V.tile_begin + (int64_t)cz * V.R + (p / kTile);
+      const int bit = (int)(p % kTile);
+      const uint64_t* mk = P.masks + t * kCPW;
+      const uint64_t w = mk[bit >> 6];
+      if (!((w >> (bit & 63)) & 1ull)) continue;
+      int rank = __popcll(w & ((1ull << (bit & 63)) - 1ull));
+      for (int j = 0; j < (bit >> 6); ++j) rank += __popcll(mk[j]);
+      g += contrib[(P.tile_rec[t] + rank) * 8 + k];
+    }
+    gvol[i] = g;
+  }
+}
+
+// ---- per-cube classification (independent of the pipeline) ---------------
 template <int ELEM>
 __global__ __launch_bounds__(256) void classify_kernel(const uint32_t* __restrict__ vol, int X,
                                                        int Y, int Z, uint8_t* ci_out,
@@ -430,10 +843,10 @@ __global__ __launch_bounds__(256) void classify_kernel(const uint32_t* __restric
   const int64_t N = (int64_t)X * Y * Z;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < N;
        i += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t r = i / X;
-    const int x = (int)(i - r * X);
-    const int z = (int)(r / Y);
-    const int y = (int)(r - (int64_t)z * Y);
+    const int64_t rr = i / X;
+    const int x = (int)(i - rr * X);
+    const int z = (int)(rr / Y);
+    const int y = (int)(rr - (int64_t)z * Y);
     uint32_t ci = 0, nv = 0;
     if (x < X - 1 && y < Y - 1 && z < Z - 1) {
       const int64_t XY = (int64_t)X * Y;
@@ -447,53 +860,151 @@ __global__ __launch_bounds__(256) void classify_kernel(const uint32_t* __restric
   }
 }
 
-// ---- launch plumbing -------------------------------------------------------
-template <int G, bool BATCH, int ELEM>
-int launch_extract(const ExtractParams& P, int lds_bytes, hipStream_t s) {
-  auto k = extract_kernel<G, BATCH, ELEM>;
-  MCX_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(k),
-                              hipFuncAttributeMaxDynamicSharedMemorySize, lds_bytes));
-  hipLaunchKernelGGL(k, dim3((unsigned)P.n_tiles), dim3(kThreads), lds_bytes, s, P);
-  MCX_HIP(hipGetLastError());
+// ================================================================ host side
+size_t al256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+int64_t tiles_per_plane(int X, int Y) { return ((int64_t)X * Y + kTile - 1) / kTile; }
+
+int64_t tiles_of(int X, int Y, int Z) {
+  if (X < 2 || Y < 2 || Z < 2) return 0;
+  return tiles_per_plane(X, Y) * (Z - 1);
+}
+
+int64_t cubes_of(int X, int Y, int Z) {
+  return (X > 1 && Y > 1 && Z > 1) ? (int64_t)(X - 1) * (Y - 1) * (Z - 1) : 0;
+}
+
+size_t fixed_ws_bytes(int64_t n_tiles) {
+  const int64_t nscan = n_tiles > 0 ? (n_tiles + kScanItems - 1) / kScanItems : 1;
+  return kWsHeader + al256(16 * (size_t)nscan) + 2 * al256(4 * (size_t)n_tiles) +
+         2 * al256(8 * (size_t)n_tiles) + al256(8 * (size_t)kCPW * (size_t)n_tiles);
+}
+
+int layout_ws(void* d_ws, size_t ws_bytes, int64_t n_tiles, ExtractParams& P) {
+  const size_t fixed = fixed_ws_bytes(n_tiles);
+  if (!d_ws || ws_bytes < fixed) return MCX_ERR_WORKSPACE;
+  char* b = static_cast<char*>(d_ws);
+  P.hdr = reinterpret_cast<WsHeader*>(b);
+  size_t o = kWsHeader;
+  P.cap_desc = n_tiles > 0 ? (n_tiles + kScanItems - 1) / kScanItems : 1;
+  P.desc = reinterpret_cast<uint64_t*>(b + o);
+  o += al256(16 * (size_t)P.cap_desc);
+  P.tileT = reinterpret_cast<int32_t*>(b + o);
+  o += al256(4 * (size_t)n_tiles);
+  P.tile_nrec = reinterpret_cast<int32_t*>(b + o);
+  o += al256(4 * (size_t)n_tiles);
+  P.tile_rec = reinterpret_cast<int64_t*>(b + o);
+  o += al256(8 * (size_t)n_tiles);
+  P.voff = reinterpret_cast<int64_t*>(b + o);
+  o += al256(8 * (size_t)n_tiles);
+  P.masks = reinterpret_cast<uint64_t*>(b + o);
+  o += al256(8 * (size_t)kCPW * (size_t)n_tiles);
+  P.rec = reinterpret_cast<uint32_t*>(b + o);
+  P.rec_cap = (int64_t)((ws_bytes - o) / (kRecWords * 4));
   return MCX_OK;
 }
 
-constexpr int kMaxLds = 160 * 1024;
-
-// Words per staged range: L + X + 1 voxels, plus up to 3 words of alignment
-// shift in front, plus the rounding of the last 16-byte chunk (the staging
-// loop writes indices up to len + shift + 2 <= L + X + 6); reads stay below
-// L + X + 4.
-int lds_bytes_for(int G, int maxX) {
-  const int L = kThreads * 4 * G;
-  const int need = L + maxX + 1 + 3 + 3;
-  const int plane = ((need + 3) / 4) * 4;
-  return (kLdsPlanes + 2 * plane) * 4;
+// Launch K1, K2, (vbase), K3 for prepared parameters.
+int run_extract(ExtractParams& P, int elem, bool batch, int maxX, hipStream_t s) {
+  const unsigned k1_blocks = (unsigned)((P.ncols + 3) / 4);
+  const unsigned k2_blocks = (unsigned)((P.n_tiles + kScanItems - 1) / kScanItems);
+  const unsigned k3_blocks =
+      (unsigned)((P.n_tiles + kTilesPerEmitGroup - 1) / kTilesPerEmitGroup);
+  P.emit_plane_words = (int)(((int64_t)kTile + maxX + 1 + 3) & ~3ll);
+  const int rec_words = kEmitThreads * kRecWords;
+  const int dat_words = rec_words > 2 * P.emit_plane_words ? rec_words : 2 * P.emit_plane_words;
+  const int k3_lds = (kEmDat + dat_words) * 4;
+  if (k3_lds > kMaxLds) return MCX_ERR_UNSUPPORTED;
+#define MCX_LAUNCH_SET(E, B)                                                                     \
+  do {                                                                                           \
+    hipLaunchKernelGGL((count_kernel<E, B>), dim3(k1_blocks), dim3(256), 0, s, P);              \
+    MCX_HIP(hipGetLastError());                                                                  \
+    hipLaunchKernelGGL(scan_kernel, dim3(k2_blocks), dim3(256), 0, s, P);                        \
+    MCX_HIP(hipGetLastError());                                                                  \
+    if (B) {                                                                                     \
+      hipLaunchKernelGGL(vbase_kernel, dim3((unsigned)((P.nvol + 256) / 256)), dim3(256), 0, s, \
+                         P);                                                                     \
+      MCX_HIP(hipGetLastError());                                                                \
+    }                                                                                            \
+    MCX_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(emit_kernel<E, B>),               \
+                                hipFuncAttributeMaxDynamicSharedMemorySize, k3_lds));            \
+    hipLaunchKernelGGL((emit_kernel<E, B>), dim3(k3_blocks), dim3(kEmitThreads), k3_lds, s, P);  \
+    MCX_HIP(hipGetLastError());                                                                  \
+  } while (0)
+  if (elem == MCX_ELEM_SHORT2) {
+    if (batch) MCX_LAUNCH_SET(MCX_ELEM_SHORT2, true);
+    else MCX_LAUNCH_SET(MCX_ELEM_SHORT2, false);
+  } else {
+    if (batch) MCX_LAUNCH_SET(MCX_ELEM_F32, true);
+    else MCX_LAUNCH_SET(MCX_ELEM_F32, false);
+  }
+#undef MCX_LAUNCH_SET
+  return MCX_OK;
 }
 
-int64_t tiles_for(int X, int Y, int Z, int L) {
-  if (X < 2 || Y < 2 || Z < 2) return 0;
-  const int64_t ncl = (int64_t)X * Y * (Z - 1);
-  return (ncl + L - 1) / L;
+VolDesc make_vol(const void* d_vol, int X, int Y, int Z, float h, int16_t* ver_voxel,
+                 int64_t tile_begin) {
+  VolDesc v{};
+  v.vol = static_cast<const uint32_t*>(d_vol);
+  v.X = X;
+  v.Y = Y;
+  v.Z = Z;
+  v.h = h;
+  v.R = (int)tiles_per_plane(X, Y);
+  v.tile_begin = tile_begin;
+  v.ver_voxel = ver_voxel;
+  v.divX = FastDiv((uint32_t)X);
+  v.divR = FastDiv((uint32_t)v.R);
+  return v;
+}
+
+// Single-volume launch parameters: workspace layout, the volume, and the
+// column decomposition (R in-plane ranges x S slabs, about 4096 waves).
+// Depends only on (X, Y, Z, ws_bytes), so a backward call sees the forward's
+// record slots.
+int prepare_single(ExtractParams& P, const void* d_vol, int X, int Y, int Z, float h,
+                   int16_t* ver_voxel, void* d_ws, size_t ws_bytes) {
+  const int64_t n_tiles = tiles_of(X, Y, Z);
+  const int rc = layout_ws(d_ws, ws_bytes, n_tiles, P);
+  if (rc) return rc;
+  P.vol0 = make_vol(d_vol, X, Y, Z, h, ver_voxel, 0);
+  P.vols = nullptr;
+  P.cols = nullptr;
+  P.nvol = 1;
+  P.n_tiles = n_tiles;
+  const int64_t R = P.vol0.R;
+  int64_t S = (4096 + R - 1) / R;
+  if (S > Z - 1) S = Z - 1;
+  P.slab = (int)((Z - 1 + S - 1) / S);
+  S = (Z - 1 + P.slab - 1) / P.slab;
+  P.ncols = R * S;
+  P.quota = P.rec_cap / P.ncols;
+  if (P.quota > (int64_t)kTile * P.slab) P.quota = (int64_t)kTile * P.slab;
+  return MCX_OK;
+}
+
+int read_header(void* d_ws, WsHeader* hh, hipStream_t s) {
+  MCX_HIP(hipMemcpyAsync(hh, d_ws, sizeof(*hh), hipMemcpyDeviceToHost, s));
+  MCX_HIP(hipStreamSynchronize(s));
+  return MCX_OK;
 }
 
 }  // namespace
-
-// Default tile: G = 4 (4096 cubes per workgroup), smaller when X is so wide
-// that the staged ranges would not fit in LDS.
-static int pick_groups(int maxX) {
-  for (int G = 4; G >= 1; G >>= 1)
-    if (lds_bytes_for(G, maxX) <= kMaxLds) return G;
-  return 0;
-}
-
 }  // namespace mcx
 
 using namespace mcx;
 
+extern "C" size_t mcx_extract_workspace_size_ex(int X, int Y, int Z, int64_t records) {
+  if (records < 0) records = 0;
+  return fixed_ws_bytes(tiles_of(X, Y, Z)) + (size_t)records * kRecWords * 4;
+}
+
+// Default record scratch: one record per 16 cubes (6.25 % of the cubes of a
+// column may be active before its tiles fall back to the staged path).
 extern "C" size_t mcx_extract_workspace_size(int X, int Y, int Z) {
-  // smallest tile (G = 1) gives the most tiles
-  return ws_bytes_for_tiles(tiles_for(X, Y, Z, kThreads * 4));
+  int64_t rec = cubes_of(X, Y, Z) / 16;
+  if (rec < 65536) rec = 65536;
+  return mcx_extract_workspace_size_ex(X, Y, Z, rec);
 }
 
 extern "C" int mcx_extract(const void* d_vol, int elem, int X, int Y, int Z, float h,
@@ -503,7 +1014,7 @@ extern "C" int mcx_extract(const void* d_vol, int elem, int X, int Y, int Z, flo
   if (!d_vol || X < 1 || Y < 1 || Z < 1 || cap < 0) return MCX_ERR_ARG;
   if (elem != MCX_ELEM_SHORT2 && elem != MCX_ELEM_F32) return MCX_ERR_ARG;
   if (elem == MCX_ELEM_F32 && (d_vv || d_ver_voxel)) return MCX_ERR_ARG;
-  if ((int64_t)X * Y * Z > (int64_t)1 << 40) return MCX_ERR_UNSUPPORTED;
+  if ((int64_t)X * Y >= ((int64_t)1 << 30)) return MCX_ERR_UNSUPPORTED;  // plane < 4 GiB
   hipStream_t s = as_stream(stream);
   int rc = MCX_OK;
   const int64_t N = (int64_t)X * Y * Z;
@@ -511,10 +1022,7 @@ extern "C" int mcx_extract(const void* d_vol, int elem, int X, int Y, int Z, flo
     rc = fill_bytes_zero(d_ver_voxel, (size_t)N * sizeof(int16_t), s);
     if (rc) return rc;
   }
-  const int G = pick_groups(X);
-  if (G == 0) return MCX_ERR_UNSUPPORTED;
-  const int L = kThreads * 4 * G;
-  const int64_t n_tiles = tiles_for(X, Y, Z, L);
+  const int64_t n_tiles = tiles_of(X, Y, Z);
   if (n_tiles == 0) {
     if (d_count) {
       rc = fill_bytes_zero(d_count, sizeof(int64_t), s);
@@ -526,51 +1034,16 @@ extern "C" int mcx_extract(const void* d_vol, int elem, int X, int Y, int Z, flo
     }
     return MCX_OK;
   }
-  if (!d_ws) return MCX_ERR_WORKSPACE;
-  const int64_t cap_tiles = ws_cap_tiles(ws_bytes);
-  if (cap_tiles < n_tiles) return MCX_ERR_WORKSPACE;
-  if (n_tiles > 0xffffffffll) return MCX_ERR_UNSUPPORTED;
-
   ExtractParams P{};
-  P.single.vol = static_cast<const uint32_t*>(d_vol);
-  P.single.X = X;
-  P.single.Y = Y;
-  P.single.Z = Z;
-  P.single.h = h;
-  P.single.tile_begin = 0;
-  P.single.ver_voxel = d_ver_voxel;
-  P.single.divX = FastDiv((uint32_t)X);
-  P.single.divY = FastDiv((uint32_t)Y);
-  P.descs = nullptr;
-  P.nvol = 1;
-  P.n_tiles = n_tiles;
-  P.L = L;
-  const int lds = lds_bytes_for(G, X);
-  P.lds_plane = (lds / 4 - kLdsPlanes) / 2;
+  rc = prepare_single(P, d_vol, X, Y, Z, h, d_ver_voxel, d_ws, ws_bytes);
+  if (rc) return rc;
   P.vv = d_vv;
   P.pos = d_pos;
   P.v2vox = d_v2vox;
   P.cap = cap;
   P.d_count = d_count;
   P.vbase = nullptr;
-  P.tile_excl = nullptr;
-  P.hdr = static_cast<WsHeader*>(d_ws);
-  P.desc = reinterpret_cast<uint64_t*>(static_cast<char*>(d_ws) + kWsHeader);
-  P.cap_tiles = cap_tiles;
-
-  if (elem == MCX_ELEM_SHORT2) {
-    switch (G) {
-      case 4: rc = launch_extract<4, false, MCX_ELEM_SHORT2>(P, lds, s); break;
-      case 2: rc = launch_extract<2, false, MCX_ELEM_SHORT2>(P, lds, s); break;
-      default: rc = launch_extract<1, false, MCX_ELEM_SHORT2>(P, lds, s); break;
-    }
-  } else {
-    switch (G) {
-      case 4: rc = launch_extract<4, false, MCX_ELEM_F32>(P, lds, s); break;
-      case 2: rc = launch_extract<2, false, MCX_ELEM_F32>(P, lds, s); break;
-      default: rc = launch_extract<1, false, MCX_ELEM_F32>(P, lds, s); break;
-    }
-  }
+  rc = run_extract(P, elem, false, X, s);
   if (rc) return rc;
   if (h_count) {
     WsHeader hh;
@@ -597,6 +1070,79 @@ extern "C" int mcx_classify(const void* d_vol, int elem, int X, int Y, int Z, ui
   else
     hipLaunchKernelGGL(classify_kernel<MCX_ELEM_F32>, dim3((unsigned)blocks), dim3(256), 0, s,
                        static_cast<const uint32_t*>(d_vol), X, Y, Z, d_ci, d_nv);
+  MCX_HIP(hipGetLastError());
+  return MCX_OK;
+}
+
+// ---- differentiable form (no counterpart in the reference) ---------------
+extern "C" size_t mcx_backward_scratch_size(int X, int Y, int Z, size_t ws_bytes) {
+  ExtractParams P{};
+  const int64_t n_tiles = tiles_of(X, Y, Z);
+  if (ws_bytes < fixed_ws_bytes(n_tiles)) return 0;
+  char dummy;
+  if (layout_ws(&dummy, ws_bytes, n_tiles, P)) return 0;
+  return (size_t)P.rec_cap * 8 * sizeof(float);
+}
+
+extern "C" int mcx_extract_diff(const float* d_vol, int X, int Y, int Z, float h, float* d_pos,
+                                int64_t cap, int64_t* d_count, int64_t* h_count, void* d_ws,
+                                size_t ws_bytes, void* stream) {
+  if (!d_vol || X < 1 || Y < 1 || Z < 1 || cap < 0) return MCX_ERR_ARG;
+  if ((int64_t)X * Y >= ((int64_t)1 << 30)) return MCX_ERR_UNSUPPORTED;
+  hipStream_t s = as_stream(stream);
+  const int64_t n_tiles = tiles_of(X, Y, Z);
+  if (n_tiles == 0) {
+    if (d_count && fill_bytes_zero(d_count, sizeof(int64_t), s)) return MCX_ERR_HIP;
+    if (h_count) {
+      MCX_HIP(hipStreamSynchronize(s));
+      *h_count = 0;
+    }
+    return MCX_OK;
+  }
+  ExtractParams P{};
+  int rc = prepare_single(P, d_vol, X, Y, Z, h, nullptr, d_ws, ws_bytes);
+  if (rc) return rc;
+  P.pos = d_pos;
+  P.cap = cap;
+  P.d_count = d_count;
+  P.write_masks = true;
+  rc = run_extract(P, MCX_ELEM_F32, false, X, s);
+  if (rc) return rc;
+  if (h_count) {
+    WsHeader hh;
+    rc = read_header(d_ws, &hh, s);
+    if (rc) return rc;
+    *h_count = hh.count;
+    if (hh.error) return MCX_ERR_INTERNAL;
+    if (hh.dense_last) return MCX_ERR_WORKSPACE;  // backward needs every record
+    if (hh.count > cap) return MCX_ERR_CAPACITY;
+  }
+  return MCX_OK;
+}
+
+extern "C" int mcx_extract_backward(const float* d_vol, int X, int Y, int Z, float h,
+                                    const float* d_grad_pos, float* d_grad_vol, void* d_ws,
+                                    size_t ws_bytes, void* d_scratch, size_t scratch_bytes,
+                                    void* stream) {
+  if (!d_vol || !d_grad_vol || X < 1 || Y < 1 || Z < 1) return MCX_ERR_ARG;
+  hipStream_t s = as_stream(stream);
+  const int64_t N = (int64_t)X * Y * Z;
+  const int64_t n_tiles = tiles_of(X, Y, Z);
+  if (n_tiles == 0) return fill_bytes_zero(d_grad_vol, (size_t)N * sizeof(float), s);
+  if (!d_grad_pos) return MCX_ERR_ARG;
+  ExtractParams P{};
+  int rc = prepare_single(P, d_vol, X, Y, Z, h, nullptr, d_ws, ws_bytes);
+  if (rc) return rc;
+  if (!d_scratch || scratch_bytes < (size_t)P.rec_cap * 8 * sizeof(float)) return MCX_ERR_WORKSPACE;
+  float* contrib = static_cast<float*>(d_scratch);
+  const unsigned g1 = (unsigned)((n_tiles + kTilesPerEmitGroup - 1) / kTilesPerEmitGroup);
+  hipLaunchKernelGGL(contrib_kernel<false>, dim3(g1), dim3(kEmitThreads), 0, s, P, d_grad_pos,
+                     contrib);
+  MCX_HIP(hipGetLastError());
+  int64_t g2 = (N + 255) / 256;
+  if (g2 > 1 << 20) g2 = 1 << 20;
+  hipLaunchKernelGGL(gather_kernel<false>, dim3((unsigned)g2), dim3(256), 0, s, P, contrib,
+                     d_grad_vol, 0);
   MCX_HIP(hipGetLastError());
   return MCX_OK;
 }

@@ -31,28 +31,22 @@ int fill_u32(void* d_ptr, uint32_t value, size_t n_words, hipStream_t s);
 int fill_bytes_zero(void* d_ptr, size_t n_bytes, hipStream_t s);
 
 // ------------------------------------------------------ workspace layout
-// [0, 256)   header: u32 ticket, u32 done, u32 parity, u32 error, i64 count
+// [0, 256)   header: u32 dense_acc, u32 dense_last, u32 parity, u32 error, i64 count
 // [256, ...) u64 descriptors, two arrays of cap_tiles entries (by parity)
 constexpr size_t kWsHeader = 256;
 struct WsHeader {
-  uint32_t ticket;
-  uint32_t done;
+  uint32_t dense_acc;   // tiles of the running call that took the staged path
+  uint32_t dense_last;  // the same count for the last completed call
   uint32_t parity;
   uint32_t error;
   int64_t count;
 };
-inline size_t ws_bytes_for_tiles(int64_t tiles) {
-  return kWsHeader + 2 * sizeof(uint64_t) * (size_t)(tiles > 0 ? tiles : 1);
-}
-inline int64_t ws_cap_tiles(size_t bytes) {
-  return bytes < kWsHeader + 16 ? 0 : (int64_t)((bytes - kWsHeader) / 16);
-}
 
 // Look-back descriptor: status in the top two bits, value below.
 constexpr uint64_t kStatAgg = 1ull << 62;
 constexpr uint64_t kStatInc = 2ull << 62;
 constexpr uint64_t kValMask = (1ull << 62) - 1;
-constexpr int kSpinLimit = 1 << 22;
+constexpr int kSpinLimit = 1 << 20;
 
 // ------------------------------------------------- division by invariants
 // n / d for 32-bit unsigned n via one mulhi (Granlund-Montgomery).
@@ -125,41 +119,54 @@ __device__ __forceinline__ void st_agent(uint64_t* p, uint64_t v) {
   __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Decoupled look-back executed by one full wave for tile t (t > 0):
-// returns the sum of all tiles before t.  Predecessors publish their
-// aggregate before they look back themselves, so every wait ends; a wait
-// beyond kSpinLimit sets *err and counts the tile as 0 so the wave exits.
-__device__ __forceinline__ int64_t lookback(const uint64_t* desc, int64_t t,
-                                            uint32_t* err) {
+// Decoupled look-back executed by one full wave for tile t (t > 0): returns
+// the sum of all tiles before t.  Lane l inspects descriptors
+// j - l*W - w (w < W), so one round trip to memory covers 64*W tiles and the
+// chain of inclusive prefixes advances faster than tiles are issued.
+// Tiles are numbered in dispatch order and publish their aggregate before
+// they look back themselves, so every wait ends once the predecessors run; a
+// wait beyond kSpinLimit sets *err and counts the tile as inclusive 0, so the
+// wave always exits.
+template <int W>
+__device__ __forceinline__ int64_t lookback(const uint64_t* desc, int64_t t, uint32_t* err) {
   const int lane = lane_id();
   int64_t acc = 0;
   int64_t j = t - 1;
   while (true) {
-    const int64_t idx = j - lane;
-    uint64_t d = kStatInc;  // virtual inclusive 0 before tile 0
-    if (idx >= 0) {
+    uint64_t d[W];
+#pragma unroll
+    for (int w = 0; w < W; ++w) {
+      const int64_t idx = j - (int64_t)lane * W - w;
+      d[w] = idx >= 0 ? ld_agent(desc + idx) : kStatInc;  // virtual inclusive 0 before tile 0
+    }
+#pragma unroll
+    for (int w = 0; w < W; ++w) {
       int spins = 0;
-      d = ld_agent(desc + idx);
-      while ((d >> 62) == 0) {
+      while ((d[w] >> 62) == 0) {
         if (++spins >= kSpinLimit) {
           atomicOr(err, 1u);
-          d = kStatInc;
+          d[w] = kStatInc;
           break;
         }
-        __builtin_amdgcn_s_sleep(2);
-        d = ld_agent(desc + idx);
+        __builtin_amdgcn_s_sleep(1);
+        d[w] = ld_agent(desc + (j - (int64_t)lane * W - w));
       }
     }
-    const uint64_t incmask = __ballot((d >> 62) == 2);
-    uint64_t v = d & kValMask;
-    if (incmask) {
-      const int first = __ffsll((unsigned long long)incmask) - 1;
-      if (lane > first) v = 0;
-      acc += (int64_t)wave_sum(v);
-      break;
+    int first = W;  // nearest inclusive descriptor of this lane
+#pragma unroll
+    for (int w = W - 1; w >= 0; --w)
+      if ((d[w] >> 62) == 2) first = w;
+    const uint64_t incmask = __ballot(first < W);
+    const int fl = incmask ? __ffsll((unsigned long long)incmask) - 1 : kWave;
+    uint64_t v = 0;
+#pragma unroll
+    for (int w = 0; w < W; ++w) {
+      const bool take = lane < fl || (lane == fl && w <= first);
+      v += take ? (d[w] & kValMask) : 0;
     }
     acc += (int64_t)wave_sum(v);
-    j -= kWave;
+    if (incmask) break;
+    j -= (int64_t)kWave * W;
   }
   return acc;
 }

@@ -56,13 +56,15 @@ def volume_kind(vol):
 class Workspace:
     """Caller-owned device workspace for the look-back state (one per stream)."""
 
-    def __init__(self, device=None, nbytes=0):
+    def __init__(self, device=None, nbytes=0, grow=True):
         self.device = torch.device(device if device is not None else "cuda")
         self.buf = None
+        self.grow = True
         self.ensure(nbytes)
+        self.grow = grow
 
     def ensure(self, nbytes):
-        if self.buf is not None and self.buf.numel() >= nbytes:
+        if self.buf is not None and (self.buf.numel() >= nbytes or not self.grow):
             return self
         nbytes = max(int(nbytes), 4096)
         self.buf = torch.empty(nbytes, dtype=torch.uint8, device=self.device)

@@ -150,16 +150,17 @@ def kinfu_scene(nobj=40, extent=3.0, seed=20261202):
     Returns float32 array (nobj, 8): kind (0 sphere, 1 box), cx, cy, cz,
     r (sphere radius) or half sizes hx, hy, hz (box), padding."""
     rng = np.random.default_rng(seed)
+    s = extent / 3.0
     objs = np.zeros((nobj, 8), np.float32)
     for i in range(nobj):
         kind = rng.integers(0, 2)
-        c = rng.uniform(0.6, extent - 0.6, size=3)
+        c = rng.uniform(0.2 * extent, 0.8 * extent, size=3)
         objs[i, 0] = kind
         objs[i, 1:4] = c
         if kind == 0:
-            objs[i, 4] = rng.uniform(0.08, 0.3)
+            objs[i, 4] = rng.uniform(0.08, 0.3) * s
         else:
-            objs[i, 4:7] = rng.uniform(0.06, 0.25, size=3)
+            objs[i, 4:7] = rng.uniform(0.06, 0.25, size=3) * s
     return objs
 
 

@@ -120,3 +120,22 @@ def test_full_size_sphere_512(dev):
     vol, h = synth.sphere_volume(512)
     n = check_against_oracle(vol, h, dev, want=("vv",))
     assert n > 2_000_000
+
+
+@pytest.mark.parametrize("records", [0, 200])
+def test_record_overflow_uses_staged_path(dev, records):
+    # with no (or little) record scratch every (or most) tile re-reads its
+    # voxel ranges from the volume; the output must not change
+    from marching_cube_amd import _capi
+    lib = _capi.load()
+    for shape in [(20, 33, 47), (9, 64, 130)]:
+        vol = synth.noise_volume(shape, seed=7, p_zero_w=0.1)
+        Z, Y, X = shape
+        ws = mesh.Workspace(dev, lib.mcx_extract_workspace_size_ex(X, Y, Z, records), grow=False)
+        ref = O.extract(vol, 0.01, want=("vv", "v2vox", "ver_voxel"))
+        v = to_dev(vol, dev)
+        r = mesh.extract(v, 0.01, want=("vv", "v2vox", "ver_voxel"), workspace=ws)
+        assert r.count == ref["n"]
+        assert r.vv.cpu().numpy().tobytes() == ref["vv"].view(np.int32).reshape(-1, 7).tobytes()
+        assert r.v2vox.cpu().numpy().tobytes() == ref["v2vox"].tobytes()
+        assert r.ver_voxel.cpu().numpy().tobytes() == ref["ver_voxel"].tobytes()
