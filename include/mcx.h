@@ -90,6 +90,33 @@ int mcx_extract(const void* d_vol, int elem, int X, int Y, int Z, float h,
 int mcx_classify(const void* d_vol, int elem, int X, int Y, int Z,
                  uint8_t* d_ci, uint8_t* d_nv, void* stream);
 
+/* ---- batched extraction (no counterpart in the reference) --------------
+ * A plan describes nvol volumes (dims_xyz: X,Y,Z per volume; h: grid size per
+ * volume; d_vols: device pointer per volume; all of one elem kind).  It is
+ * built once (device allocation + upload) and reused.  mcx_extract_batched
+ * extracts every volume in one launch sequence (count, scan, emit) into one
+ * concatenated output: volume b's vertices are [d_vbase[b], d_vbase[b+1]),
+ * d_vbase has nvol+1 entries, vox_pos is relative to the own volume.  It
+ * only launches kernels (no allocation, no memset), so it can be captured in
+ * a hipGraph.  ver_voxel is not produced in batched form. */
+typedef struct mcx_batch mcx_batch;
+int mcx_batch_create(mcx_batch** out, int nvol, const int* dims_xyz, const float* h,
+                     const void* const* d_vols, int elem);
+void mcx_batch_destroy(mcx_batch* plan);
+int64_t mcx_batch_tiles(const mcx_batch* plan);
+/* records < 0: default record scratch (one per 16 cubes) */
+size_t mcx_batch_workspace_size(const mcx_batch* plan, int64_t records);
+int mcx_extract_batched(const mcx_batch* plan, mcx_vertex_voxel* d_vv, float* d_pos,
+                        int32_t* d_v2vox, int64_t cap, int64_t* d_vbase, int64_t* d_count,
+                        int64_t* h_count, void* d_ws, size_t ws_bytes, void* stream);
+
+/* ---- per-vertex normals (replaces d_voxel_extract_surface_normals,
+ *      cu_kinect_fusion_v1.cu:3146-3238).  d_pos: n positions with a stride
+ *      of stride_floats floats (7 for vertex_voxel records, 3 for float3);
+ *      d_normals: n float3.  SHORT2 volumes. */
+int mcx_normals(const void* d_vol, int X, int Y, int Z, float h, const float* d_pos,
+                int64_t stride_floats, int64_t n, float* d_normals, void* stream);
+
 /* ---- differentiable extraction (no counterpart in the reference) -------
  * mcx_extract_diff: extraction of an fp32 field (d_pos: float3 positions, same
  * order and values as mcx_extract with d_pos) that also keeps in the
@@ -127,6 +154,12 @@ int mcx_integrate(void* d_vol, int X, int Y, int Z, float h, float trunc, int ma
                   const float* intr, const float* d_poses, const uint16_t* d_depth,
                   float depth_scale, int W, int H, int K, void* d_ws, size_t ws_bytes,
                   void* stream);
+/* Same with float depth frames (metres = value * depth_scale), the form the
+ * reference's integration takes (float *d_depth_ptr, :534). */
+int mcx_integrate_f32(void* d_vol, int X, int Y, int Z, float h, float trunc, int max_w,
+                      const float* intr, const float* d_poses, const float* d_depth,
+                      float depth_scale, int W, int H, int K, void* d_ws, size_t ws_bytes,
+                      void* stream);
 
 /* ---- synthetic depth frames (test/benchmark input; no reference
  *      counterpart: the reference reads OpenNI frames).  Ray-casts nobj

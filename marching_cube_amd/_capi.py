@@ -40,6 +40,12 @@ SIGNATURES = {
     "mcx_extract": (_i32, [_vp, _i32, _i32, _i32, _i32, _f32, _vp, _vp, _vp, _vp, _i64, _vp,
                            _vp, _vp, _sz, _vp]),
     "mcx_classify": (_i32, [_vp, _i32, _i32, _i32, _i32, _vp, _vp, _vp]),
+    "mcx_batch_create": (_i32, [_vp, _i32, _vp, _vp, _vp, _i32]),
+    "mcx_batch_destroy": (None, [_vp]),
+    "mcx_batch_tiles": (_i64, [_vp]),
+    "mcx_batch_workspace_size": (_sz, [_vp, _i64]),
+    "mcx_extract_batched": (_i32, [_vp, _vp, _vp, _vp, _i64, _vp, _vp, _vp, _vp, _sz, _vp]),
+    "mcx_normals": (_i32, [_vp, _i32, _i32, _i32, _f32, _vp, _i64, _i64, _vp, _vp]),
     "mcx_extract_diff": (_i32, [_vp, _i32, _i32, _i32, _f32, _vp, _i64, _vp, _vp, _vp, _sz, _vp]),
     "mcx_backward_scratch_size": (_sz, [_i32, _i32, _i32, _sz]),
     "mcx_extract_backward": (_i32, [_vp, _i32, _i32, _i32, _f32, _vp, _vp, _vp, _sz, _vp, _sz,
@@ -48,6 +54,8 @@ SIGNATURES = {
     "mcx_integrate_workspace_size": (_sz, [_i32, _i32]),
     "mcx_integrate": (_i32, [_vp, _i32, _i32, _i32, _f32, _f32, _i32, _vp, _vp, _vp, _f32, _i32,
                              _i32, _i32, _vp, _sz, _vp]),
+    "mcx_integrate_f32": (_i32, [_vp, _i32, _i32, _i32, _f32, _f32, _i32, _vp, _vp, _vp, _f32,
+                                 _i32, _i32, _i32, _vp, _sz, _vp]),
     "mcx_synth_depth": (_i32, [_vp, _i32, _i32, _i32, _vp, _vp, _vp, _i32, _f32, _f32, _u32,
                                _f32, _vp]),
 }

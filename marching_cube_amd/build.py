@@ -71,9 +71,25 @@ def build_dropin():
     return DROPIN_LIB
 
 
+TEST_BIN = os.path.join(ROOT, "tests", "cpp", "test_dropin")
+
+
+def build_tests():
+    """C++ program that drives the drop-in API (run by tests/test_dropin_gpu.py)."""
+    src = os.path.join(ROOT, "tests", "cpp", "test_dropin.cpp")
+    if not os.path.exists(src):
+        return None
+    _run(["g++", "-O2", "-std=c++17", "-Wall", "-I", INCLUDE, "-I", "/opt/rocm/include",
+          "-D__HIP_PLATFORM_AMD__", "-o", TEST_BIN, src, "-L", PKG, "-lmarching_cube_amd",
+          "-lmcx", "-Wl,-rpath,$ORIGIN/../../marching_cube_amd", "-L/opt/rocm/lib",
+          "-lamdhip64", "-Wl,-rpath,/opt/rocm/lib"])
+    return TEST_BIN
+
+
 def build_all():
     build_lib()
     build_dropin()
+    build_tests()
 
 
 if __name__ == "__main__":

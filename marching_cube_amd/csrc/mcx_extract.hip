@@ -37,6 +37,9 @@
 // tsdf = w = 0 and no ver_voxel write.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <vector>
+
 #include "mc_tables.inc"
 #include "mcx_internal.h"
 
@@ -1144,5 +1147,131 @@ extern "C" int mcx_extract_backward(const float* d_vol, int X, int Y, int Z, flo
   hipLaunchKernelGGL(gather_kernel<false>, dim3((unsigned)g2), dim3(256), 0, s, P, contrib,
                      d_grad_vol, 0);
   MCX_HIP(hipGetLastError());
+  return MCX_OK;
+}
+
+// ---- batched extraction (no counterpart in the reference) -----------------
+// A plan holds the per-volume descriptors and the K1 column list in device
+// memory (uploaded once); mcx_extract_batched then only launches kernels, so
+// it can be captured in a hipGraph and replayed.
+struct mcx_batch {
+  int nvol = 0;
+  int elem = MCX_ELEM_SHORT2;
+  int maxX = 0;
+  int64_t n_tiles = 0;
+  int64_t ncols = 0;
+  int64_t cubes = 0;
+  VolDesc* d_vols = nullptr;
+  Column* d_cols = nullptr;
+};
+
+extern "C" int mcx_batch_create(mcx_batch** out, int nvol, const int* dims_xyz, const float* h,
+                                const void* const* d_vols, int elem) {
+  if (!out || nvol < 1 || !dims_xyz || !h || !d_vols) return MCX_ERR_ARG;
+  if (elem != MCX_ELEM_SHORT2 && elem != MCX_ELEM_F32) return MCX_ERR_ARG;
+  std::vector<VolDesc> vols(nvol);
+  std::vector<Column> cols;
+  int64_t tb = 0, cubes = 0;
+  int maxX = 1;
+  for (int b = 0; b < nvol; ++b) {
+    const int X = dims_xyz[3 * b], Y = dims_xyz[3 * b + 1], Z = dims_xyz[3 * b + 2];
+    if (X < 1 || Y < 1 || Z < 1 || !d_vols[b]) return MCX_ERR_ARG;
+    if ((int64_t)X * Y >= ((int64_t)1 << 30)) return MCX_ERR_UNSUPPORTED;
+    vols[b] = make_vol(d_vols[b], X, Y, Z, h[b], nullptr, tb);
+    const int64_t nt = tiles_of(X, Y, Z);
+    if (nt > 0) {
+      // one column per in-plane range, slabs of at most 64 planes
+      const int R = vols[b].R;
+      const int slab = 64;
+      for (int z0 = 0; z0 < Z - 1; z0 += slab)
+        for (int r = 0; r < R; ++r) cols.push_back(Column{b, r, z0, std::min(z0 + slab, Z - 1)});
+    }
+    tb += nt;
+    cubes += cubes_of(X, Y, Z);
+    maxX = std::max(maxX, X);
+  }
+  mcx_batch* p = new mcx_batch();
+  p->nvol = nvol;
+  p->elem = elem;
+  p->maxX = maxX;
+  p->n_tiles = tb;
+  p->ncols = (int64_t)cols.size();
+  p->cubes = cubes;
+  hipError_t e = hipMalloc(reinterpret_cast<void**>(&p->d_vols), sizeof(VolDesc) * nvol);
+  if (e == hipSuccess && !cols.empty())
+    e = hipMalloc(reinterpret_cast<void**>(&p->d_cols), sizeof(Column) * cols.size());
+  if (e == hipSuccess)
+    e = hipMemcpy(p->d_vols, vols.data(), sizeof(VolDesc) * nvol, hipMemcpyHostToDevice);
+  if (e == hipSuccess && !cols.empty())
+    e = hipMemcpy(p->d_cols, cols.data(), sizeof(Column) * cols.size(), hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    (void)hipFree(p->d_vols);
+    (void)hipFree(p->d_cols);
+    delete p;
+    return hip_status(e);
+  }
+  *out = p;
+  return MCX_OK;
+}
+
+extern "C" void mcx_batch_destroy(mcx_batch* p) {
+  if (!p) return;
+  (void)hipFree(p->d_vols);
+  (void)hipFree(p->d_cols);
+  delete p;
+}
+
+extern "C" int64_t mcx_batch_tiles(const mcx_batch* p) { return p ? p->n_tiles : 0; }
+
+extern "C" size_t mcx_batch_workspace_size(const mcx_batch* p, int64_t records) {
+  if (!p) return 0;
+  if (records < 0) records = std::max<int64_t>(65536, p->cubes / 16);
+  return fixed_ws_bytes(p->n_tiles) + (size_t)records * kRecWords * 4;
+}
+
+extern "C" int mcx_extract_batched(const mcx_batch* plan, mcx_vertex_voxel* d_vv, float* d_pos,
+                                   int32_t* d_v2vox, int64_t cap, int64_t* d_vbase,
+                                   int64_t* d_count, int64_t* h_count, void* d_ws,
+                                   size_t ws_bytes, void* stream) {
+  if (!plan || !d_vbase || cap < 0) return MCX_ERR_ARG;
+  if (plan->elem == MCX_ELEM_F32 && d_vv) return MCX_ERR_ARG;
+  hipStream_t s = as_stream(stream);
+  ExtractParams P{};
+  int rc = layout_ws(d_ws, ws_bytes, plan->n_tiles, P);
+  if (rc) return rc;
+  P.vols = plan->d_vols;
+  P.cols = plan->d_cols;
+  P.nvol = plan->nvol;
+  P.n_tiles = plan->n_tiles;
+  P.ncols = plan->ncols;
+  P.quota = P.ncols > 0 ? P.rec_cap / P.ncols : 0;
+  if (P.quota > (int64_t)kTile * 64) P.quota = (int64_t)kTile * 64;
+  P.vv = d_vv;
+  P.pos = d_pos;
+  P.v2vox = d_v2vox;
+  P.cap = cap;
+  P.d_count = d_count;
+  P.vbase = d_vbase;
+  if (plan->n_tiles == 0) {
+    // no cubes anywhere: every base (and the total) is 0
+    rc = fill_bytes_zero(d_vbase, sizeof(int64_t) * (plan->nvol + 1), s);
+    if (!rc && d_count) rc = fill_bytes_zero(d_count, sizeof(int64_t), s);
+    if (rc) return rc;
+    if (h_count) {
+      MCX_HIP(hipStreamSynchronize(s));
+      *h_count = 0;
+    }
+    return MCX_OK;
+  }
+  rc = run_extract(P, plan->elem, true, plan->maxX, s);
+  if (rc) return rc;
+  if (h_count) {
+    WsHeader hh;
+    rc = read_header(d_ws, &hh, s);
+    if (rc) return rc;
+    *h_count = hh.count;
+    if (hh.error) return MCX_ERR_INTERNAL;
+    if (hh.count > cap) return MCX_ERR_CAPACITY;
+  }
   return MCX_OK;
 }

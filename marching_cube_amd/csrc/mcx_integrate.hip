@@ -55,13 +55,14 @@ struct IntegrateParams {
   int max_w;
   float fx, fy, cx, cy;
   const float* poses;      // [K][12] device
-  const uint16_t* depth;   // [K][H][W] device
+  const void* depth;       // [K][H][W] device, uint16 or float
   const float* rayfac;     // [H][W]
   float depth_scale;
   int W, H, K;
   FastDiv divX, divY;
 };
 
+template <typename DT>
 __global__ __launch_bounds__(256) void integrate_kernel(IntegrateParams P) {
   __shared__ float s_pose[kFramesPerPass * 12];
   for (int i = threadIdx.x; i < P.K * 12; i += blockDim.x) s_pose[i] = P.poses[i];
@@ -92,7 +93,7 @@ __global__ __launch_bounds__(256) void integrate_kernel(IntegrateParams P) {
     const int ui = (int)rintf(u), vi = (int)rintf(v);
     if (ui < 0 || vi < 0 || ui >= P.W - 2 || vi >= P.H - 2) continue;
     const int pix = vi * P.W + ui;
-    const uint16_t dm = P.depth[(int64_t)k * frame + pix];
+    const DT dm = static_cast<const DT*>(P.depth)[(int64_t)k * frame + pix];
     const float d = (float)dm * P.depth_scale;
     const float zval = d * P.rayfac[pix];
     if (!(zval > FLT_MIN)) continue;
@@ -218,10 +219,10 @@ extern "C" size_t mcx_integrate_workspace_size(int W, int H) {
   return (size_t)(W > 0 ? W : 0) * (size_t)(H > 0 ? H : 0) * sizeof(float);
 }
 
-extern "C" int mcx_integrate(void* d_vol, int X, int Y, int Z, float h, float trunc, int max_w,
-                             const float* intr, const float* d_poses, const uint16_t* d_depth,
-                             float depth_scale, int W, int H, int K, void* d_ws, size_t ws_bytes,
-                             void* stream) {
+template <typename DT>
+int integrate_impl(void* d_vol, int X, int Y, int Z, float h, float trunc, int max_w,
+                   const float* intr, const float* d_poses, const DT* d_depth, float depth_scale,
+                   int W, int H, int K, void* d_ws, size_t ws_bytes, void* stream) {
   if (!d_vol || !intr || X < 1 || Y < 1 || Z < 1 || W < 3 || H < 3 || K < 0) return MCX_ERR_ARG;
   if (!(h > 0.f) || !(trunc > 0.f) || max_w < 1 || max_w > 65535) return MCX_ERR_ARG;
   if (K == 0) return MCX_OK;
@@ -256,10 +257,26 @@ extern "C" int mcx_integrate(void* d_vol, int X, int Y, int Z, float h, float tr
     P.K = K - k0 < kFramesPerPass ? K - k0 : kFramesPerPass;
     P.poses = d_poses + (int64_t)12 * k0;
     P.depth = d_depth + (int64_t)k0 * W * H;
-    hipLaunchKernelGGL(integrate_kernel, dim3(blocks_for(n)), dim3(256), 0, s, P);
+    hipLaunchKernelGGL(integrate_kernel<DT>, dim3(blocks_for(n)), dim3(256), 0, s, P);
     MCX_HIP(hipGetLastError());
   }
   return MCX_OK;
+}
+
+extern "C" int mcx_integrate(void* d_vol, int X, int Y, int Z, float h, float trunc, int max_w,
+                             const float* intr, const float* d_poses, const uint16_t* d_depth,
+                             float depth_scale, int W, int H, int K, void* d_ws, size_t ws_bytes,
+                             void* stream) {
+  return integrate_impl<uint16_t>(d_vol, X, Y, Z, h, trunc, max_w, intr, d_poses, d_depth,
+                                  depth_scale, W, H, K, d_ws, ws_bytes, stream);
+}
+
+extern "C" int mcx_integrate_f32(void* d_vol, int X, int Y, int Z, float h, float trunc,
+                                 int max_w, const float* intr, const float* d_poses,
+                                 const float* d_depth, float depth_scale, int W, int H, int K,
+                                 void* d_ws, size_t ws_bytes, void* stream) {
+  return integrate_impl<float>(d_vol, X, Y, Z, h, trunc, max_w, intr, d_poses, d_depth,
+                               depth_scale, W, H, K, d_ws, ws_bytes, stream);
 }
 
 extern "C" int mcx_synth_depth(uint16_t* d_depth, int W, int H, int K, const float* intr,

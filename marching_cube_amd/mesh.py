@@ -163,6 +163,21 @@ def classify(vol):
     return ci, nv
 
 
+def normals(vol, h, positions):
+    """Per-vertex normals (V, 3) (d_voxel_extract_surface_normals,
+    cu_kinect_fusion_v1.cu:3146-3238); `positions` is a vertex_voxel tensor
+    (V, 7) or a float32 (V, 3) tensor."""
+    elem, X, Y, Z = volume_kind(vol)
+    if elem != _capi.ELEM_SHORT2:
+        raise ValueError("normals need a short2 volume")
+    stride = positions.shape[1]
+    n = positions.shape[0]
+    out = torch.empty((n, 3), dtype=torch.float32, device=vol.device)
+    _capi.check(_capi.load().mcx_normals(_ptr(vol), X, Y, Z, float(h), _ptr(positions), stride, n,
+                                         _ptr(out), _stream_handle(vol.device)), "mcx_normals")
+    return out
+
+
 def vv_positions(vv):
     """float32 (V, 3) view of the positions in a vertex_voxel tensor."""
     return vv[:, :3].view(torch.float32)

@@ -139,3 +139,16 @@ def test_record_overflow_uses_staged_path(dev, records):
         assert r.vv.cpu().numpy().tobytes() == ref["vv"].view(np.int32).reshape(-1, 7).tobytes()
         assert r.v2vox.cpu().numpy().tobytes() == ref["v2vox"].tobytes()
         assert r.ver_voxel.cpu().numpy().tobytes() == ref["ver_voxel"].tobytes()
+
+
+def test_normals_bit_exact(dev):
+    for vol, h in [synth.sphere_volume(64), (synth.noise_volume((12, 13, 14), seed=3), 0.02)]:
+        print(f"[case] normals {vol.shape}", flush=True)
+        ref = O.extract(vol, h, want=("vv", "pos"))
+        nref = O.normals(vol, h, ref["pos"])
+        v = to_dev(vol, dev)
+        r = mesh.extract(v, h, want=("vv",))
+        n_vv = mesh.normals(v, h, r.vv)
+        n_pos = mesh.normals(v, h, torch.from_numpy(ref["pos"]).to(dev))
+        assert n_vv.cpu().numpy().tobytes() == nref.tobytes()
+        assert n_pos.cpu().numpy().tobytes() == nref.tobytes()

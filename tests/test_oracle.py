@@ -202,3 +202,37 @@ def test_backward_f64_matches_central_differences():
         if checked >= 60:
             break
     assert checked >= 30
+
+
+def test_normals_closed_forms():
+    # planar field along x: every normal is exactly +x
+    n = 10
+    h = 0.1
+    x = (np.arange(n) + 0.5) * h
+    t = np.broadcast_to(((x - 0.43) * 20000)[None, None, :], (n, n, n)).astype(np.int32)
+    vol = synth.pack_short2(t, np.ones_like(t))
+    r = O.extract(vol, h, want=("pos",))
+    nr = O.normals(vol, h, r["pos"])
+    ok = np.abs(nr).sum(axis=1) > 0  # vertices whose cell is inside the volume
+    assert ok.mean() > 0.5
+    np.testing.assert_allclose(nr[ok], np.tile([1.0, 0.0, 0.0], (ok.sum(), 1)), atol=1e-6)
+    # a +-32766 corner anywhere in the cell zeroes the normal
+    t2 = t.copy()
+    t2[:, :, 0] = 32766
+    vol2 = synth.pack_short2(t2, np.ones_like(t2))
+    r2 = O.extract(vol2, h, want=("pos",))
+    nr2 = O.normals(vol2, h, r2["pos"])
+    pf = r2["pos"][:, 0] / np.float32(h)
+    assert (nr2[pf.astype(int) == 0] == 0).all()
+
+
+def test_normals_sphere_radial():
+    vol, h = synth.sphere_volume(64)
+    r = O.extract(vol, h, want=("pos",))
+    nr = O.normals(vol, h, r["pos"])
+    radial = r["pos"].astype(np.float64) - 0.5
+    radial /= np.linalg.norm(radial, axis=1)[:, None]
+    nz = np.abs(nr).sum(axis=1) > 0
+    assert nz.mean() > 0.95
+    cosang = (nr[nz] * radial[nz]).sum(axis=1)
+    assert cosang.min() > 0.99  # outward and within ~8 degrees of radial
