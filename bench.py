@@ -37,9 +37,143 @@ def parse():
     ap.add_argument("--dim", type=int, default=512)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-threads", type=int, default=16)
+    ap.add_argument("--diff-dim", type=int, default=256,
+                    help="side of the fp32 bumps field of the forward/backward leg (0 = skip)")
+    ap.add_argument("--batch-count", type=int, default=3000,
+                    help="volumes of the batched leg (0 = skip)")
     ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "extract_kinfu512_traffic.json"),
                     help="JSON with measured HBM bytes per launch (from rocprofv3 --pmc)")
     return ap.parse_args()
+
+
+def _event_ms(torch, fn, steps, warmup):
+    """Average and median ms of fn() between HIP events on the current stream."""
+    for i in range(warmup):
+        fn(i)
+    torch.cuda.synchronize()
+    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+           for _ in range(steps)]
+    for i in range(steps):
+        evs[i][0].record()
+        fn(i)
+        evs[i][1].record()
+    torch.cuda.synchronize()
+    ms = sorted(a.elapsed_time(b) for a, b in evs)
+    return sum(ms) / len(ms), ms[len(ms) // 2]
+
+
+def bench_diff(torch, dev, dim, steps, warmup):
+    """Forward vs forward+backward on the 256^3 fp32 sum-of-64-bumps field:
+    plain extraction (float3 positions) against the differentiable forward
+    (which also keeps the per-cube records) + the backward kernels, for the
+    gradient of sum of squared distances of the vertices to a sphere of
+    radius 0.35.  Kernel time between HIP events; the loss gradient is
+    computed once outside the timed region."""
+    import ctypes
+    import numpy as np
+    from marching_cube_amd import _capi, diff, mesh, synth
+    from marching_cube_amd.mesh import _ptr, _stream_handle
+    lib = _capi.load()
+    f, h = synth.bumps_volume(dim)
+    vol = torch.from_numpy(f).to(dev)
+    Z, Y, X = vol.shape
+    ex = diff.DiffExtractor(vol.shape, dev)
+    pos = ex.forward(vol, h)
+    count = int(pos.shape[0])
+    c = torch.tensor([0.5, 0.5, 0.5], dtype=torch.float32, device=dev)
+    d = pos - c
+    nrm = d.norm(dim=1, keepdim=True)
+    grad_pos = (2 * (nrm - 0.35) / nrm * d).contiguous()
+    gvol = torch.empty_like(vol)
+    # plain forward into its own buffers
+    ws = mesh.Workspace(dev)
+    pbuf = torch.empty((count, 3), dtype=torch.float32, device=dev)
+    dc = torch.zeros(1, dtype=torch.int64, device=dev)
+    assert mesh.extract_into(vol, h, pos=pbuf, cap=count, workspace=ws) == count
+    assert pbuf.cpu().numpy().tobytes() == pos.cpu().numpy().tobytes()
+    stream = _stream_handle(dev)
+
+    def plain(i):
+        mesh.extract_into(vol, h, pos=pbuf, cap=count, d_count=dc, workspace=ws, sync=False)
+
+    def fwd_diff(i):
+        _capi.check(lib.mcx_extract_diff(_ptr(vol), X, Y, Z, float(h), _ptr(ex.pos),
+                                         ex.pos.shape[0], _ptr(dc), None, _ptr(ex.ws),
+                                         ex.ws.numel(), stream), "mcx_extract_diff")
+
+    def bwd(i):
+        _capi.check(lib.mcx_extract_backward(_ptr(vol), X, Y, Z, float(h), _ptr(grad_pos),
+                                             _ptr(gvol), _ptr(ex.ws), ex.ws.numel(),
+                                             _ptr(ex.scratch), ex.scratch.numel(), stream),
+                    "mcx_extract_backward")
+
+    def fwd_bwd(i):
+        fwd_diff(i)
+        bwd(i)
+
+    plain_ms, _ = _event_ms(torch, plain, steps, warmup)
+    fdiff_ms, _ = _event_ms(torch, fwd_diff, steps, warmup)
+    fb_ms, _ = _event_ms(torch, fwd_bwd, steps, warmup)
+    bwd_ms, _ = _event_ms(torch, bwd, steps, warmup)
+    ref = ex.backward(vol, grad_pos)
+    assert gvol.cpu().numpy().tobytes() == ref.cpu().numpy().tobytes()  # deterministic
+    # end-to-end autograd step (forward, loss, backward) with a reused extractor
+    vg = vol.clone().requires_grad_(True)
+
+    def autograd_step(i):
+        p = diff.extract_positions(vg, h, extractor=ex)
+        loss = ((torch.linalg.vector_norm(p - c, dim=1) - 0.35) ** 2).sum()
+        vg.grad = None
+        loss.backward()
+    ag_ms, _ = _event_ms(torch, autograd_step, max(3, steps // 2), 2)
+    nvox = X * Y * Z
+    return {"workload": f"bumps{dim}_fp32_fwd_bwd", "vertices": count,
+            "fwd_ms": round(plain_ms, 4), "fwd_diff_ms": round(fdiff_ms, 4),
+            "bwd_ms": round(bwd_ms, 4), "fwd_bwd_ms": round(fb_ms, 4),
+            "fwd_bwd_over_fwd": round(fb_ms / plain_ms, 3),
+            "fwd_gvoxel_per_s": round(nvox / (plain_ms * 1e-3) / 1e9, 1),
+            "autograd_step_ms": round(ag_ms, 4)}
+
+
+def bench_batch(torch, dev, count, steps, warmup):
+    """3,000 uneven short2 shape volumes (sides log-uniform in [16,128] per
+    axis; sphere/box/torus at random poses): one batched launch sequence
+    against one extraction call per volume, same outputs (checked bit-exact)."""
+    import numpy as np
+    from marching_cube_amd import batch, mesh, synth
+    vols, hs = synth.batch_shapes(count, device=dev)
+    torch.cuda.synchronize()
+    plan = batch.BatchPlan(vols, hs)
+    n, vbase, out = plan.extract(want=("vv",))
+    vv = out["vv"]
+    vb = vbase.cpu().numpy()
+    dc = torch.zeros(1, dtype=torch.int64, device=dev)
+
+    def batched(i):
+        plan.extract_into(vv=vv, cap=n, vbase=vbase, d_count=dc, sync=False)
+
+    vv2 = torch.empty_like(vv)
+    ws = mesh.Workspace(dev)
+    lib_sizes = [mesh.volume_kind(v)[1:] for v in vols]
+    from marching_cube_amd import _capi
+    lib = _capi.load()
+    ws.ensure(max(lib.mcx_extract_workspace_size(X, Y, Z) for X, Y, Z in lib_sizes))
+    segs = [vv2[vb[b]:vb[b + 1]] for b in range(count)]
+
+    def per_volume(i):
+        for b in range(count):
+            mesh.extract_into(vols[b], float(hs[b]), vv=segs[b], cap=int(vb[b + 1] - vb[b]),
+                              d_count=dc, workspace=ws, sync=False)
+
+    b_ms, _ = _event_ms(torch, batched, steps, warmup)
+    p_ms, _ = _event_ms(torch, per_volume, max(2, steps // 4), 1)
+    assert vv.cpu().numpy().tobytes() == vv2.cpu().numpy().tobytes()
+    nvox = int(sum(X * Y * Z for X, Y, Z in lib_sizes))
+    return {"workload": f"batch{count}_shapes_16_128", "volumes": count, "voxels": nvox,
+            "vertices": int(n), "batched_ms": round(b_ms, 4), "per_volume_ms": round(p_ms, 4),
+            "speedup": round(p_ms / b_ms, 2),
+            "batched_gvoxel_per_s": round(nvox / (b_ms * 1e-3) / 1e9, 1),
+            "batched_mtri_per_s": round(n / 3 / (b_ms * 1e-3) / 1e6, 1)}
 
 
 def main():
@@ -135,6 +269,14 @@ def main():
     full_ms = (time.perf_counter() - tf) * 1e3 / nfull
     del v2, verv
 
+    diff_leg = bench_diff(torch, dev, args.diff_dim, args.steps, args.warmup) if args.diff_dim else None
+    if diff_leg:
+        print(f"[bench] diff {diff_leg}", file=sys.stderr, flush=True)
+    batch_leg = (bench_batch(torch, dev, args.batch_count, args.steps, args.warmup)
+                 if args.batch_count else None)
+    if batch_leg:
+        print(f"[bench] batch {batch_leg}", file=sys.stderr, flush=True)
+
     # ---- CPU baseline: the oracle's extraction on the host cores
     cpu = None
     if not args.no_cpu:
@@ -187,6 +329,8 @@ def main():
         "full_api_ms": round(full_ms, 4),
         "integrate_ms": round(integrate_ms, 2),
         "integrate_gvoxel_frames_per_s": round(nvox * args.frames / (integrate_ms * 1e-3) / 1e9, 2),
+        "fwd_bwd": diff_leg,
+        "batched": batch_leg,
     }
     print(json.dumps(out), flush=True)
     return 0

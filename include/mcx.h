@@ -68,7 +68,7 @@ int mcx_workspace_init(void* d_ws, size_t ws_bytes, void* stream);
 
 /* ---- extraction (replaces MarchingCubes::run, cu_marching_cubes_v1.cu:417-467:
  *      getOccupiedVoxels :185-206, computeOffsetsAndTotalVertexes :208-225,
- *      generateTriangles :372-396, in one fused kernel) --------------------
+ *      generateTriangles :372-396; three kernels: count, scan, emit) -------
  * Outputs (each optional, NULL = not written):
  *   d_vv        vertex_voxel records (SHORT2 only)
  *   d_pos       float3 positions
@@ -136,6 +136,17 @@ int mcx_extract_backward(const float* d_vol, int X, int Y, int Z, float h,
                          const float* d_grad_pos, float* d_grad_vol, void* d_ws,
                          size_t ws_bytes, void* d_scratch, size_t scratch_bytes,
                          void* stream);
+/* Same result without relying on the workspace contents: the per-cube
+ * records of the forward are recomputed from d_vol (the count and scan
+ * kernels again, no emission) before the backward kernels run.  The
+ * workspace can then serve other extractions between the forward and the
+ * backward (records are not kept alive per forward).  d_ws must have at least
+ * the size that the forward's mcx_extract_diff call checked (so no record
+ * scratch overflows); its contents are overwritten. */
+int mcx_extract_backward_recompute(const float* d_vol, int X, int Y, int Z, float h,
+                                   const float* d_grad_pos, float* d_grad_vol, void* d_ws,
+                                   size_t ws_bytes, void* d_scratch, size_t scratch_bytes,
+                                   void* stream);
 
 /* ---- TSDF volume (replaces cu_voxel_new / d_voxel_initialize,
  *      cu_kinect_fusion_v1.cu:364-398): every voxel = (tsdf -32766, w 0). */

@@ -50,6 +50,8 @@ SIGNATURES = {
     "mcx_backward_scratch_size": (_sz, [_i32, _i32, _i32, _sz]),
     "mcx_extract_backward": (_i32, [_vp, _i32, _i32, _i32, _f32, _vp, _vp, _vp, _sz, _vp, _sz,
                                     _vp]),
+    "mcx_extract_backward_recompute": (_i32, [_vp, _i32, _i32, _i32, _f32, _vp, _vp, _vp, _sz,
+                                              _vp, _sz, _vp]),
     "mcx_volume_init": (_i32, [_vp, _i32, _i32, _i32, _vp]),
     "mcx_integrate_workspace_size": (_sz, [_i32, _i32]),
     "mcx_integrate": (_i32, [_vp, _i32, _i32, _i32, _f32, _f32, _i32, _vp, _vp, _vp, _f32, _i32,

@@ -907,8 +907,11 @@ int layout_ws(void* d_ws, size_t ws_bytes, int64_t n_tiles, ExtractParams& P) {
   return MCX_OK;
 }
 
-// Launch K1, K2, (vbase), K3 for prepared parameters.
-int run_extract(ExtractParams& P, int elem, bool batch, int maxX, hipStream_t s) {
+// Launch K1, K2, (vbase), K3 for prepared parameters; emit = false stops
+// after the scan (records, masks and tile offsets only: the backward's
+// recompute of the forward's per-cube records).
+int run_extract(ExtractParams& P, int elem, bool batch, int maxX, hipStream_t s,
+                bool emit = true) {
   const unsigned k1_blocks = (unsigned)((P.ncols + 3) / 4);
   const unsigned k2_blocks = (unsigned)((P.n_tiles + kScanItems - 1) / kScanItems);
   const unsigned k3_blocks =
@@ -929,6 +932,7 @@ int run_extract(ExtractParams& P, int elem, bool batch, int maxX, hipStream_t s)
                          P);                                                                     \
       MCX_HIP(hipGetLastError());                                                                \
     }                                                                                            \
+    if (!emit) break;                                                                            \
     MCX_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(emit_kernel<E, B>),               \
                                 hipFuncAttributeMaxDynamicSharedMemorySize, k3_lds));            \
     hipLaunchKernelGGL((emit_kernel<E, B>), dim3(k3_blocks), dim3(kEmitThreads), k3_lds, s, P);  \
@@ -1123,11 +1127,12 @@ extern "C" int mcx_extract_diff(const float* d_vol, int X, int Y, int Z, float h
   return MCX_OK;
 }
 
-extern "C" int mcx_extract_backward(const float* d_vol, int X, int Y, int Z, float h,
-                                    const float* d_grad_pos, float* d_grad_vol, void* d_ws,
-                                    size_t ws_bytes, void* d_scratch, size_t scratch_bytes,
-                                    void* stream) {
+namespace {
+int backward_impl(const float* d_vol, int X, int Y, int Z, float h, const float* d_grad_pos,
+                  float* d_grad_vol, void* d_ws, size_t ws_bytes, void* d_scratch,
+                  size_t scratch_bytes, void* stream, bool recompute) {
   if (!d_vol || !d_grad_vol || X < 1 || Y < 1 || Z < 1) return MCX_ERR_ARG;
+  if ((int64_t)X * Y >= ((int64_t)1 << 30)) return MCX_ERR_UNSUPPORTED;
   hipStream_t s = as_stream(stream);
   const int64_t N = (int64_t)X * Y * Z;
   const int64_t n_tiles = tiles_of(X, Y, Z);
@@ -1136,6 +1141,15 @@ extern "C" int mcx_extract_backward(const float* d_vol, int X, int Y, int Z, flo
   ExtractParams P{};
   int rc = prepare_single(P, d_vol, X, Y, Z, h, nullptr, d_ws, ws_bytes);
   if (rc) return rc;
+  if (recompute) {
+    // rebuild the per-cube records, active-cube masks and tile offsets of
+    // the forward (count + scan, no emission); same layout for the same
+    // (X, Y, Z, ws_bytes), so the same records as a checked forward
+    P.write_masks = true;
+    P.cap = 0;
+    rc = run_extract(P, MCX_ELEM_F32, false, X, s, /*emit=*/false);
+    if (rc) return rc;
+  }
   if (!d_scratch || scratch_bytes < (size_t)P.rec_cap * 8 * sizeof(float)) return MCX_ERR_WORKSPACE;
   float* contrib = static_cast<float*>(d_scratch);
   const unsigned g1 = (unsigned)((n_tiles + kTilesPerEmitGroup - 1) / kTilesPerEmitGroup);
@@ -1148,6 +1162,23 @@ extern "C" int mcx_extract_backward(const float* d_vol, int X, int Y, int Z, flo
                      d_grad_vol, 0);
   MCX_HIP(hipGetLastError());
   return MCX_OK;
+}
+}  // namespace
+
+extern "C" int mcx_extract_backward(const float* d_vol, int X, int Y, int Z, float h,
+                                    const float* d_grad_pos, float* d_grad_vol, void* d_ws,
+                                    size_t ws_bytes, void* d_scratch, size_t scratch_bytes,
+                                    void* stream) {
+  return backward_impl(d_vol, X, Y, Z, h, d_grad_pos, d_grad_vol, d_ws, ws_bytes, d_scratch,
+                       scratch_bytes, stream, false);
+}
+
+extern "C" int mcx_extract_backward_recompute(const float* d_vol, int X, int Y, int Z, float h,
+                                              const float* d_grad_pos, float* d_grad_vol,
+                                              void* d_ws, size_t ws_bytes, void* d_scratch,
+                                              size_t scratch_bytes, void* stream) {
+  return backward_impl(d_vol, X, Y, Z, h, d_grad_pos, d_grad_vol, d_ws, ws_bytes, d_scratch,
+                       scratch_bytes, stream, true);
 }
 
 // ---- batched extraction (no counterpart in the reference) -----------------

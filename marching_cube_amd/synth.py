@@ -102,29 +102,68 @@ def _random_rotation(rng):
                      [2 * (x * z - y * w), 2 * (y * z + x * w), 1 - 2 * (x * x + y * y)]])
 
 
-def shape_volume(dims, kind, rng, trunc_vox=3.0):
-    """One (Z,Y,X) short2 volume with h = 1/max(dims) holding a shape SDF."""
+def shape_params(dims, kind, rng):
+    """Random pose and size of one shape in a (Z,Y,X) volume with h = 1/max(dims):
+    centre c (x,y,z), rotation R (world = R @ shape frame) and size parameters."""
     Z, Y, X = dims
     h = 1.0 / max(dims)
     ext = np.array([X, Y, Z], np.float64) * h
-    zz, yy, xx = np.meshgrid(_node_coords(Z, h), _node_coords(Y, h), _node_coords(X, h),
-                             indexing="ij")
     c = ext * rng.uniform(0.35, 0.65, size=3)
     R = _random_rotation(rng)
-    p = np.stack([xx - c[0], yy - c[1], zz - c[2]], axis=-1) @ R  # into shape frame
     m = 0.5 * ext.min()
     if kind == "sphere":
-        sdf = np.linalg.norm(p, axis=-1) - m * rng.uniform(0.4, 0.8)
+        size = np.array([m * rng.uniform(0.4, 0.8), 0.0, 0.0])
     elif kind == "box":
-        b = m * rng.uniform(0.3, 0.7, size=3)
-        q = np.abs(p) - b
-        sdf = np.linalg.norm(np.maximum(q, 0), axis=-1) + np.minimum(q.max(axis=-1), 0)
-    else:  # torus
+        size = m * rng.uniform(0.3, 0.7, size=3)
+    else:  # torus: major radius, minor radius
         R0 = m * rng.uniform(0.4, 0.6)
-        r0 = R0 * rng.uniform(0.25, 0.5)
-        qx = np.sqrt(p[..., 0] ** 2 + p[..., 1] ** 2) - R0
-        sdf = np.sqrt(qx ** 2 + p[..., 2] ** 2) - r0
-    return sdf_to_short2(sdf, trunc_vox * h), h
+        size = np.array([R0, R0 * rng.uniform(0.25, 0.5), 0.0])
+    return h, c, R, size
+
+
+def _shape_sdf(xp, p, kind, size):
+    """SDF of a shape at the origin of its frame; p (..., 3); xp = numpy or torch."""
+    if kind == "sphere":
+        return xp.sqrt((p * p).sum(-1)) - size[0]
+    if kind == "box":
+        q = xp.abs(p) - xp.asarray(size) if xp is np else p.abs() - p.new_tensor(size)
+        qp = q.clip(0, None) if xp is np else q.clamp(min=0)
+        qmax = q.max(-1) if xp is np else q.max(-1).values
+        return xp.sqrt((qp * qp).sum(-1)) + (qmax.clip(None, 0) if xp is np else qmax.clamp(max=0))
+    qx = xp.sqrt(p[..., 0] ** 2 + p[..., 1] ** 2) - size[0]
+    return xp.sqrt(qx ** 2 + p[..., 2] ** 2) - size[1]
+
+
+def shape_volume(dims, kind, rng, trunc_vox=3.0):
+    """One (Z,Y,X) short2 volume with h = 1/max(dims) holding a shape SDF."""
+    Z, Y, X = dims
+    h, c, R, size = shape_params(dims, kind, rng)
+    zz, yy, xx = np.meshgrid(_node_coords(Z, h), _node_coords(Y, h), _node_coords(X, h),
+                             indexing="ij")
+    p = np.stack([xx - c[0], yy - c[1], zz - c[2]], axis=-1) @ R  # into shape frame
+    return sdf_to_short2(_shape_sdf(np, p, kind, size), trunc_vox * h), h
+
+
+def shape_volume_device(dims, kind, rng, device, trunc_vox=3.0):
+    """shape_volume computed with torch (FP64) on `device`: a (Z,Y,X) int32
+    tensor of packed short2 voxels.  Same shape parameters as shape_volume for
+    the same rng state; values may differ from it by one tsdf unit where the
+    FP64 arithmetic rounds differently."""
+    import torch
+    Z, Y, X = dims
+    h, c, R, size = shape_params(dims, kind, rng)
+    f64 = dict(dtype=torch.float64, device=device)
+    x = (torch.arange(X, **f64) + 0.5) * h - c[0]
+    y = (torch.arange(Y, **f64) + 0.5) * h - c[1]
+    z = (torch.arange(Z, **f64) + 0.5) * h - c[2]
+    zz, yy, xx = torch.meshgrid(z, y, x, indexing="ij")
+    p = torch.stack([xx, yy, zz], dim=-1) @ torch.as_tensor(R, **f64)
+    sdf = _shape_sdf(torch, p, kind, size)
+    trunc = trunc_vox * h
+    band = sdf.abs() < trunc
+    t = torch.round((sdf / trunc).clamp(-1.0, 1.0) * TSDF_SCALE)
+    tsdf = torch.where(band, t, torch.full_like(t, TSDF_SCALE)).to(torch.int32)
+    return ((tsdf & 0xFFFF) | (band.to(torch.int32) << 16)), h
 
 
 def batch_dims(count, lo=16, hi=128, seed=20261202):
@@ -133,13 +172,19 @@ def batch_dims(count, lo=16, hi=128, seed=20261202):
     return np.clip(np.rint(d), lo, hi).astype(np.int64)  # rows are (Z, Y, X)
 
 
-def batch_shapes(count, lo=16, hi=128, seed=20261202):
+def batch_shapes(count, lo=16, hi=128, seed=20261202, device=None):
+    """`count` uneven short2 volumes (numpy uint32 arrays, or int32 device
+    tensors when `device` is given) and their grid sizes."""
     dims = batch_dims(count, lo, hi, seed)
     rng = np.random.default_rng(seed + 1)
     kinds = ("sphere", "box", "torus")
     vols, hs = [], []
     for i in range(count):
-        v, h = shape_volume(tuple(int(a) for a in dims[i]), kinds[rng.integers(0, 3)], rng)
+        d, kind = tuple(int(a) for a in dims[i]), kinds[rng.integers(0, 3)]
+        if device is None:
+            v, h = shape_volume(d, kind, rng)
+        else:
+            v, h = shape_volume_device(d, kind, rng, device)
         vols.append(v)
         hs.append(h)
     return vols, np.array(hs, np.float32)

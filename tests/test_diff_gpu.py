@@ -147,3 +147,50 @@ def test_fd_config3_256(dev):
     print(f"[fd] checked {checked} voxels, worst relative error {worst:.3e}", flush=True)
     assert checked >= 2000
     assert worst <= 1e-4
+
+
+def test_shared_extractor_reuse_and_order_check(dev):
+    """One DiffExtractor across calls: same positions and gradients as fresh
+    buffers; a backward after another forward on the extractor raises."""
+    print("[case] diff shared extractor", flush=True)
+    f, h = synth.bumps_volume(40, nbumps=10, seed=9)
+    ex = diff.DiffExtractor(f.shape, dev)
+    ref_v = torch.from_numpy(f).to(dev).requires_grad_(True)
+    ref_p = diff.extract_positions(ref_v, h)
+    (ref_p ** 2).sum().backward()
+    for _ in range(2):
+        v = torch.from_numpy(f).to(dev).requires_grad_(True)
+        p = diff.extract_positions(v, h, extractor=ex)
+        assert p.detach().cpu().numpy().tobytes() == ref_p.detach().cpu().numpy().tobytes()
+        (p ** 2).sum().backward()
+        assert v.grad.cpu().numpy().tobytes() == ref_v.grad.cpu().numpy().tobytes()
+    v1 = torch.from_numpy(f).to(dev).requires_grad_(True)
+    p1 = diff.extract_positions(v1, h, extractor=ex)
+    v2 = torch.from_numpy(f).to(dev).requires_grad_(True)
+    diff.extract_positions(v2, h, extractor=ex)
+    with pytest.raises(RuntimeError, match="another forward"):
+        p1.sum().backward()
+
+
+def test_recompute_backward_matches_stored_records(dev):
+    """recompute=True: forwards of other volumes run between a forward and its
+    backward on one shared extractor; the gradient equals the stored-record
+    backward bit for bit."""
+    print("[case] diff recompute backward", flush=True)
+    f1, h = synth.bumps_volume(48, nbumps=14, seed=21)
+    f2, _ = synth.bumps_volume(48, nbumps=9, seed=22)
+    ref = []
+    for f in (f1, f2):
+        v = torch.from_numpy(f).to(dev).requires_grad_(True)
+        p = diff.extract_positions(v, h)
+        (p ** 3).sum().backward()
+        ref.append(v.grad.cpu().numpy().tobytes())
+    ex = diff.DiffExtractor(f1.shape, dev, recompute=True)
+    v1 = torch.from_numpy(f1).to(dev).requires_grad_(True)
+    v2 = torch.from_numpy(f2).to(dev).requires_grad_(True)
+    p1 = diff.extract_positions(v1, h, extractor=ex)
+    p2 = diff.extract_positions(v2, h, extractor=ex)  # overwrites the workspace
+    (p1 ** 3).sum().backward()
+    (p2 ** 3).sum().backward()
+    assert v1.grad.cpu().numpy().tobytes() == ref[0]
+    assert v2.grad.cpu().numpy().tobytes() == ref[1]
