@@ -14,14 +14,16 @@
 //            and eight more (zero weights) give the active cubes with a few
 //            scalar ops.  Active cubes get their case and a record (offset,
 //            case, 8 corner words) in the column's scratch quota; every tile
-//            (kTile cubes of one plane) gets its vertex count.
+//            (kTile cubes of one plane) gets its vertex count, and the tiles
+//            with vertices are appended to a list (one atomic per column).
 // K2 scan    Exclusive scan of the tile counts (single pass, look-back over
 //            blocks of 4096 tiles) -> tile vertex offsets and the total.
-// K3 emit    Vertex-parallel emission: per tile, the records are staged in
-//            LDS, their vertex counts scanned, and each thread produces one
-//            output vertex, so neighbouring threads write neighbouring
-//            records.  A tile whose records did not fit in its column's quota
-//            re-stages its two voxel ranges from the volume instead.
+// K3 emit    Wave-level emission: persistent waves take tiles from the list
+//            (strided by wave); a tile's records are taken 64 at a time, one
+//            per lane, their vertex counts scanned across the wave, and each
+//            lane produces one output vertex, so neighbouring lanes write
+//            neighbouring records.  A tile whose records did not fit in its
+//            column's quota is re-read from the volume into the same form.
 //
 // The output is in canonical order: ascending cube linear id (z*X*Y + y*X + x),
 // then the reference's triangle slot order with reversed winding (:319-332).
@@ -54,9 +56,8 @@ constexpr int kTile = 64 * kCPW;       // cubes per tile (one plane of a column)
 constexpr int kRecWords = 9;           // record: (offset << 8 | case), 8 corner words
 constexpr int kScanItems = 4096;       // tiles per scan block (256 threads x 16)
 constexpr int kEmitThreads = 256;
-constexpr int kTilesPerEmitGroup = 4096 / kTile;
-static_assert(kTile <= 4 * kEmitThreads, "staged path covers 4 cubes per thread");
-constexpr int kMaxLds = 160 * 1024;
+constexpr int kEmitGroupsPerCU = 8;       // K3 persistent workgroups per CU
+constexpr int kTilesPerContribGroup = 8;  // backward BW1: tiles per workgroup
 // third word of a gfx9 raw buffer resource (stride 0, 32-bit data format)
 constexpr int kBufferWord3 = 0x00020000;
 
@@ -97,9 +98,12 @@ struct ExtractParams {
   int nvol;
   int64_t ncols;
   int slab;              // single volume: planes per column
+  int slab_max;          // most planes of any column
+  bool wide;             // all volumes take the 128-bit count kernel
   int64_t n_tiles;
   int64_t quota;         // records per column
-  int emit_plane_words;  // staged path: words per voxel range
+  int64_t ovf_begin;     // shared overflow pool of records (wide count kernel)
+  int64_t ovf_cap;
   mcx_vertex_voxel* vv;
   float* pos;
   int32_t* v2vox;
@@ -115,6 +119,7 @@ struct ExtractParams {
   int64_t* tile_rec;     // first record of the tile
   int64_t* voff;         // exclusive vertex offset per tile
   uint64_t* masks;       // active-cube bits, kCPW words per tile
+  uint32_t* nz_list;     // ids of the tiles with vertices (in append order)
   bool write_masks;
   uint32_t* rec;
   int64_t rec_cap;
@@ -155,6 +160,15 @@ __device__ __forceinline__ uint32_t cube_index(const uint32_t c[8]) {
     w0 = w0 || is_w0<ELEM>(c[k]);
   }
   return w0 ? 0u : ci;
+}
+
+// c[i] for a runtime i without dynamic register indexing (which would put
+// the array in scratch memory): a chain of selects.
+__device__ __forceinline__ uint32_t pick8(const uint32_t c[8], int i) {
+  uint32_t r = c[0];
+#pragma unroll
+  for (int k = 1; k < 8; ++k) r = i == k ? c[k] : r;
+  return r;
 }
 
 template <bool BATCH>
@@ -223,11 +237,17 @@ __global__ __launch_bounds__(256) void count_kernel(ExtractParams P) {
   // load past the plane returns 0, and every load shares one VGPR offset.
   uint32_t A[kCPW + 1], B[kCPW + 1], C[kCPW + 1], D[kCPW + 1];
   uint32_t E[kCPW + 1] = {}, F[kCPW + 1] = {};
-  const uint32_t voff = (uint32_t)(p0 + lane) * 4u;
-  // chunk kCPW is only needed on lane 0; other lanes use an offset past any
-  // plane, which returns 0 without touching memory
-  const uint32_t voff_x = lane == 0 ? voff + 256u * kCPW : 0xfffffff0u;
+  // Every offset is in the VGPR offset (plus a small immediate): the
+  // hardware range-checks those against the resource size, not the SGPR
+  // offset.  The host keeps X*Y + X + 1024 words below 2^30, so no offset
+  // wraps.
   const uint32_t plane_bytes = (uint32_t)(XY * 4);
+  const uint32_t voff = (uint32_t)(p0 + lane) * 4u;
+  const uint32_t voffW = voff + 4u * (uint32_t)X;
+  // chunk kCPW is only needed on lane 0; other lanes use an offset at the end
+  // of the plane, which returns 0 without touching memory
+  const uint32_t voff_x = lane == 0 ? voff + 256u * kCPW : plane_bytes;
+  const uint32_t voffW_x = lane == 0 ? voffW + 256u * kCPW : plane_bytes;
   // Every load is unconditional (no branch around it), so the compiler can
   // keep the next plane's loads in flight while the current planes are
   // classified.  A plane that is not needed gets an empty resource.
@@ -237,11 +257,11 @@ __global__ __launch_bounds__(256) void count_kernel(ExtractParams P) {
         need ? (int)plane_bytes : 0, kBufferWord3);
 #pragma unroll
     for (int j = 0; j < kCPW; ++j) {
-      U[j] = __builtin_amdgcn_raw_buffer_load_b32(rs, voff, 256 * j, 0);
-      W[j] = __builtin_amdgcn_raw_buffer_load_b32(rs, voff, 256 * j + 4 * X, 0);
+      U[j] = __builtin_amdgcn_raw_buffer_load_b32(rs, voff + 256u * j, 0, 0);
+      W[j] = __builtin_amdgcn_raw_buffer_load_b32(rs, voffW + 256u * j, 0, 0);
     }
     U[kCPW] = __builtin_amdgcn_raw_buffer_load_b32(rs, voff_x, 0, 0);
-    W[kCPW] = __builtin_amdgcn_raw_buffer_load_b32(rs, voff_x, 4 * X, 0);
+    W[kCPW] = __builtin_amdgcn_raw_buffer_load_b32(rs, voffW_x, 0, 0);
   };
   load_plane(z0, true, A, B);
   load_plane(z0 + 1, true, C, D);
@@ -268,6 +288,8 @@ __global__ __launch_bounds__(256) void count_kernel(ExtractParams P) {
   };
 
   int64_t used = 0;
+  uint64_t nzmask = 0;  // planes zb + i of this column whose tile has vertices
+  int zb = z0;
   for (int z = z0; z < z1; ++z) {
     load_plane(z + 2, z + 1 < z1, E, F);
     const int64_t t = V.tile_begin + (int64_t)z * V.R + r;
@@ -326,6 +348,22 @@ __global__ __launch_bounds__(256) void count_kernel(ExtractParams P) {
       if (dense) atomicAdd(&P.hdr->dense_acc, 1u);
     }
     if (!dense) used += nrec;
+    // append the column's non-empty tiles to the list, 64 planes at a time
+    if (vsum > 0) nzmask |= 1ull << (z - zb);
+    if (z - zb == 63 || z + 1 == z1) {
+      if (nzmask) {
+        const uint32_t n = (uint32_t)__popcll(nzmask);
+        uint32_t base = 0;
+        if (lane == 0) base = atomicAdd(&P.hdr->nz_count, n);
+        base = __shfl(base, 0);
+        if ((nzmask >> lane) & 1ull) {
+          const uint32_t i = (uint32_t)__popcll(nzmask & ((1ull << lane) - 1ull));
+          P.nz_list[base + i] = (uint32_t)(V.tile_begin + (int64_t)(zb + lane) * V.R + r);
+        }
+      }
+      nzmask = 0;
+      zb = z + 1;
+    }
 
 #pragma unroll
     for (int j = 0; j <= kCPW; ++j) {
@@ -334,6 +372,267 @@ __global__ __launch_bounds__(256) void count_kernel(ExtractParams P) {
       C[j] = E[j];
       D[j] = F[j];
     }
+  }
+}
+
+// ============================================================ K1, wide form
+// SHORT2 volumes with X % 4 == 0 (16-byte aligned planes and rows).  128-bit
+// loads: in chunk c (256 cubes) of a tile, lane L holds the voxels 4L..4L+3
+// of each of the four corner rows (rows y, y+1 of planes z, z+1).  Per voxel
+// position the four rows are reduced in the vector ALU with packed 16-bit
+// min/max (low half: min and max tsdf; high half: min weight), and three
+// ballots per position give "some row negative", "some row non-negative" and
+// "some weight zero" over the 64 lanes.  A cube (positions k, k + 1) is active
+// when its two positions together have both signs and no zero weight; only
+// position 4 (the next lane's position 0) needs a one-bit mask shift.  Active
+// cube 4L + k of the chunk is in cube order, so a record's rank within the
+// tile is a sum of masked popcounts.  Records go to the column's quota, or,
+// when the tile does not fit there, to a shared overflow pool (one atomic per
+// overflowing tile); only when that is exhausted does the tile take the
+// staged path.  No active-cube masks are written (the backward runs on fp32
+// volumes, which take the 32-bit kernel).
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+typedef short s16x2 __attribute__((ext_vector_type(2)));
+typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ uint32_t min_s16x2(uint32_t a, uint32_t b) {
+  return __builtin_bit_cast(uint32_t, __builtin_elementwise_min(__builtin_bit_cast(s16x2, a),
+                                                                __builtin_bit_cast(s16x2, b)));
+}
+__device__ __forceinline__ uint32_t max_s16x2(uint32_t a, uint32_t b) {
+  return __builtin_bit_cast(uint32_t, __builtin_elementwise_max(__builtin_bit_cast(s16x2, a),
+                                                                __builtin_bit_cast(s16x2, b)));
+}
+__device__ __forceinline__ uint32_t min_u16x2(uint32_t a, uint32_t b) {
+  return __builtin_bit_cast(uint32_t, __builtin_elementwise_min(__builtin_bit_cast(u16x2, a),
+                                                                __builtin_bit_cast(u16x2, b)));
+}
+
+// Workgroup id remapped so that the workgroups one XCD runs (ids b, b + 8,
+// ...: dispatch is round-robin over the 8 XCDs) are consecutive: neighbouring
+// columns then share a row through that XCD's L2.
+__device__ __forceinline__ uint32_t xcd_block() {
+  const uint32_t nb = gridDim.x, b = blockIdx.x;
+  const uint32_t q = nb >> 3, rem = nb & 7u, x = b & 7u;
+  return x * q + (x < rem ? x : rem) + (b >> 3);
+}
+
+struct Plane4 {
+  u32x4 u[2], w[2];  // rows y (u) and y + 1 (w) of chunks 0 and 1
+  uint32_t xu, xw;   // word 512 of the two rows (lane 0; neighbour of chunk 1's lane 63)
+};
+
+template <bool BATCH>
+__global__ __launch_bounds__(256) void count4_kernel(ExtractParams P) {
+  __shared__ uint8_t s_nv[256];
+  s_nv[threadIdx.x] = dNumVerts[threadIdx.x];
+  __syncthreads();
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  const int64_t col = (int64_t)xcd_block() * 4 + wave;
+  if (col >= P.ncols) return;  // whole waves exit; no barrier follows
+
+  int vi, r, z0, z1;
+  if constexpr (BATCH) {
+    const Column c = P.cols[col];
+    vi = c.vol;
+    r = c.r;
+    z0 = c.z0;
+    z1 = c.z1;
+  } else {
+    vi = 0;
+    r = (int)(col % P.vol0.R);
+    const int s = (int)(col / P.vol0.R);
+    z0 = s * P.slab;
+    z1 = min(z0 + P.slab, P.vol0.Z - 1);
+  }
+  const VolDesc& V = vol_at<BATCH>(P, vi);
+  const int X = V.X;
+  const int64_t XY = (int64_t)X * V.Y;
+  const int64_t p0 = (int64_t)r * kTile;
+  const int64_t rec_base = col * P.quota;
+
+  // lanes whose 4 voxels lie on a row without cubes (the last row, or past the
+  // plane), and lanes at the end of a row (their position 4 starts the next row)
+  uint64_t norow[2], rowend[2];
+#pragma unroll
+  for (int c = 0; c < 2; ++c) {
+    const int64_t p = p0 + 256 * c + 4 * lane;
+    bool ok = false, end = false;
+    if (p < XY) {
+      const uint32_t y = V.divX.div((uint32_t)p);
+      const uint32_t x = (uint32_t)p - y * (uint32_t)X;
+      ok = y + 1 < (uint32_t)V.Y;
+      end = x + 4 == (uint32_t)X;
+    }
+    norow[c] = __ballot(!ok);
+    rowend[c] = __ballot(end);
+  }
+
+  // Offsets in the VGPR offset (range-checked against the plane size; a
+  // 16-byte load is either wholly inside the plane or wholly outside).
+  const uint32_t plane_bytes = (uint32_t)(XY * 4);
+  const uint32_t voff = (uint32_t)(p0 + 4 * lane) * 4u;
+  const uint32_t voffW = voff + 4u * (uint32_t)X;
+  const uint32_t voff_x = lane == 0 ? (uint32_t)(p0 + kTile) * 4u : plane_bytes;
+  const uint32_t voffW_x = lane == 0 ? voff_x + 4u * (uint32_t)X : plane_bytes;
+  auto load_plane = [&](int q, bool need, Plane4& L) {
+    const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
+        const_cast<uint32_t*>(V.vol + (need ? (int64_t)q * XY : 0)), (short)0,
+        need ? (int)plane_bytes : 0, kBufferWord3);
+    L.u[0] = __builtin_amdgcn_raw_buffer_load_b128(rs, voff, 0, 0);
+    L.u[1] = __builtin_amdgcn_raw_buffer_load_b128(rs, voff + 1024u, 0, 0);
+    L.w[0] = __builtin_amdgcn_raw_buffer_load_b128(rs, voffW, 0, 0);
+    L.w[1] = __builtin_amdgcn_raw_buffer_load_b128(rs, voffW + 1024u, 0, 0);
+    L.xu = __builtin_amdgcn_raw_buffer_load_b32(rs, voff_x, 0, 0);
+    L.xw = __builtin_amdgcn_raw_buffer_load_b32(rs, voffW_x, 0, 0);
+  };
+  Plane4 A, B, C;
+  load_plane(z0, true, A);
+  load_plane(z0 + 1, true, B);
+
+  int64_t used = 0;
+  uint64_t nzmask = 0;  // planes zb + i of this column whose tile has vertices
+  int zb = z0;
+  for (int z = z0; z < z1; ++z) {
+    load_plane(z + 2, z + 1 < z1, C);
+    const int64_t t = V.tile_begin + (int64_t)z * V.R + r;
+
+    // ---- pass 1: active masks m[c][k] (bit L: cube 256c + 4L + k)
+    uint64_t m[2][4];
+    uint64_t nxn, nxp, nxw;  // bit 0: position 0 of the next chunk's lane 0
+    {
+      const uint32_t mn = min_s16x2(min_s16x2(A.xu, A.xw), min_s16x2(B.xu, B.xw));
+      const uint32_t mx = max_s16x2(max_s16x2(A.xu, A.xw), max_s16x2(B.xu, B.xw));
+      const uint32_t mw = min_u16x2(min_u16x2(A.xu, A.xw), min_u16x2(B.xu, B.xw));
+      nxn = __ballot((int16_t)mn < 0);
+      nxp = __ballot((int16_t)mx >= 0);
+      nxw = __ballot(mw < 0x10000u);
+    }
+#pragma unroll
+    for (int c = 1; c >= 0; --c) {
+      uint64_t bn[5], bp[5], bw[5];
+#pragma unroll
+      for (int pos = 0; pos < 4; ++pos) {
+        const uint32_t a = A.u[c][pos], b = A.w[c][pos], cc = B.u[c][pos], d = B.w[c][pos];
+        const uint32_t mn = min_s16x2(min_s16x2(a, b), min_s16x2(cc, d));
+        const uint32_t mx = max_s16x2(max_s16x2(a, b), max_s16x2(cc, d));
+        const uint32_t mw = min_u16x2(min_u16x2(a, b), min_u16x2(cc, d));
+        bn[pos] = __ballot((int16_t)mn < 0);
+        bp[pos] = __ballot((int16_t)mx >= 0);
+        bw[pos] = __ballot(mw < 0x10000u) | norow[c];
+      }
+      bn[4] = (bn[0] >> 1) | (nxn << 63);
+      bp[4] = (bp[0] >> 1) | (nxp << 63);
+      bw[4] = (bw[0] >> 1) | (nxw << 63) | rowend[c];
+#pragma unroll
+      for (int k = 0; k < 4; ++k)
+        m[c][k] = (bn[k] | bn[k + 1]) & (bp[k] | bp[k + 1]) & ~(bw[k] | bw[k + 1]);
+      nxn = bn[0];
+      nxp = bp[0];
+      nxw = bw[0];
+    }
+    int64_t nrec = 0;
+#pragma unroll
+    for (int c = 0; c < 2; ++c)
+#pragma unroll
+      for (int k = 0; k < 4; ++k) nrec += __popcll(m[c][k]);
+
+    // ---- destination of the tile's records
+    const int64_t room = P.quota - used;
+    int64_t rbase = rec_base + used;
+    bool dense = false, in_quota = true;
+    if (nrec > room) {
+      in_quota = false;
+      unsigned long long ob = 0;
+      if (lane == 0) ob = atomicAdd(reinterpret_cast<unsigned long long*>(&P.hdr->ovf_next),
+                                    (unsigned long long)nrec);
+      ob = __shfl(ob, 0);
+      if ((int64_t)ob + nrec <= P.ovf_cap) rbase = P.ovf_begin + (int64_t)ob;
+      else dense = true;
+    }
+
+    // ---- pass 2: cases, vertex counts and records of the active cubes
+    uint32_t vsum = 0;
+    int64_t cbase = 0;
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+      const uint64_t any = m[c][0] | m[c][1] | m[c][2] | m[c][3];
+      if (any == 0) continue;
+      uint32_t pre = 0;
+#pragma unroll
+      for (int k = 0; k < 4; ++k)
+        pre += __builtin_amdgcn_mbcnt_hi((uint32_t)(m[c][k] >> 32),
+                                         __builtin_amdgcn_mbcnt_lo((uint32_t)m[c][k], 0u));
+      // position 4 of the lane: the next lane's position 0; lane 63 takes
+      // lane 0 of chunk 1 (c = 0) or the extra words (c = 1)
+      uint32_t nau = 0, naw = 0, nbu = 0, nbw = 0;
+      if (m[c][3]) {
+        nau = __shfl_down(A.u[c][0], 1);
+        naw = __shfl_down(A.w[c][0], 1);
+        nbu = __shfl_down(B.u[c][0], 1);
+        nbw = __shfl_down(B.w[c][0], 1);
+        const uint32_t lau = __builtin_amdgcn_readlane(c == 0 ? A.u[1][0] : A.xu, 0);
+        const uint32_t law = __builtin_amdgcn_readlane(c == 0 ? A.w[1][0] : A.xw, 0);
+        const uint32_t lbu = __builtin_amdgcn_readlane(c == 0 ? B.u[1][0] : B.xu, 0);
+        const uint32_t lbw = __builtin_amdgcn_readlane(c == 0 ? B.w[1][0] : B.xw, 0);
+        if (lane == 63) {
+          nau = lau;
+          naw = law;
+          nbu = lbu;
+          nbw = lbw;
+        }
+      }
+      uint32_t own = 0;
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        if ((m[c][k] >> lane) & 1ull) {
+          const uint32_t a0 = A.u[c][k], a3 = A.w[c][k], b0 = B.u[c][k], b3 = B.w[c][k];
+          const uint32_t a1 = k < 3 ? A.u[c][(k + 1) & 3] : nau;
+          const uint32_t a2 = k < 3 ? A.w[c][(k + 1) & 3] : naw;
+          const uint32_t b1 = k < 3 ? B.u[c][(k + 1) & 3] : nbu;
+          const uint32_t b2 = k < 3 ? B.w[c][(k + 1) & 3] : nbw;
+          const uint32_t cw[8] = {a0, a1, a2, a3, b0, b1, b2, b3};
+          const uint32_t ci = cube_index<MCX_ELEM_SHORT2>(cw);
+          vsum += s_nv[ci];
+          if (!dense) {
+            uint32_t* rp = P.rec + (rbase + cbase + pre + own) * kRecWords;
+            rp[0] = ((uint32_t)(256 * c + 4 * lane + k) << 8) | ci;
+#pragma unroll
+            for (int q = 0; q < 8; ++q) rp[1 + q] = cw[q];
+          }
+          ++own;
+        }
+      }
+      cbase += __popcll(m[c][0]) + __popcll(m[c][1]) + __popcll(m[c][2]) + __popcll(m[c][3]);
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) vsum += __shfl_xor(vsum, o);
+    if (lane == 0) {
+      P.tileT[t] = (int32_t)vsum;
+      P.tile_nrec[t] = dense ? -1 : (int32_t)nrec;
+      P.tile_rec[t] = rbase;
+      if (dense) atomicAdd(&P.hdr->dense_acc, 1u);
+    }
+    if (in_quota) used += nrec;
+    // append the column's non-empty tiles to the list, 64 planes at a time
+    if (vsum > 0) nzmask |= 1ull << (z - zb);
+    if (z - zb == 63 || z + 1 == z1) {
+      if (nzmask) {
+        const uint32_t n = (uint32_t)__popcll(nzmask);
+        uint32_t base = 0;
+        if (lane == 0) base = atomicAdd(&P.hdr->nz_count, n);
+        base = __shfl(base, 0);
+        if ((nzmask >> lane) & 1ull) {
+          const uint32_t i = (uint32_t)__popcll(nzmask & ((1ull << lane) - 1ull));
+          P.nz_list[base + i] = (uint32_t)(V.tile_begin + (int64_t)(zb + lane) * V.R + r);
+        }
+      }
+      nzmask = 0;
+      zb = z + 1;
+    }
+    A = B;
+    B = C;
   }
 }
 
@@ -380,6 +679,9 @@ __global__ __launch_bounds__(256) void scan_kernel(ExtractParams P) {
         if (P.d_count) *P.d_count = excl + total;
         P.hdr->dense_last = P.hdr->dense_acc;  // K1 has finished: hand the count over
         P.hdr->dense_acc = 0;
+        P.hdr->nz_total = P.hdr->nz_count;     // the emit kernel's work list
+        P.hdr->nz_count = 0;
+        P.hdr->ovf_next = 0;
         // every block has read the parity (the look-back saw all their
         // aggregates): flip it for the next call
         __hip_atomic_store(&P.hdr->parity, parity ^ 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -414,12 +716,41 @@ __global__ __launch_bounds__(256) void vbase_kernel(ExtractParams P) {
 }
 
 // ============================================================ K3: emit
-// Output vertex `idx`: slot q of cube (x, y, z) with case ci and corner words
-// cw[8] in the reference corner order.
+// Exclusive scan of one int per thread over a 256-thread workgroup; returns
+// the thread's exclusive prefix and the total through *tot (s_tmp: 4 ints).
+__device__ __forceinline__ int block_excl_scan(int v, int* s_tmp, int* tot) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const uint32_t incl = wave_incl_scan32((uint32_t)v);
+  if (lane == 63) s_tmp[wave] = (int)incl;
+  __syncthreads();
+  int pre = 0, all = 0;
+#pragma unroll
+  for (int w = 0; w < 4; ++w) {
+    if (w < wave) pre += s_tmp[w];
+    all += s_tmp[w];
+  }
+  __syncthreads();
+  *tot = all;
+  return (int)incl - v + pre;
+}
+
+struct TileInfo {      // backward BW1 per-tile info in LDS (10 words)
+  int64_t delta;       // output index of the tile's first record vertex minus
+                       // the item-list vertex prefix at that record
+  int64_t rstart;      // first record in scratch
+  int64_t t;           // tile id
+  int32_t nrec;        // records (-1: staged path)
+  int32_t T;           // vertices
+  int32_t lbeg;        // first position in the item's record list
+  int32_t vi;          // volume
+};
+static_assert(sizeof(TileInfo) == 40, "tile info is 10 words");
+
+// Write one output vertex directly (wave-level emission).
 template <int ELEM>
-__device__ __forceinline__ void emit_vertex(const ExtractParams& P, const VolDesc& V,
-                                            const uint64_t* s_case, const uint32_t cw[8], int x,
-                                            int y, int z, uint32_t ci, int q, int64_t idx) {
+__device__ __forceinline__ void write_vertex(const ExtractParams& P, const VolDesc& V,
+                                             const uint64_t* s_case, const uint32_t cw[8], int x,
+                                             int y, int z, uint32_t ci, int q, int64_t idx) {
   if (idx >= P.cap) return;
   // output slot q = 3*tri + r holds table corner 2 - r of triangle tri (:324-332)
   const int m = 3 * (q / 3) + 2 - (q % 3);
@@ -427,8 +758,8 @@ __device__ __forceinline__ void emit_vertex(const ExtractParams& P, const VolDes
   const int ca = (int)((kEdgeA >> (3 * e)) & 7u);
   const int cb = (int)((kEdgeB >> (3 * e)) & 7u);
   const float h = V.h;
-  const float fa = unpack_f<ELEM>(cw[ca]);
-  const float fb = unpack_f<ELEM>(cw[cb]);
+  const float fa = unpack_f<ELEM>(pick8(cw, ca));
+  const float fb = unpack_f<ELEM>(pick8(cw, cb));
   const float pax = ((float)(x + corner_dx(ca)) + 0.5f) * h;
   const float pay = ((float)(y + corner_dy(ca)) + 0.5f) * h;
   const float paz = ((float)(z + corner_dz(ca)) + 0.5f) * h;
@@ -452,7 +783,7 @@ __device__ __forceinline__ void emit_vertex(const ExtractParams& P, const VolDes
     li = ((int64_t)vz * V.Y + vy) * V.X + vx;
     if constexpr (ELEM == MCX_ELEM_SHORT2) {
       const unsigned dx = (unsigned)(vx - x), dy = (unsigned)(vy - y), dz = (unsigned)(vz - z);
-      tw = (dx | dy | dz) <= 1u ? cw[corner_of((int)dx, (int)dy, (int)dz)] : V.vol[li];
+      tw = (dx | dy | dz) <= 1u ? pick8(cw, corner_of((int)dx, (int)dy, (int)dz)) : V.vol[li];
     }
   }
   if (P.vv) {
@@ -480,219 +811,108 @@ __device__ __forceinline__ void emit_vertex(const ExtractParams& P, const VolDes
   if (V.ver_voxel && inb) V.ver_voxel[li] = (int16_t)(tw & 0xffffu);
 }
 
-// Exclusive scan of one int per thread over a 256-thread workgroup; returns
-// the thread's exclusive prefix and the total through *tot (s_tmp: 4 ints).
-__device__ __forceinline__ int block_excl_scan(int v, int* s_tmp, int* tot) {
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const uint32_t incl = wave_incl_scan32((uint32_t)v);
-  if (lane == 63) s_tmp[wave] = (int)incl;
-  __syncthreads();
-  int pre = 0, all = 0;
-#pragma unroll
-  for (int w = 0; w < 4; ++w) {
-    if (w < wave) pre += s_tmp[w];
-    all += s_tmp[w];
-  }
-  __syncthreads();
-  *tot = all;
-  return (int)incl - v + pre;
-}
-
-// LDS word layout of the emit kernel
-constexpr int kEmCase = 0;    // 256 x u64
-constexpr int kEmNv = 512;    // 256 x u8
-constexpr int kEmTmp = 576;   // scan scratch (8 words)
-constexpr int kEmTile = 584;  // per-tile info, kTilesPerEmitGroup x 8 words
-constexpr int kEmMap = kEmTile + 8 * kTilesPerEmitGroup;  // kEmitThreads map entries
-constexpr int kEmDat = kEmMap + kEmitThreads;              // records or staged voxel ranges
-
-struct TileInfo {      // 8 words in LDS
-  int64_t delta;       // output index of the tile's first record vertex minus
-                       // the group-list vertex prefix at that record
-  int64_t rstart;      // first record in scratch
-  int32_t nrec;        // records (-1: staged path)
-  int32_t T;           // vertices
-  int32_t lbeg;        // first position in the group's record list
-  int32_t vi;          // volume
-};
-static_assert(sizeof(TileInfo) == 32, "tile info is 8 words");
-
-// Emit the vertices of one tile from its two staged voxel ranges (the path
-// for tiles whose records did not fit in the scratch quota).
+// Emit the vertices of up to 64 cubes held one per lane (record word 0 =
+// offset in tile << 8 | case, corner words cw; nv = 0 for lanes without a
+// cube), in lane order, to output indices out, out + 1, ...  Returns the
+// number of vertices (wave-uniform).
 template <int ELEM>
-__device__ void emit_staged(const ExtractParams& P, const VolDesc& V, int64_t t, int64_t vbeg,
-                            const uint64_t* s_case, const uint8_t* s_nv, int* s_tmp,
-                            uint32_t* s_map, uint32_t* s_dat) {
-  const int tid = threadIdx.x;
-  const int X = V.X;
-  const int64_t XY = (int64_t)X * V.Y;
-  const int64_t local = t - V.tile_begin;
-  const int z = (int)V.divR.div((uint32_t)local);
-  const int64_t p0 = (local - (int64_t)z * V.R) * kTile;
-  const int plane_words = P.emit_plane_words;
-  const int64_t N = XY * V.Z;
-  for (int pz = 0; pz < 2; ++pz) {
-    const int64_t start = (int64_t)(z + pz) * XY + p0;
-    int64_t len = N - start;
-    if (len > (int64_t)kTile + X + 1) len = (int64_t)kTile + X + 1;
-    uint32_t* dst = s_dat + pz * plane_words;
-    for (int64_t k = tid; k < len; k += kEmitThreads) dst[k] = V.vol[start + k];
-  }
-  __syncthreads();
-  // thread owns cubes 4*tid .. 4*tid+3 of the tile (those below kTile)
-  uint32_t ciw = 0;
-  int cnt = 0;
+__device__ __forceinline__ int emit_lanes(const ExtractParams& P, const VolDesc& V,
+                                          const uint64_t* s_case, uint32_t w0,
+                                          const uint32_t cw[8], int nv, int z, int64_t p0,
+                                          int64_t out) {
+  const int lane = threadIdx.x & 63;
+  const uint32_t incl = wave_incl_scan32((uint32_t)nv);
+  const int ex = (int)incl - nv;
+  const int tot = __shfl((int)incl, 63);
+  for (int vb = 0; vb < tot; vb += 64) {
+    const int j = vb + lane;
+    // the last lane whose first vertex is at or before j (lanes without
+    // vertices share their start with the next lane, so this one holds j)
+    int lo = 0;
 #pragma unroll
-  for (int k = 0; k < 4; ++k) {
-    const int o = 4 * tid + k;
-    const int64_t p = p0 + o;
-    uint32_t ci = 0;
-    if (o < kTile && p < XY) {
-      const uint32_t y = V.divX.div((uint32_t)p);
-      const uint32_t x = (uint32_t)p - y * (uint32_t)X;
-      if (x + 1 < (uint32_t)X && y + 1 < (uint32_t)V.Y) {
-        const uint32_t* a = s_dat + o;
-        const uint32_t* b = s_dat + plane_words + o;
-        const uint32_t cw[8] = {a[0], a[1], a[X + 1], a[X], b[0], b[1], b[X + 1], b[X]};
-        ci = cube_index<ELEM>(cw);
-      }
+    for (int step = 32; step > 0; step >>= 1) {
+      const int ex_mid = __shfl(ex, lo + step);
+      if (ex_mid <= j) lo += step;
     }
-    ciw |= ci << (8 * k);
-    cnt += s_nv[ci];
-  }
-  int tot;
-  const int ex = block_excl_scan(cnt, s_tmp, &tot);
-  for (int vb = 0; vb < tot; vb += kEmitThreads) {
-    int off = ex;
+    const int q = j - __shfl(ex, lo);
+    const uint32_t rw = __shfl(w0, lo);
+    uint32_t c[8];
 #pragma unroll
-    for (int k = 0; k < 4; ++k) {
-      const uint32_t ci = (ciw >> (8 * k)) & 0xffu;
-      const int nv = s_nv[ci];
-      for (int q = 0; q < nv; ++q) {
-        const int j = off + q - vb;
-        if (j >= 0 && j < kEmitThreads)
-          s_map[j] = ((uint32_t)(4 * tid + k) << 12) | (ci << 4) | (uint32_t)q;
-      }
-      off += nv;
-    }
-    __syncthreads();
-    if (vb + tid < tot) {
-      const uint32_t e = s_map[tid];
-      const int o = (int)(e >> 12);
-      const uint32_t* a = s_dat + o;
-      const uint32_t* b = s_dat + plane_words + o;
-      const uint32_t cw[8] = {a[0], a[1], a[X + 1], a[X], b[0], b[1], b[X + 1], b[X]};
-      const int64_t p = p0 + o;
+    for (int k = 0; k < 8; ++k) c[k] = __shfl(cw[k], lo);
+    if (j < tot) {
+      const int64_t p = p0 + (rw >> 8);
       const uint32_t y = V.divX.div((uint32_t)p);
-      const int x = (int)((uint32_t)p - y * (uint32_t)X);
-      emit_vertex<ELEM>(P, V, s_case, cw, x, (int)y, z, (e >> 4) & 0xffu, (int)(e & 15u),
-                        vbeg + vb + tid);
+      const int x = (int)((uint32_t)p - y * (uint32_t)V.X);
+      write_vertex<ELEM>(P, V, s_case, c, x, (int)y, z, rw & 0xffu, q, out + j);
     }
-    __syncthreads();
   }
+  return tot;
 }
 
+// Wave-level emission: wave w of the grid takes list entries w, w + nwaves,
+// ... (no atomics: one counter shared by every wave serialises).  A tile's vertices are
+// [voff[t], voff[t] + T), in cube order, so the output does not depend on
+// which wave takes which tile.  Records are taken 64 at a time, one per lane;
+// a tile whose records did not fit the scratch is re-read from the volume, 64
+// cubes at a time, into the same per-lane form.
 template <int ELEM, bool BATCH>
 __global__ __launch_bounds__(kEmitThreads) void emit_kernel(ExtractParams P) {
-  extern __shared__ uint32_t smem[];
-  uint64_t* s_case = reinterpret_cast<uint64_t*>(smem + kEmCase);
-  uint8_t* s_nv = reinterpret_cast<uint8_t*>(smem + kEmNv);
-  int* s_tmp = reinterpret_cast<int*>(smem + kEmTmp);
-  TileInfo* s_tile = reinterpret_cast<TileInfo*>(smem + kEmTile);
-  uint32_t* s_map = smem + kEmMap;
-  uint32_t* s_dat = smem + kEmDat;
-  const int tid = threadIdx.x;
-  const int64_t t0 = (int64_t)blockIdx.x * kTilesPerEmitGroup;
-  s_case[tid] = dCase[tid];
-  s_nv[tid] = dNumVerts[tid];
-  if (tid < kTilesPerEmitGroup) {
-    const int64_t t = t0 + tid;
-    TileInfo ti{};
-    if (t < P.n_tiles) {
-      ti.T = P.tileT[t];
-      ti.nrec = P.tile_nrec[t];
-      ti.rstart = P.tile_rec[t];
-      ti.delta = P.voff[t];
-      ti.vi = BATCH ? volume_of_tile(P, t) : 0;
-    }
-    s_tile[tid] = ti;
-  }
-  __syncthreads();
-  if (tid == 0) {
-    // group record list: records of the tiles in order; the vertices of a
-    // tile's records are the tile's vertices, so a record's output index is
-    // delta[tile] + (list prefix of vertex counts before the record)
-    int lpos = 0;
-    int64_t vpre = 0;
-    for (int u = 0; u < kTilesPerEmitGroup; ++u) {
-      TileInfo& ti = s_tile[u];
-      ti.lbeg = lpos;
-      ti.delta -= vpre;
-      if (ti.T > 0 && ti.nrec >= 0) {
-        lpos += ti.nrec;
-        vpre += ti.T;
-      }
-    }
-    s_tmp[7] = lpos;
-  }
-  __syncthreads();
-  const int nlist = s_tmp[7];
-
-  int64_t carry = 0;  // vertex prefix of the list chunks before this one
-  for (int rb = 0; rb < nlist; rb += kEmitThreads) {
-    const int k = rb + tid;
-    int nv = 0, u = 0;
-    uint32_t* my = s_dat + tid * kRecWords;
-    if (k < nlist) {
-      while (u + 1 < kTilesPerEmitGroup &&
-             !(s_tile[u].T > 0 && s_tile[u].nrec >= 0 && k < s_tile[u].lbeg + s_tile[u].nrec))
-        ++u;
-      const uint32_t* src = P.rec + (s_tile[u].rstart + (k - s_tile[u].lbeg)) * kRecWords;
+  __shared__ uint64_t s_case[256];
+  s_case[threadIdx.x] = dCase[threadIdx.x];
+  __syncthreads();  // the only workgroup barrier: waves run independently below
+  const int lane = threadIdx.x & 63;
+  const uint32_t nz = P.hdr->nz_total;
+  const uint32_t nwaves = gridDim.x * (kEmitThreads / 64);
+  const uint32_t wave0 = blockIdx.x * (kEmitThreads / 64) +
+                         (uint32_t)__builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  for (uint32_t item = wave0; item < nz; item += nwaves) {
+    const int64_t t = P.nz_list[item];
+    const int nrec = P.tile_nrec[t];
+    int64_t out = P.voff[t];
+    const int vi = BATCH ? volume_of_tile(P, t) : 0;
+    const VolDesc& V = vol_at<BATCH>(P, vi);
+    const int64_t local = t - V.tile_begin;
+    const int z = (int)V.divR.div((uint32_t)local);
+    const int64_t p0 = (local - (int64_t)z * V.R) * kTile;
+    if (nrec >= 0) {
+      const uint32_t* rec = P.rec + P.tile_rec[t] * kRecWords;
+      for (int rb = 0; rb < nrec; rb += 64) {
+        const int k = rb + lane;
+        uint32_t w0 = 0, cw[8] = {};
+        int nv = 0;
+        if (k < nrec) {
+          const uint32_t* src = rec + (int64_t)k * kRecWords;
+          w0 = src[0];
 #pragma unroll
-      for (int q = 0; q < kRecWords; ++q) my[q] = src[q];
-      nv = s_nv[my[0] & 0xffu];
-    }
-    int tot;
-    const int ex = block_excl_scan(nv, s_tmp, &tot);
-    // record tile and start offset for the vertex pass
-    const uint32_t rec_u = (uint32_t)u;
-    for (int vb = 0; vb < tot; vb += kEmitThreads) {
-      for (int q = 0; q < nv; ++q) {
-        const int j = ex + q - vb;
-        if (j >= 0 && j < kEmitThreads)
-          s_map[j] = ((uint32_t)tid << 8) | (rec_u << 4) | (uint32_t)q;
+          for (int q = 0; q < 8; ++q) cw[q] = src[1 + q];
+          nv = dNumVerts[w0 & 0xffu];
+        }
+        out += emit_lanes<ELEM>(P, V, s_case, w0, cw, nv, z, p0, out);
       }
-      __syncthreads();
-      if (vb + tid < tot) {
-        const uint32_t e = s_map[tid];
-        const int rk = (int)(e >> 8);
-        const int ut = (int)((e >> 4) & 15u);
-        const int q = (int)(e & 15u);
-        const uint32_t* rc = s_dat + rk * kRecWords;
-        const TileInfo& ti = s_tile[ut];
-        const VolDesc& V = vol_at<BATCH>(P, ti.vi);
-        const int64_t local = t0 + ut - V.tile_begin;
-        const int z = (int)V.divR.div((uint32_t)local);
-        const int64_t p = (local - (int64_t)z * V.R) * kTile + (rc[0] >> 8);
-        const uint32_t y = V.divX.div((uint32_t)p);
-        const int x = (int)((uint32_t)p - y * (uint32_t)V.X);
-        const uint32_t cw[8] = {rc[1], rc[2], rc[3], rc[4], rc[5], rc[6], rc[7], rc[8]};
-        emit_vertex<ELEM>(P, V, s_case, cw, x, (int)y, z, rc[0] & 0xffu, q,
-                          ti.delta + carry + vb + tid);
+    } else {
+      const int X = V.X;
+      const int64_t XY = (int64_t)X * V.Y;
+      const uint32_t* vol = V.vol + (int64_t)z * XY;
+      for (int cb = 0; cb < kTile; cb += 64) {
+        const int o = cb + lane;
+        const int64_t p = p0 + o;
+        uint32_t w0 = 0, cw[8] = {};
+        int nv = 0;
+        if (p < XY) {
+          const uint32_t y = V.divX.div((uint32_t)p);
+          const uint32_t x = (uint32_t)p - y * (uint32_t)X;
+          if (x + 1 < (uint32_t)X && y + 1 < (uint32_t)V.Y) {
+            const uint32_t* a = vol + p;
+            const uint32_t* b = a + XY;
+            cw[0] = a[0]; cw[1] = a[1]; cw[2] = a[X + 1]; cw[3] = a[X];
+            cw[4] = b[0]; cw[5] = b[1]; cw[6] = b[X + 1]; cw[7] = b[X];
+            const uint32_t ci = cube_index<ELEM>(cw);
+            w0 = ((uint32_t)o << 8) | ci;
+            nv = dNumVerts[ci];
+          }
+        }
+        out += emit_lanes<ELEM>(P, V, s_case, w0, cw, nv, z, p0, out);
       }
-      __syncthreads();
-    }
-    carry += tot;
-  }
-
-  // tiles whose records overflowed the quota
-  for (int u = 0; u < kTilesPerEmitGroup; ++u) {
-    const TileInfo ti = s_tile[u];
-    if (ti.T > 0 && ti.nrec < 0) {
-      const int64_t t = t0 + u;
-      const VolDesc& V = vol_at<BATCH>(P, ti.vi);
-      emit_staged<ELEM>(P, V, t, P.voff[t], s_case, s_nv, s_tmp, s_map, s_dat);
-      __syncthreads();
     }
   }
 }
@@ -711,12 +931,12 @@ __global__ __launch_bounds__(kEmitThreads) void contrib_kernel(ExtractParams P, 
   __shared__ uint64_t s_case[256];
   __shared__ uint8_t s_nv[256];
   __shared__ int s_tmp[8];
-  __shared__ TileInfo s_tile[kTilesPerEmitGroup];
+  __shared__ TileInfo s_tile[kTilesPerContribGroup];
   const int tid = threadIdx.x;
-  const int64_t t0 = (int64_t)blockIdx.x * kTilesPerEmitGroup;
+  const int64_t t0 = (int64_t)blockIdx.x * kTilesPerContribGroup;
   s_case[tid] = dCase[tid];
   s_nv[tid] = dNumVerts[tid];
-  if (tid < kTilesPerEmitGroup) {
+  if (tid < kTilesPerContribGroup) {
     const int64_t t = t0 + tid;
     TileInfo ti{};
     if (t < P.n_tiles) {
@@ -732,7 +952,7 @@ __global__ __launch_bounds__(kEmitThreads) void contrib_kernel(ExtractParams P, 
   if (tid == 0) {
     int lpos = 0;
     int64_t vpre = 0;
-    for (int u = 0; u < kTilesPerEmitGroup; ++u) {
+    for (int u = 0; u < kTilesPerContribGroup; ++u) {
       TileInfo& ti = s_tile[u];
       ti.lbeg = lpos;
       ti.delta -= vpre;
@@ -752,7 +972,7 @@ __global__ __launch_bounds__(kEmitThreads) void contrib_kernel(ExtractParams P, 
     uint32_t rc[kRecWords] = {};
     int64_t ri = 0;
     if (k < nlist) {
-      while (u + 1 < kTilesPerEmitGroup &&
+      while (u + 1 < kTilesPerContribGroup &&
              !(s_tile[u].T > 0 && s_tile[u].nrec >= 0 && k < s_tile[u].lbeg + s_tile[u].nrec))
         ++u;
       ri = s_tile[u].rstart + (k - s_tile[u].lbeg);
@@ -866,6 +1086,10 @@ __global__ __launch_bounds__(256) void classify_kernel(const uint32_t* __restric
 // ================================================================ host side
 size_t al256(size_t b) { return (b + 255) & ~(size_t)255; }
 
+// Byte offsets of the count kernel's plane loads (in-plane word + X + 1024)
+// stay below 2^32.
+bool plane_ok(int X, int Y) { return (int64_t)X * Y + X + 1024 < ((int64_t)1 << 30); }
+
 int64_t tiles_per_plane(int X, int Y) { return ((int64_t)X * Y + kTile - 1) / kTile; }
 
 int64_t tiles_of(int X, int Y, int Z) {
@@ -879,11 +1103,12 @@ int64_t cubes_of(int X, int Y, int Z) {
 
 size_t fixed_ws_bytes(int64_t n_tiles) {
   const int64_t nscan = n_tiles > 0 ? (n_tiles + kScanItems - 1) / kScanItems : 1;
-  return kWsHeader + al256(16 * (size_t)nscan) + 2 * al256(4 * (size_t)n_tiles) +
+  return kWsHeader + al256(16 * (size_t)nscan) + 3 * al256(4 * (size_t)n_tiles) +
          2 * al256(8 * (size_t)n_tiles) + al256(8 * (size_t)kCPW * (size_t)n_tiles);
 }
 
 int layout_ws(void* d_ws, size_t ws_bytes, int64_t n_tiles, ExtractParams& P) {
+  if (n_tiles >= ((int64_t)1 << 32)) return MCX_ERR_UNSUPPORTED;  // 32-bit tile ids
   const size_t fixed = fixed_ws_bytes(n_tiles);
   if (!d_ws || ws_bytes < fixed) return MCX_ERR_WORKSPACE;
   char* b = static_cast<char*>(d_ws);
@@ -902,9 +1127,24 @@ int layout_ws(void* d_ws, size_t ws_bytes, int64_t n_tiles, ExtractParams& P) {
   o += al256(8 * (size_t)n_tiles);
   P.masks = reinterpret_cast<uint64_t*>(b + o);
   o += al256(8 * (size_t)kCPW * (size_t)n_tiles);
+  P.nz_list = reinterpret_cast<uint32_t*>(b + o);
+  o += al256(4 * (size_t)n_tiles);
   P.rec = reinterpret_cast<uint32_t*>(b + o);
   P.rec_cap = (int64_t)((ws_bytes - o) / (kRecWords * 4));
   return MCX_OK;
+}
+
+// Compute units of the current device (cached per device).
+int cu_count() {
+  static int cache[64] = {};
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) dev = 0;
+  if (dev >= 0 && dev < 64 && cache[dev] > 0) return cache[dev];
+  int n = 0;
+  if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n <= 0)
+    n = 256;
+  if (dev >= 0 && dev < 64) cache[dev] = n;
+  return n;
 }
 
 // Launch K1, K2, (vbase), K3 for prepared parameters; emit = false stops
@@ -912,18 +1152,24 @@ int layout_ws(void* d_ws, size_t ws_bytes, int64_t n_tiles, ExtractParams& P) {
 // recompute of the forward's per-cube records).
 int run_extract(ExtractParams& P, int elem, bool batch, int maxX, hipStream_t s,
                 bool emit = true) {
+  (void)maxX;
+  // record scratch: per-column quotas; the wide count kernel keeps a quarter
+  // of the scratch as a shared overflow pool
+  const bool wide = P.wide && elem == MCX_ELEM_SHORT2 && !P.write_masks;
+  const int64_t pool = wide ? P.rec_cap / 4 : 0;
+  P.quota = P.ncols > 0 ? (P.rec_cap - pool) / P.ncols : 0;
+  if (P.quota > (int64_t)kTile * P.slab_max) P.quota = (int64_t)kTile * P.slab_max;
+  P.ovf_begin = P.quota * P.ncols;
+  P.ovf_cap = wide ? P.rec_cap - P.ovf_begin : 0;
   const unsigned k1_blocks = (unsigned)((P.ncols + 3) / 4);
   const unsigned k2_blocks = (unsigned)((P.n_tiles + kScanItems - 1) / kScanItems);
-  const unsigned k3_blocks =
-      (unsigned)((P.n_tiles + kTilesPerEmitGroup - 1) / kTilesPerEmitGroup);
-  P.emit_plane_words = (int)(((int64_t)kTile + maxX + 1 + 3) & ~3ll);
-  const int rec_words = kEmitThreads * kRecWords;
-  const int dat_words = rec_words > 2 * P.emit_plane_words ? rec_words : 2 * P.emit_plane_words;
-  const int k3_lds = (kEmDat + dat_words) * 4;
-  if (k3_lds > kMaxLds) return MCX_ERR_UNSUPPORTED;
+  const int64_t items = (P.n_tiles + 3) / 4;  // a workgroup is 4 waves, a wave takes tiles
+  const int64_t k3_max = (int64_t)cu_count() * kEmitGroupsPerCU;
+  const unsigned k3_blocks = (unsigned)(items < k3_max ? items : k3_max);
 #define MCX_LAUNCH_SET(E, B)                                                                     \
   do {                                                                                           \
-    hipLaunchKernelGGL((count_kernel<E, B>), dim3(k1_blocks), dim3(256), 0, s, P);              \
+    if (wide) hipLaunchKernelGGL(count4_kernel<B>, dim3(k1_blocks), dim3(256), 0, s, P);        \
+    else hipLaunchKernelGGL((count_kernel<E, B>), dim3(k1_blocks), dim3(256), 0, s, P);         \
     MCX_HIP(hipGetLastError());                                                                  \
     hipLaunchKernelGGL(scan_kernel, dim3(k2_blocks), dim3(256), 0, s, P);                        \
     MCX_HIP(hipGetLastError());                                                                  \
@@ -933,9 +1179,7 @@ int run_extract(ExtractParams& P, int elem, bool batch, int maxX, hipStream_t s,
       MCX_HIP(hipGetLastError());                                                                \
     }                                                                                            \
     if (!emit) break;                                                                            \
-    MCX_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(emit_kernel<E, B>),               \
-                                hipFuncAttributeMaxDynamicSharedMemorySize, k3_lds));            \
-    hipLaunchKernelGGL((emit_kernel<E, B>), dim3(k3_blocks), dim3(kEmitThreads), k3_lds, s, P);  \
+    hipLaunchKernelGGL((emit_kernel<E, B>), dim3(k3_blocks), dim3(kEmitThreads), 0, s, P);      \
     MCX_HIP(hipGetLastError());                                                                  \
   } while (0)
   if (elem == MCX_ELEM_SHORT2) {
@@ -985,8 +1229,8 @@ int prepare_single(ExtractParams& P, const void* d_vol, int X, int Y, int Z, flo
   P.slab = (int)((Z - 1 + S - 1) / S);
   S = (Z - 1 + P.slab - 1) / P.slab;
   P.ncols = R * S;
-  P.quota = P.rec_cap / P.ncols;
-  if (P.quota > (int64_t)kTile * P.slab) P.quota = (int64_t)kTile * P.slab;
+  P.slab_max = P.slab;
+  P.wide = X % 4 == 0 && (reinterpret_cast<uintptr_t>(d_vol) & 15u) == 0;
   return MCX_OK;
 }
 
@@ -1021,7 +1265,7 @@ extern "C" int mcx_extract(const void* d_vol, int elem, int X, int Y, int Z, flo
   if (!d_vol || X < 1 || Y < 1 || Z < 1 || cap < 0) return MCX_ERR_ARG;
   if (elem != MCX_ELEM_SHORT2 && elem != MCX_ELEM_F32) return MCX_ERR_ARG;
   if (elem == MCX_ELEM_F32 && (d_vv || d_ver_voxel)) return MCX_ERR_ARG;
-  if ((int64_t)X * Y >= ((int64_t)1 << 30)) return MCX_ERR_UNSUPPORTED;  // plane < 4 GiB
+  if (!plane_ok(X, Y)) return MCX_ERR_UNSUPPORTED;
   hipStream_t s = as_stream(stream);
   int rc = MCX_OK;
   const int64_t N = (int64_t)X * Y * Z;
@@ -1095,7 +1339,7 @@ extern "C" int mcx_extract_diff(const float* d_vol, int X, int Y, int Z, float h
                                 int64_t cap, int64_t* d_count, int64_t* h_count, void* d_ws,
                                 size_t ws_bytes, void* stream) {
   if (!d_vol || X < 1 || Y < 1 || Z < 1 || cap < 0) return MCX_ERR_ARG;
-  if ((int64_t)X * Y >= ((int64_t)1 << 30)) return MCX_ERR_UNSUPPORTED;
+  if (!plane_ok(X, Y)) return MCX_ERR_UNSUPPORTED;
   hipStream_t s = as_stream(stream);
   const int64_t n_tiles = tiles_of(X, Y, Z);
   if (n_tiles == 0) {
@@ -1132,7 +1376,7 @@ int backward_impl(const float* d_vol, int X, int Y, int Z, float h, const float*
                   float* d_grad_vol, void* d_ws, size_t ws_bytes, void* d_scratch,
                   size_t scratch_bytes, void* stream, bool recompute) {
   if (!d_vol || !d_grad_vol || X < 1 || Y < 1 || Z < 1) return MCX_ERR_ARG;
-  if ((int64_t)X * Y >= ((int64_t)1 << 30)) return MCX_ERR_UNSUPPORTED;
+  if (!plane_ok(X, Y)) return MCX_ERR_UNSUPPORTED;
   hipStream_t s = as_stream(stream);
   const int64_t N = (int64_t)X * Y * Z;
   const int64_t n_tiles = tiles_of(X, Y, Z);
@@ -1152,7 +1396,7 @@ int backward_impl(const float* d_vol, int X, int Y, int Z, float h, const float*
   }
   if (!d_scratch || scratch_bytes < (size_t)P.rec_cap * 8 * sizeof(float)) return MCX_ERR_WORKSPACE;
   float* contrib = static_cast<float*>(d_scratch);
-  const unsigned g1 = (unsigned)((n_tiles + kTilesPerEmitGroup - 1) / kTilesPerEmitGroup);
+  const unsigned g1 = (unsigned)((n_tiles + kTilesPerContribGroup - 1) / kTilesPerContribGroup);
   hipLaunchKernelGGL(contrib_kernel<false>, dim3(g1), dim3(kEmitThreads), 0, s, P, d_grad_pos,
                      contrib);
   MCX_HIP(hipGetLastError());
@@ -1189,6 +1433,7 @@ struct mcx_batch {
   int nvol = 0;
   int elem = MCX_ELEM_SHORT2;
   int maxX = 0;
+  bool wide = true;  // every volume suits the 128-bit count kernel
   int64_t n_tiles = 0;
   int64_t ncols = 0;
   int64_t cubes = 0;
@@ -1204,10 +1449,11 @@ extern "C" int mcx_batch_create(mcx_batch** out, int nvol, const int* dims_xyz, 
   std::vector<Column> cols;
   int64_t tb = 0, cubes = 0;
   int maxX = 1;
+  bool wide = true;
   for (int b = 0; b < nvol; ++b) {
     const int X = dims_xyz[3 * b], Y = dims_xyz[3 * b + 1], Z = dims_xyz[3 * b + 2];
     if (X < 1 || Y < 1 || Z < 1 || !d_vols[b]) return MCX_ERR_ARG;
-    if ((int64_t)X * Y >= ((int64_t)1 << 30)) return MCX_ERR_UNSUPPORTED;
+    if (!plane_ok(X, Y)) return MCX_ERR_UNSUPPORTED;
     vols[b] = make_vol(d_vols[b], X, Y, Z, h[b], nullptr, tb);
     const int64_t nt = tiles_of(X, Y, Z);
     if (nt > 0) {
@@ -1220,8 +1466,10 @@ extern "C" int mcx_batch_create(mcx_batch** out, int nvol, const int* dims_xyz, 
     tb += nt;
     cubes += cubes_of(X, Y, Z);
     maxX = std::max(maxX, X);
+    wide = wide && X % 4 == 0 && (reinterpret_cast<uintptr_t>(d_vols[b]) & 15u) == 0;
   }
   mcx_batch* p = new mcx_batch();
+  p->wide = wide;
   p->nvol = nvol;
   p->elem = elem;
   p->maxX = maxX;
@@ -1275,8 +1523,8 @@ extern "C" int mcx_extract_batched(const mcx_batch* plan, mcx_vertex_voxel* d_vv
   P.nvol = plan->nvol;
   P.n_tiles = plan->n_tiles;
   P.ncols = plan->ncols;
-  P.quota = P.ncols > 0 ? P.rec_cap / P.ncols : 0;
-  if (P.quota > (int64_t)kTile * 64) P.quota = (int64_t)kTile * 64;
+  P.slab_max = 64;
+  P.wide = plan->wide;
   P.vv = d_vv;
   P.pos = d_pos;
   P.v2vox = d_v2vox;

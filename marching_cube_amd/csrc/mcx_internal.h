@@ -40,7 +40,11 @@ struct WsHeader {
   uint32_t parity;
   uint32_t error;
   int64_t count;
+  uint32_t nz_count;    // non-empty tiles appended by the running count kernel
+  uint32_t nz_total;    // the same, frozen by the scan for the emit kernel
+  uint64_t ovf_next;    // records taken from the shared overflow pool
 };
+static_assert(sizeof(WsHeader) <= 256, "header fits its slot");
 
 // Look-back descriptor: status in the top two bits, value below.
 constexpr uint64_t kStatAgg = 1ull << 62;
