@@ -41,6 +41,9 @@ def parse():
                     help="side of the fp32 bumps field of the forward/backward leg (0 = skip)")
     ap.add_argument("--batch-count", type=int, default=3000,
                     help="volumes of the batched leg (0 = skip)")
+    ap.add_argument("--calib-copy", type=int, default=0,
+                    help="after the legs, copy a 512 MiB buffer this many times (a plain copy "
+                         "kernel of known size: the scale for rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "extract_kinfu512_traffic.json"),
                     help="JSON with measured HBM bytes per launch (from rocprofv3 --pmc)")
     return ap.parse_args()
@@ -298,6 +301,14 @@ def main():
                "kind": "port",
                "sample": f"full {dim}^3 volume (same input), vertex_voxel output, median of 3 "
                          f"after 1 warm-up, {tmed:.3f} s per pass, {nt} OpenMP threads"}
+
+    if args.calib_copy:
+        src = torch.empty(128 << 20, dtype=torch.int32, device=dev)
+        dst = torch.empty_like(src)
+        for _ in range(args.calib_copy):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
 
     out = {
         "metric": "Gvoxel/s MC extract at 512³ (% HBM peak); fwd+bwd vs fwd-only time",

@@ -57,6 +57,7 @@ constexpr int kRecWords = 9;           // record: (offset << 8 | case), 8 corner
 constexpr int kScanItems = 4096;       // tiles per scan block (256 threads x 16)
 constexpr int kEmitThreads = 256;
 constexpr int kEmitGroupsPerCU = 8;       // K3 persistent workgroups per CU
+constexpr int kColumnsPerCU = 64;         // K1 columns (waves) per CU, single volume
 constexpr int kTilesPerContribGroup = 8;  // backward BW1: tiles per workgroup
 // third word of a gfx9 raw buffer resource (stride 0, 32-bit data format)
 constexpr int kBufferWord3 = 0x00020000;
@@ -163,12 +164,14 @@ __device__ __forceinline__ uint32_t cube_index(const uint32_t c[8]) {
 }
 
 // c[i] for a runtime i without dynamic register indexing (which would put
-// the array in scratch memory): a chain of selects.
+// the array in scratch memory): a tree of selects on the bits of i (a chain
+// of i == k selects is turned back into an indexed load by the compiler).
 __device__ __forceinline__ uint32_t pick8(const uint32_t c[8], int i) {
-  uint32_t r = c[0];
-#pragma unroll
-  for (int k = 1; k < 8; ++k) r = i == k ? c[k] : r;
-  return r;
+  const bool b0 = (i & 1) != 0, b1 = (i & 2) != 0, b2 = (i & 4) != 0;
+  const uint32_t r01 = b0 ? c[1] : c[0], r23 = b0 ? c[3] : c[2];
+  const uint32_t r45 = b0 ? c[5] : c[4], r67 = b0 ? c[7] : c[6];
+  const uint32_t r03 = b1 ? r23 : r01, r47 = b1 ? r67 : r45;
+  return b2 ? r47 : r03;
 }
 
 template <bool BATCH>
@@ -408,27 +411,42 @@ __device__ __forceinline__ uint32_t min_u16x2(uint32_t a, uint32_t b) {
                                                                 __builtin_bit_cast(u16x2, b)));
 }
 
-// Workgroup id remapped so that the workgroups one XCD runs (ids b, b + 8,
-// ...: dispatch is round-robin over the 8 XCDs) are consecutive: neighbouring
-// columns then share a row through that XCD's L2.
+// Workgroup id remapped so that runs of kXcdGroup consecutive workgroups
+// (consecutive columns) execute on one XCD (dispatch is round-robin over the
+// 8 XCDs: ids b, b + 8, ... run on XCD b % 8): neighbouring columns share a
+// row through that XCD's L2, and the runs rotate over the XCDs so that every
+// XCD gets work from every slab.  Ids of an incomplete last round map to
+// themselves.
+#ifndef MCX_XCD_GROUP
+#define MCX_XCD_GROUP 8
+#endif
+constexpr uint32_t kXcdGroup = MCX_XCD_GROUP;
 __device__ __forceinline__ uint32_t xcd_block() {
   const uint32_t nb = gridDim.x, b = blockIdx.x;
-  const uint32_t q = nb >> 3, rem = nb & 7u, x = b & 7u;
-  return x * q + (x < rem ? x : rem) + (b >> 3);
+  const uint32_t x = b & 7u, i = b >> 3;
+  const uint32_t g = i / kXcdGroup, j = i - g * kXcdGroup;
+  if (g >= nb / (8u * kXcdGroup)) return b;
+  return (g * 8u + x) * kXcdGroup + j;
 }
+
+// DPP control: wave_shl:1 (lane i reads lane i + 1; lane 63 keeps `old`)
+constexpr int kDppWaveShl1 = 0x130;
 
 struct Plane4 {
   u32x4 u[2], w[2];  // rows y (u) and y + 1 (w) of chunks 0 and 1
-  uint32_t xu, xw;   // word 512 of the two rows (lane 0; neighbour of chunk 1's lane 63)
+  uint32_t xu, xw;   // word 512 of the two rows (wave-uniform; neighbour of chunk 1's lane 63)
 };
+typedef const __attribute__((address_space(4))) uint32_t const_u32;  // scalar loads
 
 template <bool BATCH>
 __global__ __launch_bounds__(256) void count4_kernel(ExtractParams P) {
   __shared__ uint8_t s_nv[256];
+  __shared__ uint32_t s_rec[4][256 * kRecWords];  // per wave: one chunk's records
   s_nv[threadIdx.x] = dNumVerts[threadIdx.x];
   __syncthreads();
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  uint32_t* const my_rec = s_rec[wave];
   const int64_t col = (int64_t)xcd_block() * 4 + wave;
   if (col >= P.ncols) return;  // whole waves exit; no barrier follows
 
@@ -454,7 +472,9 @@ __global__ __launch_bounds__(256) void count4_kernel(ExtractParams P) {
 
   // lanes whose 4 voxels lie on a row without cubes (the last row, or past the
   // plane), and lanes at the end of a row (their position 4 starts the next row)
-  uint64_t norow[2], rowend[2];
+  // as word masks: lane_ok = 0 on lanes without cubes, lane_mid = 0 on lanes
+  // at a row end (where position 4 is not a neighbour)
+  uint32_t lane_ok[2], lane_mid[2];
 #pragma unroll
   for (int c = 0; c < 2; ++c) {
     const int64_t p = p0 + 256 * c + 4 * lane;
@@ -465,8 +485,8 @@ __global__ __launch_bounds__(256) void count4_kernel(ExtractParams P) {
       ok = y + 1 < (uint32_t)V.Y;
       end = x + 4 == (uint32_t)X;
     }
-    norow[c] = __ballot(!ok);
-    rowend[c] = __ballot(end);
+    lane_ok[c] = ok ? 0xffffffffu : 0u;
+    lane_mid[c] = end ? 0u : 0xffffffffu;
   }
 
   // Offsets in the VGPR offset (range-checked against the plane size; a
@@ -474,8 +494,8 @@ __global__ __launch_bounds__(256) void count4_kernel(ExtractParams P) {
   const uint32_t plane_bytes = (uint32_t)(XY * 4);
   const uint32_t voff = (uint32_t)(p0 + 4 * lane) * 4u;
   const uint32_t voffW = voff + 4u * (uint32_t)X;
-  const uint32_t voff_x = lane == 0 ? (uint32_t)(p0 + kTile) * 4u : plane_bytes;
-  const uint32_t voffW_x = lane == 0 ? voff_x + 4u * (uint32_t)X : plane_bytes;
+  // the word after the tile in each row (wave-uniform scalar loads; 0 past the plane)
+  const bool has_xu = p0 + kTile < XY, has_xw = p0 + kTile + X < XY;
   auto load_plane = [&](int q, bool need, Plane4& L) {
     const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
         const_cast<uint32_t*>(V.vol + (need ? (int64_t)q * XY : 0)), (short)0,
@@ -484,53 +504,65 @@ __global__ __launch_bounds__(256) void count4_kernel(ExtractParams P) {
     L.u[1] = __builtin_amdgcn_raw_buffer_load_b128(rs, voff + 1024u, 0, 0);
     L.w[0] = __builtin_amdgcn_raw_buffer_load_b128(rs, voffW, 0, 0);
     L.w[1] = __builtin_amdgcn_raw_buffer_load_b128(rs, voffW + 1024u, 0, 0);
-    L.xu = __builtin_amdgcn_raw_buffer_load_b32(rs, voff_x, 0, 0);
-    L.xw = __builtin_amdgcn_raw_buffer_load_b32(rs, voffW_x, 0, 0);
+    const_u32* row = (const_u32*)(V.vol) + (int64_t)q * XY + p0 + kTile;
+    L.xu = need && has_xu ? row[0] : 0u;
+    L.xw = need && has_xw ? row[X] : 0u;
   };
-  Plane4 A, B, C;
+  // Four plane buffers whose roles rotate over an unrolled loop of four steps
+  // (no register copies, which would wait for in-flight loads): the step for
+  // cubes between planes z and z + 1 loads plane z + 3.
+  Plane4 A, B, C, D;
   load_plane(z0, true, A);
   load_plane(z0 + 1, true, B);
+  load_plane(z0 + 2, z0 + 1 < z1, C);
 
   int64_t used = 0;
   uint64_t nzmask = 0;  // planes zb + i of this column whose tile has vertices
   int zb = z0;
-  for (int z = z0; z < z1; ++z) {
-    load_plane(z + 2, z + 1 < z1, C);
+  auto step = [&](Plane4& A, Plane4& B, Plane4& L, int z) __attribute__((always_inline)) {
+    load_plane(z + 3, z + 2 < z1, L);
     const int64_t t = V.tile_begin + (int64_t)z * V.R + r;
 
-    // ---- pass 1: active masks m[c][k] (bit L: cube 256c + 4L + k)
+    // ---- pass 1: active masks m[c][k] (bit L: cube 256c + 4L + k).  Per lane
+    // and voxel position, over the four corner rows: OR and AND of the words
+    // (bit 15: some / every tsdf negative) and the minimum word (below 0x10000:
+    // some weight zero; forced to 0 on lanes without cubes).  A cube combines
+    // positions k and k + 1; position 4 is the next lane's position 0 (a DPP
+    // lane shift; lane 63 takes the next chunk's lane 0, or the words after
+    // the tile).
     uint64_t m[2][4];
-    uint64_t nxn, nxp, nxw;  // bit 0: position 0 of the next chunk's lane 0
+    uint32_t nxo, nxn, nxm;  // position 0 of the next chunk's lane 0 (wave-uniform)
     {
-      const uint32_t mn = min_s16x2(min_s16x2(A.xu, A.xw), min_s16x2(B.xu, B.xw));
-      const uint32_t mx = max_s16x2(max_s16x2(A.xu, A.xw), max_s16x2(B.xu, B.xw));
-      const uint32_t mw = min_u16x2(min_u16x2(A.xu, A.xw), min_u16x2(B.xu, B.xw));
-      nxn = __ballot((int16_t)mn < 0);
-      nxp = __ballot((int16_t)mx >= 0);
-      nxw = __ballot(mw < 0x10000u);
+      nxo = A.xu | A.xw | B.xu | B.xw;
+      nxn = A.xu & A.xw & B.xu & B.xw;
+      nxm = min(min(A.xu, A.xw), min(B.xu, B.xw));
     }
 #pragma unroll
     for (int c = 1; c >= 0; --c) {
-      uint64_t bn[5], bp[5], bw[5];
+      uint32_t o[5], n[5], mn[5];
 #pragma unroll
       for (int pos = 0; pos < 4; ++pos) {
         const uint32_t a = A.u[c][pos], b = A.w[c][pos], cc = B.u[c][pos], d = B.w[c][pos];
-        const uint32_t mn = min_s16x2(min_s16x2(a, b), min_s16x2(cc, d));
-        const uint32_t mx = max_s16x2(max_s16x2(a, b), max_s16x2(cc, d));
-        const uint32_t mw = min_u16x2(min_u16x2(a, b), min_u16x2(cc, d));
-        bn[pos] = __ballot((int16_t)mn < 0);
-        bp[pos] = __ballot((int16_t)mx >= 0);
-        bw[pos] = __ballot(mw < 0x10000u) | norow[c];
+        o[pos] = a | b | cc | d;
+        n[pos] = a & b & cc & d;
+        mn[pos] = min(min(min(min(a, b), cc), d), lane_ok[c]);  // two v_min3
       }
-      bn[4] = (bn[0] >> 1) | (nxn << 63);
-      bp[4] = (bp[0] >> 1) | (nxp << 63);
-      bw[4] = (bw[0] >> 1) | (nxw << 63) | rowend[c];
+      o[4] = __builtin_amdgcn_update_dpp(nxo, o[0], kDppWaveShl1, 0xf, 0xf, false);
+      n[4] = __builtin_amdgcn_update_dpp(nxn, n[0], kDppWaveShl1, 0xf, 0xf, false);
+      mn[4] = min(__builtin_amdgcn_update_dpp(nxm, mn[0], kDppWaveShl1, 0xf, 0xf, false),
+                  lane_mid[c]);
 #pragma unroll
-      for (int k = 0; k < 4; ++k)
-        m[c][k] = (bn[k] | bn[k + 1]) & (bp[k] | bp[k + 1]) & ~(bw[k] | bw[k + 1]);
-      nxn = bn[0];
-      nxp = bp[0];
-      nxw = bw[0];
+      for (int k = 0; k < 4; ++k) {
+        const uint32_t mixed = (o[k] | o[k + 1]) & ~(n[k] & n[k + 1]);
+        // all ones when bit 15 of mixed is set (both signs among the corners)
+        const uint32_t both = (uint32_t)((int32_t)(mixed << 16) >> 31);
+        m[c][k] = __ballot((min(mn[k], mn[k + 1]) & both) >= 0x10000u);
+      }
+      if (c == 1) {
+        nxo = __builtin_amdgcn_readlane(o[0], 0);
+        nxn = __builtin_amdgcn_readlane(n[0], 0);
+        nxm = __builtin_amdgcn_readlane(mn[0], 0);
+      }
     }
     int64_t nrec = 0;
 #pragma unroll
@@ -552,7 +584,10 @@ __global__ __launch_bounds__(256) void count4_kernel(ExtractParams P) {
       else dense = true;
     }
 
-    // ---- pass 2: cases, vertex counts and records of the active cubes
+    // ---- pass 2: the active cubes of a chunk are staged in the wave's LDS
+    // area in cube order (offset and 8 corner words), then taken 64 at a
+    // time, one per lane, for their case and vertex count, and written to the
+    // scratch with consecutive lanes on consecutive words.
     uint32_t vsum = 0;
     int64_t cbase = 0;
 #pragma unroll
@@ -565,46 +600,54 @@ __global__ __launch_bounds__(256) void count4_kernel(ExtractParams P) {
         pre += __builtin_amdgcn_mbcnt_hi((uint32_t)(m[c][k] >> 32),
                                          __builtin_amdgcn_mbcnt_lo((uint32_t)m[c][k], 0u));
       // position 4 of the lane: the next lane's position 0; lane 63 takes
-      // lane 0 of chunk 1 (c = 0) or the extra words (c = 1)
+      // lane 0 of chunk 1 (c = 0) or the words after the tile (c = 1)
       uint32_t nau = 0, naw = 0, nbu = 0, nbw = 0;
       if (m[c][3]) {
-        nau = __shfl_down(A.u[c][0], 1);
-        naw = __shfl_down(A.w[c][0], 1);
-        nbu = __shfl_down(B.u[c][0], 1);
-        nbw = __shfl_down(B.w[c][0], 1);
-        const uint32_t lau = __builtin_amdgcn_readlane(c == 0 ? A.u[1][0] : A.xu, 0);
-        const uint32_t law = __builtin_amdgcn_readlane(c == 0 ? A.w[1][0] : A.xw, 0);
-        const uint32_t lbu = __builtin_amdgcn_readlane(c == 0 ? B.u[1][0] : B.xu, 0);
-        const uint32_t lbw = __builtin_amdgcn_readlane(c == 0 ? B.w[1][0] : B.xw, 0);
-        if (lane == 63) {
-          nau = lau;
-          naw = law;
-          nbu = lbu;
-          nbw = lbw;
-        }
+        const uint32_t lau = c == 0 ? __builtin_amdgcn_readlane(A.u[1][0], 0) : A.xu;
+        const uint32_t law = c == 0 ? __builtin_amdgcn_readlane(A.w[1][0], 0) : A.xw;
+        const uint32_t lbu = c == 0 ? __builtin_amdgcn_readlane(B.u[1][0], 0) : B.xu;
+        const uint32_t lbw = c == 0 ? __builtin_amdgcn_readlane(B.w[1][0], 0) : B.xw;
+        nau = __builtin_amdgcn_update_dpp(lau, A.u[c][0], kDppWaveShl1, 0xf, 0xf, false);
+        naw = __builtin_amdgcn_update_dpp(law, A.w[c][0], kDppWaveShl1, 0xf, 0xf, false);
+        nbu = __builtin_amdgcn_update_dpp(lbu, B.u[c][0], kDppWaveShl1, 0xf, 0xf, false);
+        nbw = __builtin_amdgcn_update_dpp(lbw, B.w[c][0], kDppWaveShl1, 0xf, 0xf, false);
       }
       uint32_t own = 0;
 #pragma unroll
       for (int k = 0; k < 4; ++k) {
         if ((m[c][k] >> lane) & 1ull) {
-          const uint32_t a0 = A.u[c][k], a3 = A.w[c][k], b0 = B.u[c][k], b3 = B.w[c][k];
-          const uint32_t a1 = k < 3 ? A.u[c][(k + 1) & 3] : nau;
-          const uint32_t a2 = k < 3 ? A.w[c][(k + 1) & 3] : naw;
-          const uint32_t b1 = k < 3 ? B.u[c][(k + 1) & 3] : nbu;
-          const uint32_t b2 = k < 3 ? B.w[c][(k + 1) & 3] : nbw;
-          const uint32_t cw[8] = {a0, a1, a2, a3, b0, b1, b2, b3};
-          const uint32_t ci = cube_index<MCX_ELEM_SHORT2>(cw);
-          vsum += s_nv[ci];
-          if (!dense) {
-            uint32_t* rp = P.rec + (rbase + cbase + pre + own) * kRecWords;
-            rp[0] = ((uint32_t)(256 * c + 4 * lane + k) << 8) | ci;
-#pragma unroll
-            for (int q = 0; q < 8; ++q) rp[1 + q] = cw[q];
-          }
+          uint32_t* rp = my_rec + (pre + own) * kRecWords;
+          rp[0] = (uint32_t)(256 * c + 4 * lane + k) << 8;
+          rp[1] = A.u[c][k];
+          rp[2] = k < 3 ? A.u[c][(k + 1) & 3] : nau;
+          rp[3] = k < 3 ? A.w[c][(k + 1) & 3] : naw;
+          rp[4] = A.w[c][k];
+          rp[5] = B.u[c][k];
+          rp[6] = k < 3 ? B.u[c][(k + 1) & 3] : nbu;
+          rp[7] = k < 3 ? B.w[c][(k + 1) & 3] : nbw;
+          rp[8] = B.w[c][k];
           ++own;
         }
       }
-      cbase += __popcll(m[c][0]) + __popcll(m[c][1]) + __popcll(m[c][2]) + __popcll(m[c][3]);
+      const int nc = __popcll(m[c][0]) + __popcll(m[c][1]) + __popcll(m[c][2]) + __popcll(m[c][3]);
+      wave_sync();
+      for (int i = lane; i < nc; i += 64) {
+        uint32_t* rp = my_rec + i * kRecWords;
+        // active cubes have no zero weight: the case is the corner signs
+        uint32_t ci = 0;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) ci |= ((rp[1 + q] >> 15) & 1u) << q;
+        rp[0] |= ci;
+        vsum += s_nv[ci];
+      }
+      wave_sync();
+      if (!dense) {
+        // the chunk's records are consecutive in scratch
+        uint32_t* dst = P.rec + (rbase + cbase) * kRecWords;
+        for (int w = lane; w < nc * kRecWords; w += 64) dst[w] = my_rec[w];
+        wave_sync();
+      }
+      cbase += nc;
     }
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) vsum += __shfl_xor(vsum, o);
@@ -631,8 +674,12 @@ __global__ __launch_bounds__(256) void count4_kernel(ExtractParams P) {
       nzmask = 0;
       zb = z + 1;
     }
-    A = B;
-    B = C;
+  };
+  for (int z = z0; z < z1; z += 4) {
+    step(A, B, D, z);
+    if (z + 1 < z1) step(B, C, A, z + 1);
+    if (z + 2 < z1) step(C, D, B, z + 2);
+    if (z + 3 < z1) step(D, A, C, z + 3);
   }
 }
 
@@ -746,12 +793,13 @@ struct TileInfo {      // backward BW1 per-tile info in LDS (10 words)
 };
 static_assert(sizeof(TileInfo) == 40, "tile info is 10 words");
 
-// Write one output vertex directly (wave-level emission).
+// One output vertex: slot q of cube (x, y, z) with case ci and corner words
+// cw[8] in the reference corner order.  The 7 words of its vertex_voxel
+// record go to the wave's staging area o[0..6]; ver_voxel is written here.
 template <int ELEM>
-__device__ __forceinline__ void write_vertex(const ExtractParams& P, const VolDesc& V,
-                                             const uint64_t* s_case, const uint32_t cw[8], int x,
-                                             int y, int z, uint32_t ci, int q, int64_t idx) {
-  if (idx >= P.cap) return;
+__device__ __forceinline__ void make_vertex(const VolDesc& V, const uint64_t* s_case,
+                                            const uint32_t cw[8], int x, int y, int z,
+                                            uint32_t ci, int q, bool write_vv, uint32_t* o) {
   // output slot q = 3*tri + r holds table corner 2 - r of triangle tri (:324-332)
   const int m = 3 * (q / 3) + 2 - (q % 3);
   const int e = (int)((s_case[ci] >> (4 * m)) & 0xfu);
@@ -786,40 +834,27 @@ __device__ __forceinline__ void write_vertex(const ExtractParams& P, const VolDe
       tw = (dx | dy | dz) <= 1u ? pick8(cw, corner_of((int)dx, (int)dy, (int)dz)) : V.vol[li];
     }
   }
-  if (P.vv) {
-    uint32_t* rr = reinterpret_cast<uint32_t*>(P.vv) + 7 * idx;
-    rr[0] = __float_as_uint(px);
-    rr[1] = __float_as_uint(py);
-    rr[2] = __float_as_uint(pz);
-    rr[3] = (uint32_t)vx;
-    rr[4] = (uint32_t)vy;
-    rr[5] = (uint32_t)vz;
-    rr[6] = tw;
-  }
-  if (P.pos) {
-    float* o = P.pos + 3 * idx;
-    o[0] = px;
-    o[1] = py;
-    o[2] = pz;
-  }
-  if (P.v2vox) {
-    int32_t* o = P.v2vox + 3 * idx;
-    o[0] = vx;
-    o[1] = vy;
-    o[2] = vz;
-  }
-  if (V.ver_voxel && inb) V.ver_voxel[li] = (int16_t)(tw & 0xffffu);
+  o[0] = __float_as_uint(px);
+  o[1] = __float_as_uint(py);
+  o[2] = __float_as_uint(pz);
+  o[3] = (uint32_t)vx;
+  o[4] = (uint32_t)vy;
+  o[5] = (uint32_t)vz;
+  o[6] = tw;
+  if (write_vv && V.ver_voxel && inb) V.ver_voxel[li] = (int16_t)(tw & 0xffffu);
 }
 
 // Emit the vertices of up to 64 cubes held one per lane (record word 0 =
 // offset in tile << 8 | case, corner words cw; nv = 0 for lanes without a
-// cube), in lane order, to output indices out, out + 1, ...  Returns the
-// number of vertices (wave-uniform).
+// cube), in lane order, to output indices out, out + 1, ...  Vertices are
+// staged 64 at a time in the wave's area s_out (64 x 7 words) and written
+// with consecutive lanes on consecutive words.  Returns the number of
+// vertices (wave-uniform).
 template <int ELEM>
 __device__ __forceinline__ int emit_lanes(const ExtractParams& P, const VolDesc& V,
                                           const uint64_t* s_case, uint32_t w0,
                                           const uint32_t cw[8], int nv, int z, int64_t p0,
-                                          int64_t out) {
+                                          int64_t out, uint32_t* s_out) {
   const int lane = threadIdx.x & 63;
   const uint32_t incl = wave_incl_scan32((uint32_t)nv);
   const int ex = (int)incl - nv;
@@ -839,32 +874,56 @@ __device__ __forceinline__ int emit_lanes(const ExtractParams& P, const VolDesc&
     uint32_t c[8];
 #pragma unroll
     for (int k = 0; k < 8; ++k) c[k] = __shfl(cw[k], lo);
-    if (j < tot) {
+    const int64_t vbase = out + vb;
+    const int n = min(64, tot - vb);
+    const int64_t room = P.cap - vbase;
+    const int nw = room <= 0 ? 0 : (room < n ? (int)room : n);  // vertices below the capacity
+    if (lane < nw) {
       const int64_t p = p0 + (rw >> 8);
       const uint32_t y = V.divX.div((uint32_t)p);
       const int x = (int)((uint32_t)p - y * (uint32_t)V.X);
-      write_vertex<ELEM>(P, V, s_case, c, x, (int)y, z, rw & 0xffu, q, out + j);
+      make_vertex<ELEM>(V, s_case, c, x, (int)y, z, rw & 0xffu, q, true, s_out + 7 * lane);
     }
+    wave_sync();
+    if (P.vv) {
+      uint32_t* dst = reinterpret_cast<uint32_t*>(P.vv) + 7 * vbase;
+      for (int w = lane; w < 7 * nw; w += 64) dst[w] = s_out[w];
+    }
+    if (P.pos || P.v2vox) {
+      for (int w = lane; w < 3 * nw; w += 64) {
+        const int v = w / 3, f = w - 3 * v;
+        if (P.pos) P.pos[3 * vbase + w] = __uint_as_float(s_out[7 * v + f]);
+        if (P.v2vox) P.v2vox[3 * vbase + w] = (int32_t)s_out[7 * v + 3 + f];
+      }
+    }
+    wave_sync();
   }
   return tot;
 }
 
 // Wave-level emission: wave w of the grid takes list entries w, w + nwaves,
-// ... (no atomics: one counter shared by every wave serialises).  A tile's vertices are
-// [voff[t], voff[t] + T), in cube order, so the output does not depend on
-// which wave takes which tile.  Records are taken 64 at a time, one per lane;
-// a tile whose records did not fit the scratch is re-read from the volume, 64
-// cubes at a time, into the same per-lane form.
+// ... (no atomics: one counter shared by every wave serialises).  A tile's
+// vertices are [voff[t], voff[t] + T), in cube order, so the output does not
+// depend on which wave takes which tile.  Records are read 64 at a time with
+// consecutive lanes on consecutive words into the wave's LDS area, then taken
+// one per lane; a tile whose records did not fit the scratch is re-read from
+// the volume, 64 cubes at a time, into the same per-lane form.
 template <int ELEM, bool BATCH>
 __global__ __launch_bounds__(kEmitThreads) void emit_kernel(ExtractParams P) {
   __shared__ uint64_t s_case[256];
+  __shared__ uint8_t s_nv[256];
+  __shared__ uint32_t s_wrec[kEmitThreads / 64][64 * kRecWords];
+  __shared__ uint32_t s_wout[kEmitThreads / 64][64 * 7];
   s_case[threadIdx.x] = dCase[threadIdx.x];
+  s_nv[threadIdx.x] = dNumVerts[threadIdx.x];
   __syncthreads();  // the only workgroup barrier: waves run independently below
   const int lane = threadIdx.x & 63;
+  const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  uint32_t* const my_rec = s_wrec[wv];
+  uint32_t* const my_out = s_wout[wv];
   const uint32_t nz = P.hdr->nz_total;
   const uint32_t nwaves = gridDim.x * (kEmitThreads / 64);
-  const uint32_t wave0 = blockIdx.x * (kEmitThreads / 64) +
-                         (uint32_t)__builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  const uint32_t wave0 = blockIdx.x * (kEmitThreads / 64) + (uint32_t)wv;
   for (uint32_t item = wave0; item < nz; item += nwaves) {
     const int64_t t = P.nz_list[item];
     const int nrec = P.tile_nrec[t];
@@ -877,17 +936,20 @@ __global__ __launch_bounds__(kEmitThreads) void emit_kernel(ExtractParams P) {
     if (nrec >= 0) {
       const uint32_t* rec = P.rec + P.tile_rec[t] * kRecWords;
       for (int rb = 0; rb < nrec; rb += 64) {
-        const int k = rb + lane;
+        const int n = min(64, nrec - rb);
+        const uint32_t* src = rec + (int64_t)rb * kRecWords;
+        for (int w = lane; w < n * kRecWords; w += 64) my_rec[w] = src[w];
+        wave_sync();
         uint32_t w0 = 0, cw[8] = {};
         int nv = 0;
-        if (k < nrec) {
-          const uint32_t* src = rec + (int64_t)k * kRecWords;
-          w0 = src[0];
+        if (lane < n) {
+          w0 = my_rec[lane * kRecWords];
 #pragma unroll
-          for (int q = 0; q < 8; ++q) cw[q] = src[1 + q];
-          nv = dNumVerts[w0 & 0xffu];
+          for (int q = 0; q < 8; ++q) cw[q] = my_rec[lane * kRecWords + 1 + q];
+          nv = s_nv[w0 & 0xffu];
         }
-        out += emit_lanes<ELEM>(P, V, s_case, w0, cw, nv, z, p0, out);
+        wave_sync();
+        out += emit_lanes<ELEM>(P, V, s_case, w0, cw, nv, z, p0, out, my_out);
       }
     } else {
       const int X = V.X;
@@ -908,10 +970,10 @@ __global__ __launch_bounds__(kEmitThreads) void emit_kernel(ExtractParams P) {
             cw[4] = b[0]; cw[5] = b[1]; cw[6] = b[X + 1]; cw[7] = b[X];
             const uint32_t ci = cube_index<ELEM>(cw);
             w0 = ((uint32_t)o << 8) | ci;
-            nv = dNumVerts[ci];
+            nv = s_nv[ci];
           }
         }
-        out += emit_lanes<ELEM>(P, V, s_case, w0, cw, nv, z, p0, out);
+        out += emit_lanes<ELEM>(P, V, s_case, w0, cw, nv, z, p0, out, my_out);
       }
     }
   }
@@ -1210,7 +1272,9 @@ VolDesc make_vol(const void* d_vol, int X, int Y, int Z, float h, int16_t* ver_v
 }
 
 // Single-volume launch parameters: workspace layout, the volume, and the
-// column decomposition (R in-plane ranges x S slabs, about 4096 waves).
+// column decomposition (R in-plane ranges x S slabs, about kColumnsPerCU
+// waves per CU: several rounds of waves per SIMD, so that columns with much
+// surface do not leave SIMDs idle at the end).
 // Depends only on (X, Y, Z, ws_bytes), so a backward call sees the forward's
 // record slots.
 int prepare_single(ExtractParams& P, const void* d_vol, int X, int Y, int Z, float h,
@@ -1224,7 +1288,8 @@ int prepare_single(ExtractParams& P, const void* d_vol, int X, int Y, int Z, flo
   P.nvol = 1;
   P.n_tiles = n_tiles;
   const int64_t R = P.vol0.R;
-  int64_t S = (4096 + R - 1) / R;
+  const int64_t target = (int64_t)cu_count() * kColumnsPerCU;
+  int64_t S = (target + R - 1) / R;
   if (S > Z - 1) S = Z - 1;
   P.slab = (int)((Z - 1 + S - 1) / S);
   S = (Z - 1 + P.slab - 1) / P.slab;

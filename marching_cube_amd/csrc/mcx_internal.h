@@ -116,6 +116,14 @@ __device__ __forceinline__ uint64_t wave_sum(uint64_t v) {
   return v;
 }
 
+// Orders a wave's own LDS writes before its later LDS reads of other lanes'
+// data (wave-private staging areas; no workgroup barrier needed).
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 __device__ __forceinline__ uint64_t ld_agent(const uint64_t* p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }

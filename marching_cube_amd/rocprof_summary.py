@@ -1,0 +1,120 @@
+"""Summaries of rocprofv3 output for the profiles/ folder.
+
+    python -m marching_cube_amd.rocprof_summary stats  <kernel_stats.csv> [--match extract]
+    python -m marching_cube_amd.rocprof_summary traffic <fetch_dir> <write_dir> --out X.json
+
+`traffic` reads two `rocprofv3 --pmc FETCH_SIZE` / `--pmc WRITE_SIZE` runs of
+the same command (``pm_counter_collection.csv``), scales each counter by what
+it reads for the plain copy kernel of the run (torch's copy of a 512 MiB
+buffer, ``bench.py --calib-copy``), and sums the extraction kernels (count,
+scan, emit) of one call: HBM bytes per extraction.
+"""
+import argparse
+import collections
+import csv
+import json
+import os
+import sys
+
+EXTRACT_KERNELS = ("count4_kernel", "count_kernel", "scan_kernel", "emit_kernel")
+COPY_BYTES = 512 << 20
+
+
+def kernel_stats(path, match=None):
+    rows = []
+    with open(path) as f:
+        for r in csv.DictReader(f):
+            name = r["Name"]
+            if match and not any(m in name for m in match):
+                continue
+            rows.append((name, int(r["Calls"]), float(r["AverageNs"]) / 1e3,
+                         float(r["TotalDurationNs"]) / 1e3))
+    return rows
+
+
+def _per_kernel(path, counter):
+    d = collections.defaultdict(list)
+    with open(path) as f:
+        for r in csv.DictReader(f):
+            if r["Counter_Name"] == counter:
+                d[r["Kernel_Name"]].append(float(r["Counter_Value"]))
+    return d
+
+
+def _short(name):
+    """Kernel name without return type, namespaces and parameter list."""
+    n = name[5:] if name.startswith("void ") else name
+    n = n.replace("mcx::(anonymous namespace)::", "").replace("at::native::", "")
+    depth, cut = 0, len(n)
+    for i, ch in enumerate(n):  # first '(' outside template brackets
+        if ch == "<":
+            depth += 1
+        elif ch == ">":
+            depth -= 1
+        elif ch == "(" and depth == 0:
+            cut = i
+            break
+    return n[:cut][:70]
+
+
+def traffic(fetch_dir, write_dir):
+    fetch = _per_kernel(os.path.join(fetch_dir, "pm_counter_collection.csv"), "FETCH_SIZE")
+    write = _per_kernel(os.path.join(write_dir, "pm_counter_collection.csv"), "WRITE_SIZE")
+    # the copy kernel: the dispatches whose WRITE_SIZE is closest to the copy size
+    def copy_scale(d, counter_bytes_expected):
+        cands = []
+        for name, vals in d.items():
+            if "copy" in name.lower():
+                for v in vals:
+                    cands.append(v * 1024.0)
+        best = max(cands) if cands else None
+        return (counter_bytes_expected / best) if best else None
+    f_scale = copy_scale(fetch, COPY_BYTES)
+    w_scale = copy_scale(write, COPY_BYTES)
+    out = {"fetch_scale": f_scale, "write_scale": w_scale, "kernels": {}}
+    total = 0.0
+    for name in sorted(set(fetch) | set(write)):
+        short = _short(name)
+        if not any(k in name for k in EXTRACT_KERNELS):
+            continue
+        fv = fetch.get(name, [])
+        wv = write.get(name, [])
+        fb = (sum(fv) / len(fv)) * 1024.0 * f_scale if fv else 0.0
+        wb = (sum(wv) / len(wv)) * 1024.0 * w_scale if wv else 0.0
+        out["kernels"][short] = {"dispatches": len(fv), "fetch_bytes": round(fb),
+                                 "write_bytes": round(wb)}
+        total += fb + wb
+    out["hbm_bytes_per_launch"] = round(total)
+    return out
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    sub = ap.add_subparsers(dest="cmd", required=True)
+    a = sub.add_parser("stats")
+    a.add_argument("csv")
+    a.add_argument("--match", nargs="*")
+    b = sub.add_parser("traffic")
+    b.add_argument("fetch_dir")
+    b.add_argument("write_dir")
+    b.add_argument("--out")
+    b.add_argument("--meta", default="{}", help="JSON merged into the output")
+    args = ap.parse_args(argv)
+    if args.cmd == "stats":
+        print("| kernel | calls | avg us | total us |")
+        print("|---|---|---|---|")
+        for name, calls, avg, tot in kernel_stats(args.csv, args.match):
+            print(f"| {_short(name)} | {calls} | {avg:.2f} | {tot:.1f} |")
+        return 0
+    res = traffic(args.fetch_dir, args.write_dir)
+    res.update(json.loads(args.meta))
+    txt = json.dumps(res, indent=1)
+    if args.out:
+        with open(args.out, "w") as f:
+            f.write(txt + "\n")
+    print(txt)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())
