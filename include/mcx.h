@@ -172,6 +172,35 @@ int mcx_integrate_f32(void* d_vol, int X, int Y, int Z, float h, float trunc, in
                       float depth_scale, int W, int H, int K, void* d_ws, size_t ws_bytes,
                       void* stream);
 
+/* ---- fp32 TSDF volumes, differentiable integration (no counterpart in the
+ *      reference).  tsdf (float) and weight (int32) in separate [z][y][x]
+ *      arrays; the observation of mcx_integrate (same projection, ray-distance
+ *      sdf, skips and weight cap), stored unpacked:
+ *        f <- fma(f, w, t) / (w + 1),  w <- min(w + 1, max_w),
+ *        t = clamp(sdf / trunc, -1, 1),
+ *      with float depth frames (metres = value * depth_scale).
+ * mcx_tsdf_init: every voxel = (tsdf_value, 0).
+ * mcx_integrate_tsdf: fuse K frames (workspace: mcx_integrate_workspace_size).
+ * mcx_integrate_tsdf_backward: d_grad_depth (K x H x W) = dL/d(depth value)
+ *   from d_grad_tsdf = dL/df after the call that fused these K frames into a
+ *   volume whose weights before that call were d_weight0 (NULL: all 0).  Per
+ *   voxel the observations are recomputed; each one's df/dt is the closed form
+ *   of the capped running average; dt/dd is 0 where t is clamped.  Sums over
+ *   voxels use int64 fixed-point atomics (exact, independent of order), so the
+ *   result is deterministic.  K <= 512.  Workspace:
+ *   mcx_integrate_tsdf_backward_workspace_size(W, H, K) bytes. */
+int mcx_tsdf_init(float* d_tsdf, int32_t* d_weight, int64_t n, float tsdf_value, void* stream);
+int mcx_integrate_tsdf(float* d_tsdf, int32_t* d_weight, int X, int Y, int Z, float h,
+                       float trunc, int max_w, const float* intr, const float* d_poses,
+                       const float* d_depth, float depth_scale, int W, int H, int K,
+                       void* d_ws, size_t ws_bytes, void* stream);
+size_t mcx_integrate_tsdf_backward_workspace_size(int W, int H, int K);
+int mcx_integrate_tsdf_backward(const float* d_grad_tsdf, const int32_t* d_weight0, int X,
+                                int Y, int Z, float h, float trunc, int max_w,
+                                const float* intr, const float* d_poses, const float* d_depth,
+                                float depth_scale, int W, int H, int K, float* d_grad_depth,
+                                void* d_ws, size_t ws_bytes, void* stream);
+
 /* ---- synthetic depth frames (test/benchmark input; no reference
  *      counterpart: the reference reads OpenNI frames).  Ray-casts nobj
  *      objects (8 floats each: kind 0 sphere {c, r} / 1 box {c, half sizes})

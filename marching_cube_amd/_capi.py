@@ -53,6 +53,13 @@ SIGNATURES = {
     "mcx_extract_backward_recompute": (_i32, [_vp, _i32, _i32, _i32, _f32, _vp, _vp, _vp, _sz,
                                               _vp, _sz, _vp]),
     "mcx_volume_init": (_i32, [_vp, _i32, _i32, _i32, _vp]),
+    "mcx_tsdf_init": (_i32, [_vp, _vp, _i64, _f32, _vp]),
+    "mcx_integrate_tsdf": (_i32, [_vp, _vp, _i32, _i32, _i32, _f32, _f32, _i32, _vp, _vp, _vp,
+                                  _f32, _i32, _i32, _i32, _vp, _sz, _vp]),
+    "mcx_integrate_tsdf_backward_workspace_size": (_sz, [_i32, _i32, _i32]),
+    "mcx_integrate_tsdf_backward": (_i32, [_vp, _vp, _i32, _i32, _i32, _f32, _f32, _i32, _vp,
+                                           _vp, _vp, _f32, _i32, _i32, _i32, _vp, _vp, _sz,
+                                           _vp]),
     "mcx_integrate_workspace_size": (_sz, [_i32, _i32]),
     "mcx_integrate": (_i32, [_vp, _i32, _i32, _i32, _f32, _f32, _i32, _vp, _vp, _vp, _f32, _i32,
                              _i32, _i32, _vp, _sz, _vp]),

@@ -1027,6 +1027,9 @@ __global__ __launch_bounds__(kEmitThreads) void contrib_kernel(ExtractParams P, 
   }
   __syncthreads();
   const int nlist = s_tmp[7];
+  // records were dropped (staged path) by the forward: no exact gradient;
+  // make it loud (NaN) rather than silently wrong
+  const float poison = P.hdr->dense_last ? __uint_as_float(0x7fc00000u) : 0.0f;
   int64_t carry = 0;
   for (int rb = 0; rb < nlist; rb += kEmitThreads) {
     const int k = rb + tid;
@@ -1080,8 +1083,8 @@ __global__ __launch_bounds__(kEmitThreads) void contrib_kernel(ExtractParams P, 
         for (int w = 0; w < 8; ++w) c[w] += (ca == w ? ga : 0.0f) + (cb == w ? gb : 0.0f);
       }
       float4* dst = reinterpret_cast<float4*>(contrib + ri * 8);
-      dst[0] = make_float4(c[0], c[1], c[2], c[3]);
-      dst[1] = make_float4(c[4], c[5], c[6], c[7]);
+      dst[0] = make_float4(c[0] + poison, c[1] + poison, c[2] + poison, c[3] + poison);
+      dst[1] = make_float4(c[4] + poison, c[5] + poison, c[6] + poison, c[7] + poison);
     }
     carry += tot;
   }

@@ -24,6 +24,9 @@
 #include <float.h>
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <cmath>
+
 #include "mcx_internal.h"
 
 namespace mcx {
@@ -112,6 +115,170 @@ __global__ __launch_bounds__(256) void integrate_kernel(IntegrateParams P) {
     cur = (uint32_t)(uint16_t)(int16_t)fixedp | ((uint32_t)wnew << 16);
   }
   P.vol[i] = cur;
+}
+
+// ---- fp32 TSDF volumes (differentiable form; no counterpart in the reference)
+// tsdf (fp32) and weight (int32) in separate [z][y][x] arrays; the same
+// observation as above, stored unpacked:
+//   f <- fma(f, w, t) / (w + 1),  w <- min(w + 1, max_w),  t = clamp(sdf / trunc, -1, 1).
+// Observation k of voxel (px, py, pz): pixel, t and dt/d(depth value) (zero
+// where t is clamped).  Returns false where the voxel is not updated.
+template <typename DT>
+__device__ __forceinline__ bool observe(const IntegrateParams& P, const float* R, int k,
+                                        float px, float py, float pz, int* pix_out, float* t_out,
+                                        float* dtdd_out) {
+  const float dx = px - R[9], dy = py - R[10], dz = pz - R[11];
+  const float qx = __fmaf_rn(R[6], dz, __fmaf_rn(R[3], dy, R[0] * dx));
+  const float qy = __fmaf_rn(R[7], dz, __fmaf_rn(R[4], dy, R[1] * dx));
+  const float qz = __fmaf_rn(R[8], dz, __fmaf_rn(R[5], dy, R[2] * dx));
+  if (qz < FLT_MIN) return false;
+  const float u = __fmaf_rn(P.fx, qx / qz, P.cx);
+  const float v = __fmaf_rn(P.fy, qy / qz, P.cy);
+  if (!(fabsf(u) < 1e9f) || !(fabsf(v) < 1e9f)) return false;
+  const int ui = (int)rintf(u), vi = (int)rintf(v);
+  if (ui < 0 || vi < 0 || ui >= P.W - 2 || vi >= P.H - 2) return false;
+  const int pix = vi * P.W + ui;
+  const DT dm = static_cast<const DT*>(P.depth)[(int64_t)k * P.W * P.H + pix];
+  const float rf = P.rayfac[pix];
+  const float d = (float)dm * P.depth_scale;
+  const float zval = d * rf;
+  if (!(zval > FLT_MIN)) return false;
+  const float qn = sqrtf(__fmaf_rn(qz, qz, __fmaf_rn(qy, qy, qx * qx)));
+  const float sdf = zval - qn;
+  if (!(sdf >= -P.trunc)) return false;
+  const float ts = sdf / P.trunc;
+  *pix_out = pix;
+  *t_out = fminf(fmaxf(ts, -1.0f), 1.0f);
+  *dtdd_out = (ts > -1.0f && ts < 1.0f) ? P.depth_scale * rf / P.trunc : 0.0f;
+  return true;
+}
+
+__global__ __launch_bounds__(256) void integrate_tsdf_kernel(IntegrateParams P, float* tsdf,
+                                                             int32_t* wvol) {
+  __shared__ float s_pose[kFramesPerPass * 12];
+  for (int i = threadIdx.x; i < P.K * 12; i += blockDim.x) s_pose[i] = P.poses[i];
+  __syncthreads();
+  const int64_t n = (int64_t)P.X * P.Y * P.Z;
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t xy = (uint32_t)((int64_t)P.X * P.Y);
+  const int z = (int)(i / xy);
+  const uint32_t rem = (uint32_t)(i - (int64_t)z * xy);
+  const uint32_t y = P.divX.div(rem);
+  const int x = (int)(rem - y * (uint32_t)P.X);
+  const float px = ((float)x + 0.5f) * P.h;
+  const float py = ((float)(int)y + 0.5f) * P.h;
+  const float pz = ((float)z + 0.5f) * P.h;
+  float f = tsdf[i];
+  int w = wvol[i];
+  for (int k = 0; k < P.K; ++k) {
+    int pix;
+    float t, dtdd;
+    if (!observe<float>(P, s_pose + 12 * k, k, px, py, pz, &pix, &t, &dtdd)) continue;
+    f = __fmaf_rn(f, (float)w, t) / (float)(w + 1);
+    w = w + 1 < P.max_w ? w + 1 : P.max_w;
+  }
+  tsdf[i] = f;
+  wvol[i] = w;
+}
+
+// Deterministic max of |g| over n floats: per-block max, then an integer
+// atomicMax on the bit pattern (order-preserving for non-negative floats).
+__global__ __launch_bounds__(256) void maxabs_kernel(const float* g, int64_t n, uint32_t* out) {
+  __shared__ uint32_t s_m[4];
+  uint32_t m = 0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    m = max(m, __float_as_uint(g[i]) & 0x7fffffffu);
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) m = max(m, (uint32_t)__shfl_xor((int)m, o));
+  if ((threadIdx.x & 63) == 0) s_m[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) atomicMax(out, max(max(s_m[0], s_m[1]), max(s_m[2], s_m[3])));
+}
+
+// Fixed-point scale of the depth-gradient accumulators: contributions are
+// bounded by max|g| * max(dt/dd); at most 2^20 of them reach one pixel, so
+// scale * bound * 2^20 <= 2^62 keeps every int64 sum exact.
+__device__ __forceinline__ double grad_scale(uint32_t maxg_bits, float max_dtdd) {
+  const double bound = (double)__uint_as_float(maxg_bits) * (double)max_dtdd;
+  return bound > 0.0 ? 4398046511104.0 / bound : 1.0;  // 2^42 / bound
+}
+
+// Backward of integrate_tsdf_kernel for the K frames of one call: for every
+// voxel with dL/df != 0, recompute its observations; observation i (weight
+// before it wb_i = min(w0 + i, max_w)) has df/dt_i = 1/(wb_i + 1) * prod over
+// later observations j of wb_j / (wb_j + 1) (closed form: the uncapped part
+// telescopes, the capped part is a power of max_w / (max_w + 1)); the
+// contribution g * df/dt_i * dt_i/dd goes to the observed pixel as an int64
+// fixed-point atomic add (integer sums do not depend on the order).
+__global__ __launch_bounds__(256) void integrate_tsdf_backward_kernel(
+    IntegrateParams P, const float* __restrict__ g, const int32_t* __restrict__ w0vol,
+    const uint32_t* maxg, float max_dtdd, unsigned long long* acc) {
+  __shared__ float s_pose[kFramesPerPass * 12];
+  for (int i = threadIdx.x; i < P.K * 12; i += blockDim.x) s_pose[i] = P.poses[i];
+  __syncthreads();
+  const int64_t n = (int64_t)P.X * P.Y * P.Z;
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float gi = g[i];
+  if (gi == 0.0f) return;
+  const double scale = grad_scale(*maxg, max_dtdd);
+  const uint32_t xy = (uint32_t)((int64_t)P.X * P.Y);
+  const int z = (int)(i / xy);
+  const uint32_t rem = (uint32_t)(i - (int64_t)z * xy);
+  const uint32_t y = P.divX.div(rem);
+  const int x = (int)(rem - y * (uint32_t)P.X);
+  const float px = ((float)x + 0.5f) * P.h;
+  const float py = ((float)(int)y + 0.5f) * P.h;
+  const float pz = ((float)z + 0.5f) * P.h;
+  const int M = P.max_w;
+  const int w0 = w0vol ? w0vol[i] : 0;
+  int nobs = 0;
+  for (int k = 0; k < P.K; ++k) {
+    int pix;
+    float t, dtdd;
+    if (observe<float>(P, s_pose + 12 * k, k, px, py, pz, &pix, &t, &dtdd)) ++nobs;
+  }
+  const int jc = M - w0 > 0 ? M - w0 : 0;  // first observation that sees the cap
+  const double cap_a = (double)M / (double)(M + 1);
+  int o = 0;
+  for (int k = 0; k < P.K; ++k) {
+    int pix;
+    float t, dtdd;
+    if (!observe<float>(P, s_pose + 12 * k, k, px, py, pz, &pix, &t, &dtdd)) continue;
+    const int wb = w0 + o < M ? w0 + o : M;
+    double c = 1.0 / (double)(wb + 1);
+    const int last_unc = (nobs - 1 < jc - 1) ? nobs - 1 : jc - 1;  // last uncapped index
+    if (o + 1 <= last_unc) c *= (double)(w0 + o + 1) / (double)(w0 + last_unc + 1);
+    const int first_cap = o + 1 > jc ? o + 1 : jc;
+    const int ncap = nobs - first_cap;
+    if (ncap > 0) c *= pow(cap_a, (double)ncap);
+    const double contrib = (double)gi * c * (double)dtdd;
+    if (contrib != 0.0) {
+      const long long q = llrint(contrib * scale);
+      atomicAdd(acc + (int64_t)k * P.W * P.H + pix, (unsigned long long)q);
+    }
+    ++o;
+  }
+}
+
+__global__ __launch_bounds__(256) void fixed_to_float_kernel(const unsigned long long* acc,
+                                                             int64_t n, const uint32_t* maxg,
+                                                             float max_dtdd, float* out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double scale = grad_scale(*maxg, max_dtdd);
+  out[i] = (float)((double)(long long)acc[i] / scale);
+}
+
+__global__ __launch_bounds__(256) void fill_f32_i32_kernel(float* f, int32_t* w, int64_t n,
+                                                           float fv, int32_t wv) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    if (f) f[i] = fv;
+    if (w) w[i] = wv;
+  }
 }
 
 // ---- synthetic depth frames: ray-cast spheres and axis-aligned boxes -------
@@ -219,23 +386,18 @@ extern "C" size_t mcx_integrate_workspace_size(int W, int H) {
   return (size_t)(W > 0 ? W : 0) * (size_t)(H > 0 ? H : 0) * sizeof(float);
 }
 
-template <typename DT>
-int integrate_impl(void* d_vol, int X, int Y, int Z, float h, float trunc, int max_w,
-                   const float* intr, const float* d_poses, const DT* d_depth, float depth_scale,
-                   int W, int H, int K, void* d_ws, size_t ws_bytes, void* stream) {
-  if (!d_vol || !intr || X < 1 || Y < 1 || Z < 1 || W < 3 || H < 3 || K < 0) return MCX_ERR_ARG;
+// Checks shared by the integration entry points; fills the launch parameters
+// (all but vol, K, poses and depth) and computes the per-pixel ray factors.
+int prepare_integrate(IntegrateParams& P, int X, int Y, int Z, float h, float trunc, int max_w,
+                      const float* intr, float depth_scale, int W, int H, int K, float* rayfac,
+                      hipStream_t s) {
+  if (!intr || X < 1 || Y < 1 || Z < 1 || W < 3 || H < 3 || K < 0) return MCX_ERR_ARG;
   if (!(h > 0.f) || !(trunc > 0.f) || max_w < 1 || max_w > 65535) return MCX_ERR_ARG;
-  if (K == 0) return MCX_OK;
-  if (!d_poses || !d_depth) return MCX_ERR_ARG;
   if ((int64_t)X * Y > 0xffffffffll) return MCX_ERR_UNSUPPORTED;
-  if (!d_ws || ws_bytes < mcx_integrate_workspace_size(W, H)) return MCX_ERR_WORKSPACE;
-  hipStream_t s = as_stream(stream);
-  float* rayfac = static_cast<float*>(d_ws);
   hipLaunchKernelGGL(rayfac_kernel, dim3(blocks_for((int64_t)W * H)), dim3(256), 0, s, rayfac, W,
                      H, intr[0], intr[1], intr[2], intr[3]);
   MCX_HIP(hipGetLastError());
-  IntegrateParams P;
-  P.vol = static_cast<uint32_t*>(d_vol);
+  P.vol = nullptr;
   P.X = X;
   P.Y = Y;
   P.Z = Z;
@@ -252,6 +414,26 @@ int integrate_impl(void* d_vol, int X, int Y, int Z, float h, float trunc, int m
   P.H = H;
   P.divX = FastDiv((uint32_t)X);
   P.divY = FastDiv((uint32_t)Y);
+  P.K = 0;
+  P.poses = nullptr;
+  P.depth = nullptr;
+  return MCX_OK;
+}
+
+template <typename DT>
+int integrate_impl(void* d_vol, int X, int Y, int Z, float h, float trunc, int max_w,
+                   const float* intr, const float* d_poses, const DT* d_depth, float depth_scale,
+                   int W, int H, int K, void* d_ws, size_t ws_bytes, void* stream) {
+  if (!d_vol) return MCX_ERR_ARG;
+  if (K > 0 && (!d_poses || !d_depth)) return MCX_ERR_ARG;
+  if (!d_ws || ws_bytes < mcx_integrate_workspace_size(W, H)) return MCX_ERR_WORKSPACE;
+  if (K == 0) return MCX_OK;
+  hipStream_t s = as_stream(stream);
+  IntegrateParams P;
+  int rc = prepare_integrate(P, X, Y, Z, h, trunc, max_w, intr, depth_scale, W, H, K,
+                             static_cast<float*>(d_ws), s);
+  if (rc) return rc;
+  P.vol = static_cast<uint32_t*>(d_vol);
   const int64_t n = (int64_t)X * Y * Z;
   for (int k0 = 0; k0 < K; k0 += kFramesPerPass) {
     P.K = K - k0 < kFramesPerPass ? K - k0 : kFramesPerPass;
@@ -262,6 +444,21 @@ int integrate_impl(void* d_vol, int X, int Y, int Z, float h, float trunc, int m
   }
   return MCX_OK;
 }
+
+// Largest dt/d(depth value) of any pixel in the window (the ray factor grows
+// towards the corners), with a margin: the bound of the fixed-point scale.
+float max_dtdd(const float* intr, float depth_scale, float trunc, int W, int H) {
+  double m = 0.0;
+  const int us[2] = {0, W - 3}, vs[2] = {0, H - 3};
+  for (int a = 0; a < 2; ++a)
+    for (int b = 0; b < 2; ++b) {
+      const double ax = ((double)us[a] - intr[2]) / intr[0], ay = ((double)vs[b] - intr[3]) / intr[1];
+      m = std::max(m, std::sqrt(ax * ax + ay * ay + 1.0));
+    }
+  return (float)(1.01 * m * (double)depth_scale / (double)trunc);
+}
+
+size_t al256(size_t b) { return (b + 255) & ~(size_t)255; }
 
 extern "C" int mcx_integrate(void* d_vol, int X, int Y, int Z, float h, float trunc, int max_w,
                              const float* intr, const float* d_poses, const uint16_t* d_depth,
@@ -290,6 +487,93 @@ extern "C" int mcx_synth_depth(uint16_t* d_depth, int W, int H, int K, const flo
   hipLaunchKernelGGL(synth_kernel, dim3(blocks_for(n)), dim3(256), 0, as_stream(stream), d_depth,
                      W, H, K, intr[0], intr[1], intr[2], intr[3], d_poses, d_objs, nobj,
                      noise_sigma_m, dropout, seed, depth_unit_m);
+  MCX_HIP(hipGetLastError());
+  return MCX_OK;
+}
+
+// ---- fp32 TSDF volumes: differentiable integration (no counterpart in the
+//      reference; the forward is the same observation, stored unpacked)
+extern "C" int mcx_tsdf_init(float* d_tsdf, int32_t* d_weight, int64_t n, float tsdf_value,
+                             void* stream) {
+  if (n < 0 || (n > 0 && !d_tsdf && !d_weight)) return MCX_ERR_ARG;
+  if (n == 0) return MCX_OK;
+  int64_t b = (n + 255) / 256;
+  if (b > 65536) b = 65536;
+  hipLaunchKernelGGL(fill_f32_i32_kernel, dim3((unsigned)b), dim3(256), 0, as_stream(stream),
+                     d_tsdf, d_weight, n, tsdf_value, 0);
+  MCX_HIP(hipGetLastError());
+  return MCX_OK;
+}
+
+extern "C" int mcx_integrate_tsdf(float* d_tsdf, int32_t* d_weight, int X, int Y, int Z, float h,
+                                  float trunc, int max_w, const float* intr,
+                                  const float* d_poses, const float* d_depth, float depth_scale,
+                                  int W, int H, int K, void* d_ws, size_t ws_bytes,
+                                  void* stream) {
+  if (!d_tsdf || !d_weight) return MCX_ERR_ARG;
+  if (K > 0 && (!d_poses || !d_depth)) return MCX_ERR_ARG;
+  if (!d_ws || ws_bytes < mcx_integrate_workspace_size(W, H)) return MCX_ERR_WORKSPACE;
+  if (K == 0) return MCX_OK;
+  hipStream_t s = as_stream(stream);
+  IntegrateParams P;
+  int rc = prepare_integrate(P, X, Y, Z, h, trunc, max_w, intr, depth_scale, W, H, K,
+                             static_cast<float*>(d_ws), s);
+  if (rc) return rc;
+  const int64_t n = (int64_t)X * Y * Z;
+  for (int k0 = 0; k0 < K; k0 += kFramesPerPass) {
+    P.K = K - k0 < kFramesPerPass ? K - k0 : kFramesPerPass;
+    P.poses = d_poses + (int64_t)12 * k0;
+    P.depth = d_depth + (int64_t)k0 * W * H;
+    hipLaunchKernelGGL(integrate_tsdf_kernel, dim3(blocks_for(n)), dim3(256), 0, s, P, d_tsdf,
+                       d_weight);
+    MCX_HIP(hipGetLastError());
+  }
+  return MCX_OK;
+}
+
+extern "C" size_t mcx_integrate_tsdf_backward_workspace_size(int W, int H, int K) {
+  if (W < 1 || H < 1 || K < 0) return 0;
+  return al256((size_t)W * H * sizeof(float)) + 256 + (size_t)K * W * H * 8;
+}
+
+extern "C" int mcx_integrate_tsdf_backward(const float* d_grad_tsdf, const int32_t* d_weight0,
+                                           int X, int Y, int Z, float h, float trunc, int max_w,
+                                           const float* intr, const float* d_poses,
+                                           const float* d_depth, float depth_scale, int W,
+                                           int H, int K, float* d_grad_depth, void* d_ws,
+                                           size_t ws_bytes, void* stream) {
+  if (!d_grad_tsdf) return MCX_ERR_ARG;
+  if (K > 0 && (!d_poses || !d_depth || !d_grad_depth)) return MCX_ERR_ARG;
+  if (!d_ws || ws_bytes < mcx_integrate_tsdf_backward_workspace_size(W, H, K))
+    return MCX_ERR_WORKSPACE;
+  if (K == 0) return MCX_OK;
+  if (K > kFramesPerPass) return MCX_ERR_UNSUPPORTED;  // one pass: weights known per voxel
+  hipStream_t s = as_stream(stream);
+  char* b = static_cast<char*>(d_ws);
+  float* rayfac = reinterpret_cast<float*>(b);
+  uint32_t* maxg = reinterpret_cast<uint32_t*>(b + al256((size_t)W * H * sizeof(float)));
+  unsigned long long* acc = reinterpret_cast<unsigned long long*>(
+      b + al256((size_t)W * H * sizeof(float)) + 256);
+  IntegrateParams P;
+  int rc = prepare_integrate(P, X, Y, Z, h, trunc, max_w, intr, depth_scale, W, H, K, rayfac, s);
+  if (rc) return rc;
+  const int64_t n = (int64_t)X * Y * Z;
+  const int64_t npix = (int64_t)K * W * H;
+  rc = fill_bytes_zero(maxg, 256 + (size_t)npix * 8, s);  // max and accumulators
+  if (rc) return rc;
+  int64_t mb = (n + 255) / 256;
+  if (mb > 4096) mb = 4096;
+  hipLaunchKernelGGL(maxabs_kernel, dim3((unsigned)mb), dim3(256), 0, s, d_grad_tsdf, n, maxg);
+  MCX_HIP(hipGetLastError());
+  const float mdt = max_dtdd(intr, depth_scale, trunc, W, H);
+  P.K = K;
+  P.poses = d_poses;
+  P.depth = d_depth;
+  hipLaunchKernelGGL(integrate_tsdf_backward_kernel, dim3(blocks_for(n)), dim3(256), 0, s, P,
+                     d_grad_tsdf, d_weight0, maxg, mdt, acc);
+  MCX_HIP(hipGetLastError());
+  hipLaunchKernelGGL(fixed_to_float_kernel, dim3(blocks_for(npix)), dim3(256), 0, s, acc, npix,
+                     maxg, mdt, d_grad_depth);
   MCX_HIP(hipGetLastError());
   return MCX_OK;
 }

@@ -94,3 +94,106 @@ void orc_integrate(uint32_t* vol, int X, int Y, int Z, float h, float trunc,
         vol[i] = cur;
       }
 }
+
+/* ---- fp32 TSDF volumes (differentiable form): the same observation stored
+ * unpacked, f <- fma(f, w, t) / (w + 1), w <- min(w + 1, max_w); float depth
+ * frames.  The pixel lookup and every float operation follow update_voxel. */
+static inline int observe_f32(float px, float py, float pz, float trunc, const float* intr,
+                              const float* P, const float* depth, float depth_scale, int W,
+                              int H, float* t_out) {
+  float fx = intr[0], fy = intr[1], cx = intr[2], cy = intr[3];
+  float dx = px - P[9], dy = py - P[10], dz = pz - P[11];
+  float qx = fmaf(P[6], dz, fmaf(P[3], dy, P[0] * dx));
+  float qy = fmaf(P[7], dz, fmaf(P[4], dy, P[1] * dx));
+  float qz = fmaf(P[8], dz, fmaf(P[5], dy, P[2] * dx));
+  if (qz < FLT_MIN) return 0;
+  float u = fmaf(fx, qx / qz, cx);
+  float v = fmaf(fy, qy / qz, cy);
+  if (!(fabsf(u) < 1e9f) || !(fabsf(v) < 1e9f)) return 0;
+  int ui = (int)rintf(u), vi = (int)rintf(v);
+  if (ui < 0 || vi < 0 || ui >= W - 2 || vi >= H - 2) return 0;
+  float d = depth[(int64_t)vi * W + ui] * depth_scale;
+  float ax = ((float)ui - cx) / fx, ay = ((float)vi - cy) / fy;
+  float zval = d * sqrtf(fmaf(ay, ay, ax * ax) + 1.0f);
+  if (!(zval > FLT_MIN)) return 0;
+  float qn = sqrtf(fmaf(qz, qz, fmaf(qy, qy, qx * qx)));
+  float sdf = zval - qn;
+  if (!(sdf >= -trunc)) return 0;
+  float t = sdf / trunc;
+  if (t > 1.0f) t = 1.0f;
+  if (t < -1.0f) t = -1.0f;
+  *t_out = t;
+  return 1;
+}
+
+void orc_integrate_tsdf(float* tsdf, int32_t* w, int X, int Y, int Z, float h, float trunc,
+                        int max_w, const float* intr, const float* poses, const float* depth,
+                        float depth_scale, int W, int H, int nframes) {
+#pragma omp parallel for schedule(static)
+  for (int z = 0; z < Z; ++z)
+    for (int y = 0; y < Y; ++y)
+      for (int x = 0; x < X; ++x) {
+        int64_t i = ((int64_t)z * Y + y) * X + x;
+        float px = ((float)x + 0.5f) * h, py = ((float)y + 0.5f) * h,
+              pz = ((float)z + 0.5f) * h;
+        float f = tsdf[i];
+        int wi = w[i];
+        for (int k = 0; k < nframes; ++k) {
+          float t;
+          if (!observe_f32(px, py, pz, trunc, intr, poses + 12 * k,
+                           depth + (int64_t)k * W * H, depth_scale, W, H, &t))
+            continue;
+          f = fmaf(f, (float)wi, t) / (float)(wi + 1);
+          wi = wi + 1 < max_w ? wi + 1 : max_w;
+        }
+        tsdf[i] = f;
+        w[i] = wi;
+      }
+}
+
+/* FP64 form for finite differences with respect to the depth values: the
+ * pixel of each observation is chosen exactly as above (in fp32, from the
+ * pose: independent of the depth), everything that depends on the depth is
+ * evaluated in double. */
+void orc_integrate_tsdf_f64(double* tsdf, int32_t* w, int X, int Y, int Z, float h, float trunc,
+                            int max_w, const float* intr, const float* poses,
+                            const double* depth, double depth_scale, int W, int H, int nframes) {
+  const float fx = intr[0], fy = intr[1], cx = intr[2], cy = intr[3];
+#pragma omp parallel for schedule(static)
+  for (int z = 0; z < Z; ++z)
+    for (int y = 0; y < Y; ++y)
+      for (int x = 0; x < X; ++x) {
+        int64_t i = ((int64_t)z * Y + y) * X + x;
+        float px = ((float)x + 0.5f) * h, py = ((float)y + 0.5f) * h,
+              pz = ((float)z + 0.5f) * h;
+        double f = tsdf[i];
+        int wi = w[i];
+        for (int k = 0; k < nframes; ++k) {
+          const float* P = poses + 12 * k;
+          float dx = px - P[9], dy = py - P[10], dz = pz - P[11];
+          float qx = fmaf(P[6], dz, fmaf(P[3], dy, P[0] * dx));
+          float qy = fmaf(P[7], dz, fmaf(P[4], dy, P[1] * dx));
+          float qz = fmaf(P[8], dz, fmaf(P[5], dy, P[2] * dx));
+          if (qz < FLT_MIN) continue;
+          float u = fmaf(fx, qx / qz, cx);
+          float v = fmaf(fy, qy / qz, cy);
+          if (!(fabsf(u) < 1e9f) || !(fabsf(v) < 1e9f)) continue;
+          int ui = (int)rintf(u), vi = (int)rintf(v);
+          if (ui < 0 || vi < 0 || ui >= W - 2 || vi >= H - 2) continue;
+          double d = depth[(int64_t)k * W * H + (int64_t)vi * W + ui] * depth_scale;
+          double ax = ((double)ui - cx) / fx, ay = ((double)vi - cy) / fy;
+          double zval = d * sqrt(ay * ay + ax * ax + 1.0);
+          if (!(zval > (double)FLT_MIN)) continue;
+          double qn = sqrt((double)qz * qz + (double)qy * qy + (double)qx * qx);
+          double sdf = zval - qn;
+          if (!(sdf >= -(double)trunc)) continue;
+          double t = sdf / trunc;
+          if (t > 1.0) t = 1.0;
+          if (t < -1.0) t = -1.0;
+          f = (f * wi + t) / (double)(wi + 1);
+          wi = wi + 1 < max_w ? wi + 1 : max_w;
+        }
+        tsdf[i] = f;
+        w[i] = wi;
+      }
+}

@@ -54,6 +54,9 @@ static inline void read_voxel(const void* vol, int elem, int64_t i, float* f,
     const int16_t* p = (const int16_t*)vol + 2 * i;
     *f = (float)p[0] / ORC_TSDF_DIV;
     *w = (int)(uint16_t)p[1];
+  } else if (elem == ORC_ELEM_F64) {
+    *f = (float)((const double*)vol)[i];
+    *w = 1;
   } else {
     *f = ((const float*)vol)[i];
     *w = 1;
@@ -243,6 +246,7 @@ int64_t orc_extract(const void* vol, int elem, int X, int Y, int Z, float h,
 
 static inline double read_f64(const void* vol, int elem, int64_t i) {
   if (elem == ORC_ELEM_SHORT2) return (double)((const int16_t*)vol)[2 * i] / 32766.0;
+  if (elem == ORC_ELEM_F64) return ((const double*)vol)[i];
   return (double)((const float*)vol)[i];
 }
 

@@ -30,6 +30,7 @@ extern "C" {
 /* voxel element kinds (same values as the product's MCX_ELEM_*) */
 #define ORC_ELEM_SHORT2 0 /* cu_voxel_t {int16 tsdf; uint16 w} (structure.h:14-18) */
 #define ORC_ELEM_F32 1    /* plain fp32 field, no weights */
+#define ORC_ELEM_F64 2    /* plain FP64 field (finite-difference checks only) */
 
 /* vertex_voxel (structure.h:29-35): 28 bytes */
 typedef struct {
@@ -81,6 +82,12 @@ void orc_extract_backward_f64(const float* vol, int X, int Y, int Z, double h,
 
 /* TSDF volume init + projective integration (see integrate_oracle.c). */
 void orc_volume_init(uint32_t* vol, int64_t n);
+void orc_integrate_tsdf(float* tsdf, int32_t* w, int X, int Y, int Z, float h, float trunc,
+                        int max_w, const float* intr, const float* poses, const float* depth,
+                        float depth_scale, int W, int H, int nframes);
+void orc_integrate_tsdf_f64(double* tsdf, int32_t* w, int X, int Y, int Z, float h, float trunc,
+                            int max_w, const float* intr, const float* poses,
+                            const double* depth, double depth_scale, int W, int H, int nframes);
 void orc_integrate(uint32_t* vol, int X, int Y, int Z, float h, float trunc,
                    int max_w, const float* intr, const float* poses,
                    const uint16_t* depth, float depth_scale, int W, int H,

@@ -16,6 +16,7 @@ LIB_PATH = os.path.join(HERE, "liboracle.so")
 
 ELEM_SHORT2 = 0
 ELEM_F32 = 1
+ELEM_F64 = 2  # finite-difference checks only
 
 # vertex_voxel (structure.h:29-35), 28 bytes
 VERTEX_VOXEL = np.dtype([("pos", "<f4", (3,)), ("vox", "<i4", (3,)),
@@ -45,6 +46,12 @@ def lib():
         L.orc_normals.argtypes = [vp, i32, i32, i32, f32, vp, i64, i64, vp]
         L.orc_volume_init.restype = None
         L.orc_volume_init.argtypes = [vp, i64]
+        L.orc_integrate_tsdf.restype = None
+        L.orc_integrate_tsdf.argtypes = [vp, vp, i32, i32, i32, f32, f32, i32, vp, vp, vp, f32,
+                                         i32, i32, i32]
+        L.orc_integrate_tsdf_f64.restype = None
+        L.orc_integrate_tsdf_f64.argtypes = [vp, vp, i32, i32, i32, f32, f32, i32, vp, vp, vp,
+                                             ctypes.c_double, i32, i32, i32]
         L.orc_integrate.restype = None
         L.orc_integrate.argtypes = [vp, i32, i32, i32, f32, f32, i32, vp, vp, vp, f32,
                                     i32, i32, i32]
@@ -62,6 +69,8 @@ def _ptr(a):
 def _elem(vol):
     if vol.dtype == np.float32:
         return ELEM_F32
+    if vol.dtype == np.float64:
+        return ELEM_F64
     if vol.dtype in (np.uint32, np.int32) or (vol.dtype == np.int16 and vol.shape[-1] == 2):
         return ELEM_SHORT2
     raise TypeError(f"unsupported volume dtype {vol.dtype}")
@@ -169,3 +178,32 @@ def integrate(vol, h, trunc, max_w, intr, poses, depth, depth_scale=0.001):
     lib().orc_integrate(_ptr(vol), X, Y, Z, h, trunc, max_w, _ptr(intr), _ptr(poses),
                         _ptr(depth), depth_scale, W, H, K)
     return vol
+
+
+def integrate_tsdf(tsdf, w, h, trunc, max_w, intr, poses, depth, depth_scale=1.0):
+    """In-place fp32 TSDF integration (mcx_integrate_tsdf semantics): tsdf
+    float32 and w int32 (Z,Y,X), float depth frames (K,H,W)."""
+    assert tsdf.dtype == np.float32 and w.dtype == np.int32
+    assert tsdf.flags.c_contiguous and w.flags.c_contiguous and tsdf.shape == w.shape
+    Z, Y, X = tsdf.shape
+    intr = np.ascontiguousarray(intr, np.float32)
+    poses = np.ascontiguousarray(poses, np.float32).reshape(-1, 12)
+    depth = np.ascontiguousarray(depth, np.float32)
+    K, H, W = depth.shape
+    lib().orc_integrate_tsdf(_ptr(tsdf), _ptr(w), X, Y, Z, h, trunc, max_w, _ptr(intr),
+                             _ptr(poses), _ptr(depth), depth_scale, W, H, K)
+    return tsdf, w
+
+
+def integrate_tsdf_f64(tsdf, w, h, trunc, max_w, intr, poses, depth, depth_scale=1.0):
+    """FP64 form of integrate_tsdf (pixels chosen as in fp32; depth-dependent
+    arithmetic in double): tsdf float64, depth float64."""
+    assert tsdf.dtype == np.float64 and w.dtype == np.int32
+    Z, Y, X = tsdf.shape
+    intr = np.ascontiguousarray(intr, np.float32)
+    poses = np.ascontiguousarray(poses, np.float32).reshape(-1, 12)
+    depth = np.ascontiguousarray(depth, np.float64)
+    K, H, W = depth.shape
+    lib().orc_integrate_tsdf_f64(_ptr(tsdf), _ptr(w), X, Y, Z, h, trunc, max_w, _ptr(intr),
+                                 _ptr(poses), _ptr(depth), depth_scale, W, H, K)
+    return tsdf, w

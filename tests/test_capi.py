@@ -49,6 +49,20 @@ def test_argument_errors_without_device():
     assert lib.mcx_extract(fake, 1, 4, 4, 4, 1.0, fake, None, None, None, 10, None,
                            None, None, 0, None) == _capi.ERR_ARG
     assert lib.mcx_classify(None, 0, 4, 4, 4, None, None, None) == _capi.ERR_ARG
+    # differentiable integration: the backward handles one pass of <= 512 frames;
+    # the checks return before any device work
+    intr = (ctypes.c_float * 4)(60.0, 60.0, 31.5, 23.5)
+    K, W, H = 600, 64, 48
+    wsb = lib.mcx_integrate_tsdf_backward_workspace_size(W, H, K)
+    assert wsb >= K * W * H * 8
+    assert lib.mcx_integrate_tsdf_backward(fake, None, 8, 8, 8, 0.1, 0.3, 128, intr, fake, fake,
+                                           1.0, W, H, K, fake, fake, wsb, None) \
+        == _capi.ERR_UNSUPPORTED
+    assert lib.mcx_integrate_tsdf_backward(None, None, 8, 8, 8, 0.1, 0.3, 128, intr, fake, fake,
+                                           1.0, W, H, 4, fake, fake, wsb, None) == _capi.ERR_ARG
+    assert lib.mcx_integrate_tsdf_backward(fake, None, 8, 8, 8, 0.1, 0.3, 128, intr, fake, fake,
+                                           1.0, W, H, 4, fake, fake, 16, None) \
+        == _capi.ERR_WORKSPACE
 
 
 def test_workspace_size_grows_with_volume():

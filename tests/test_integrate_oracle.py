@@ -80,3 +80,68 @@ def test_outside_window_and_dropout_unchanged():
     assert (w[:2, 3, 7] == 0).all()
     t0, w0 = run([0.0])  # all pixels dropped
     assert (w0 == 0).all() and (t0 == -32766).all()
+
+
+# ---- fp32 TSDF volumes (differentiable form) --------------------------------
+def run_tsdf(depths_m, max_w=128, init=1.0):
+    tsdf = np.full((N, N, N), init, np.float32)
+    w = np.zeros((N, N, N), np.int32)
+    frames = np.stack([np.full((H, W), d, np.float32) for d in depths_m])
+    poses = np.stack([axis_pose()] * len(depths_m))
+    return O.integrate_tsdf(tsdf, w, HV, TRUNC, max_w, INTR, poses, frames)
+
+
+def test_tsdf_axis_running_average_closed_form():
+    """Two frames along the optical axis: f = (t1 + t2) / 2 with
+    t = clamp((d - distance) / trunc, -1, 1); unobserved voxels keep (init, 0)."""
+    d1, d2 = 1.43, 1.47
+    f, w = run_tsdf([d1, d2])
+    for z in range(N):
+        t1, s1 = expect_tsdf(d1, z)
+        t2, s2 = expect_tsdf(d2, z)
+        obs = [t for t, s in ((t1, s1), (t2, s2)) if s >= -TRUNC]
+        if not obs:
+            assert f[z, 3, 3] == 1.0 and w[z, 3, 3] == 0
+            continue
+        assert w[z, 3, 3] == len(obs)
+        assert abs(f[z, 3, 3] - np.mean(obs)) < 2e-6, z
+
+
+def test_tsdf_weight_cap_gives_moving_average():
+    """max_w = 2: after the cap, f <- (2 f + t) / 3 (closed form for 4 frames)."""
+    ds = [1.43, 1.45, 1.40, 1.44]
+    f, w = run_tsdf(ds, max_w=2)
+    z = 4
+    ts = [float(expect_tsdf(d, z)[0]) for d in ds]
+    ref = ts[0]
+    ref = (ref * 1 + ts[1]) / 2
+    ref = (ref * 2 + ts[2]) / 3
+    ref = (ref * 2 + ts[3]) / 3
+    assert w[z, 3, 3] == 2
+    assert abs(f[z, 3, 3] - ref) < 2e-6
+
+
+def test_tsdf_f64_matches_f32_and_derivative_closed_form():
+    """The FP64 form agrees with fp32 to rounding; its central difference in
+    one frame's depth equals the closed form d f / d d1 = 1 / (2 trunc) (two
+    observations, ray factor 1, unclamped band)."""
+    d1, d2 = 1.43, 1.47
+    f32, _ = run_tsdf([d1, d2])
+    frames = np.stack([np.full((H, W), d, np.float64) for d in (d1, d2)])
+    poses = np.stack([axis_pose()] * 2)
+
+    def f64(fr):
+        t = np.full((N, N, N), 1.0, np.float64)
+        w = np.zeros((N, N, N), np.int32)
+        return O.integrate_tsdf_f64(t, w, HV, TRUNC, 128, INTR, poses, fr)[0]
+
+    base = f64(frames)
+    assert np.abs(base - f32).max() < 5e-6
+    z = 4  # inside the band in both frames
+    assert abs(expect_tsdf(d1, z)[0]) < 1 and abs(expect_tsdf(d2, z)[0]) < 1
+    dl = 1e-6
+    fp, fm = frames.copy(), frames.copy()
+    fp[0, 15, 20] += dl
+    fm[0, 15, 20] -= dl
+    fd = (f64(fp)[z, 3, 3] - f64(fm)[z, 3, 3]) / (2 * dl)
+    assert abs(fd - 1.0 / (2 * TRUNC)) < 1e-6 / TRUNC
