@@ -41,6 +41,10 @@ def parse():
                     help="side of the fp32 bumps field of the forward/backward leg (0 = skip)")
     ap.add_argument("--batch-count", type=int, default=3000,
                     help="volumes of the batched leg (0 = skip)")
+    ap.add_argument("--graph-count", type=int, default=256,
+                    help="volumes of the graph-captured differentiable fusion leg (0 = skip)")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="skip the launch-shape autotuner (default shapes)")
     ap.add_argument("--calib-copy", type=int, default=0,
                     help="after the legs, copy a 512 MiB buffer this many times (a plain copy "
                          "kernel of known size: the scale for rocprofv3 FETCH_SIZE/WRITE_SIZE)")
@@ -179,6 +183,48 @@ def bench_batch(torch, dev, count, steps, warmup):
             "batched_mtri_per_s": round(n / 3 / (b_ms * 1e-3) / 1e6, 1)}
 
 
+def bench_fusion_graph(torch, dev, nvol, steps):
+    """256 volumes with sides uniform in [64, 192], each fused from 16
+    synthetic 320x240 uint16 depth frames (3 random spheres/boxes in front of
+    a far background box, 1.5 mm noise, 2 % dropout) converted to float
+    metres; integrate -> extract -> Chamfer loss to 64 points on a sphere ->
+    backward to every depth pixel, recorded once in a HIP graph.  Replays
+    timed with HIP events, against the same pass launched eagerly."""
+    import numpy as np
+    from marching_cube_amd import fusion, fusion_diff, synth
+    rng = np.random.default_rng(20261202)
+    W, H, K = 320, 240, 16
+    intr = [262.5, 262.5, 159.5, 119.5]
+    shapes, hs, depth, poses, targets = [], [], [], [], []
+    bg = np.array([[1, 0.5, 0.5, 0.5, 20.0, 20.0, 20.0, 0]], np.float32)
+    for b in range(nvol):
+        n = int(rng.integers(64, 193))
+        h = 1.0 / n
+        objs = np.concatenate([synth.kinfu_scene(3, extent=1.0, seed=b), bg])
+        p = synth.orbit_poses(K, extent=1.0, radius=1.6, seed=b)
+        d16 = fusion.synth_depth(intr, p, objs, W, H, sigma=0.0015, dropout=0.02, seed=b,
+                                 device=dev)
+        depth.append((d16.to(torch.float32) * 0.001).contiguous())
+        poses.append(torch.from_numpy(p).to(dev))
+        i = np.arange(64) + 0.5
+        phi = np.arccos(1 - 2 * i / 64)
+        th = np.pi * (1 + 5 ** 0.5) * i
+        pts = 0.5 + 0.25 * np.stack([np.cos(th) * np.sin(phi), np.sin(th) * np.sin(phi),
+                                     np.cos(phi)], -1)
+        targets.append(torch.from_numpy(pts.astype(np.float32)).to(dev))
+        shapes.append((n, n, n))
+        hs.append(h)
+    g = fusion_diff.FusionGraph(shapes, hs, depth, poses, targets, intr, trunc=0.03)
+    e_ms, _ = _event_ms(torch, lambda i: g.step(), 2, 1)
+    g.capture()
+    g_ms, _ = _event_ms(torch, lambda i: g.replay(), max(3, steps // 4), 1)
+    nvox = int(sum(a * b * c for a, b, c in shapes))
+    return {"workload": f"fusion{nvol}_64_192_diff_graph", "volumes": nvol, "voxels": nvox,
+            "frames_per_volume": K, "frame_size": [W, H],
+            "vertices": int(g.count.sum().item()), "graph_ms": round(g_ms, 3),
+            "eager_ms": round(e_ms, 3), "speedup": round(e_ms / g_ms, 2)}
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -214,6 +260,18 @@ def main():
     vols = [vox.data, vox.data.clone()]
     print(f"[bench] volume {dim}^3 fused from {args.frames} frames in {integrate_ms:.1f} ms "
           f"(setup {time.perf_counter() - t0:.1f} s)", file=sys.stderr, flush=True)
+
+    # ---- launch shapes: the autotuner times its candidates on this volume
+    # (part of the set-up; outputs do not depend on them)
+    tuning = None
+    if not args.no_tune:
+        from marching_cube_amd import autotune
+        t = autotune.tune_extract(vols[0], h)
+        tuning = {"columns_per_cu": t["columns_per_cu"],
+                  "emit_groups_per_cu": t["emit_groups_per_cu"], "candidates": len(t["table"])}
+        print(f"[bench] autotune {t['columns_per_cu']} columns/CU, "
+              f"{t['emit_groups_per_cu']} emit groups/CU ({t['ms']:.3f} ms)", file=sys.stderr,
+              flush=True)
 
     # ---- size the output and check the two copies agree
     ws = mesh.Workspace(dev)
@@ -279,6 +337,9 @@ def main():
                  if args.batch_count else None)
     if batch_leg:
         print(f"[bench] batch {batch_leg}", file=sys.stderr, flush=True)
+    graph_leg = bench_fusion_graph(torch, dev, args.graph_count, args.steps) if args.graph_count else None
+    if graph_leg:
+        print(f"[bench] fusion graph {graph_leg}", file=sys.stderr, flush=True)
 
     # ---- CPU baseline: the oracle's extraction on the host cores
     cpu = None
@@ -342,6 +403,8 @@ def main():
         "integrate_gvoxel_frames_per_s": round(nvox * args.frames / (integrate_ms * 1e-3) / 1e9, 2),
         "fwd_bwd": diff_leg,
         "batched": batch_leg,
+        "fusion_graph": graph_leg,
+        "tuning": tuning,
     }
     print(json.dumps(out), flush=True)
     return 0

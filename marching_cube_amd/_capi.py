@@ -22,6 +22,10 @@ ERR_UNSUPPORTED = 6
 ELEM_SHORT2 = 0
 ELEM_F32 = 1
 
+TUNE_COLUMNS_PER_CU = 0
+TUNE_EMIT_GROUPS_PER_CU = 1
+TUNE_BATCH_SLAB = 2
+
 _vp = ctypes.c_void_p
 _i32 = ctypes.c_int
 _i64 = ctypes.c_int64
@@ -53,6 +57,8 @@ SIGNATURES = {
     "mcx_extract_backward_recompute": (_i32, [_vp, _i32, _i32, _i32, _f32, _vp, _vp, _vp, _sz,
                                               _vp, _sz, _vp]),
     "mcx_volume_init": (_i32, [_vp, _i32, _i32, _i32, _vp]),
+    "mcx_set_tuning": (_i32, [_i32, _i32]),
+    "mcx_get_tuning": (_i32, [_i32]),
     "mcx_tsdf_init": (_i32, [_vp, _vp, _i64, _f32, _vp]),
     "mcx_integrate_tsdf": (_i32, [_vp, _vp, _i32, _i32, _i32, _f32, _f32, _i32, _vp, _vp, _vp,
                                   _f32, _i32, _i32, _i32, _vp, _sz, _vp]),

@@ -56,8 +56,16 @@ constexpr int kTile = 64 * kCPW;       // cubes per tile (one plane of a column)
 constexpr int kRecWords = 9;           // record: (offset << 8 | case), 8 corner words
 constexpr int kScanItems = 4096;       // tiles per scan block (256 threads x 16)
 constexpr int kEmitThreads = 256;
-constexpr int kEmitGroupsPerCU = 8;       // K3 persistent workgroups per CU
-constexpr int kColumnsPerCU = 64;         // K1 columns (waves) per CU, single volume
+// Launch-shape tunables (process-wide, set by mcx_set_tuning / the
+// autotuner; read by every launch on the host): K1 columns (waves) per CU for
+// single volumes, K3 persistent workgroups per CU, planes per column of
+// batched plans (taken when a plan is created).
+struct Tuning {
+  int columns_per_cu = 64;
+  int emit_groups_per_cu = 8;
+  int batch_slab = 64;
+};
+Tuning g_tuning;
 constexpr int kTilesPerContribGroup = 8;  // backward BW1: tiles per workgroup
 // third word of a gfx9 raw buffer resource (stride 0, 32-bit data format)
 constexpr int kBufferWord3 = 0x00020000;
@@ -190,11 +198,21 @@ __device__ __forceinline__ int volume_of_tile(const ExtractParams& P, int64_t t)
   return lo;
 }
 
+// The scan kernel's look-back descriptors of this call: their place in the
+// workspace depends on the volume size, so a workspace reused with other
+// sizes holds stale flags there.  Workgroup 0 of the count kernel (which
+// completes before the scan starts) zeroes them.
+__device__ __forceinline__ void clear_scan_descriptors(const ExtractParams& P) {
+  if (blockIdx.x == 0)
+    for (int64_t i = threadIdx.x; i < P.cap_desc; i += blockDim.x) P.desc[i] = 0;
+}
+
 // ============================================================ K1: count
 template <int ELEM, bool BATCH>
 __global__ __launch_bounds__(256) void count_kernel(ExtractParams P) {
   __shared__ uint8_t s_nv[256];
   s_nv[threadIdx.x] = dNumVerts[threadIdx.x];
+  clear_scan_descriptors(P);
   __syncthreads();
   const int lane = threadIdx.x & 63;
   // wave index made wave-uniform so that everything derived from the column
@@ -443,6 +461,7 @@ __global__ __launch_bounds__(256) void count4_kernel(ExtractParams P) {
   __shared__ uint8_t s_nv[256];
   __shared__ uint32_t s_rec[4][256 * kRecWords];  // per wave: one chunk's records
   s_nv[threadIdx.x] = dNumVerts[threadIdx.x];
+  clear_scan_descriptors(P);
   __syncthreads();
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
@@ -706,10 +725,8 @@ __global__ __launch_bounds__(256) void scan_kernel(ExtractParams P) {
     total += s_wave[w];
   }
   const int64_t texcl = (int64_t)incl - s + wpre;
-  const uint32_t parity =
-      __hip_atomic_load(&P.hdr->parity, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & 1u;
-  uint64_t* desc = P.desc + (int64_t)parity * P.cap_desc;
-  uint64_t* desc_other = P.desc + (int64_t)(parity ^ 1u) * P.cap_desc;
+  // the descriptors were zeroed by the count kernel of this call
+  uint64_t* desc = P.desc;
   if (wave == 0) {
     int64_t excl = 0;
     if (blk == 0) {
@@ -729,16 +746,8 @@ __global__ __launch_bounds__(256) void scan_kernel(ExtractParams P) {
         P.hdr->nz_total = P.hdr->nz_count;     // the emit kernel's work list
         P.hdr->nz_count = 0;
         P.hdr->ovf_next = 0;
-        // every block has read the parity (the look-back saw all their
-        // aggregates): flip it for the next call
-        __hip_atomic_store(&P.hdr->parity, parity ^ 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       }
     }
-  } else {
-    const int64_t k = (P.cap_desc + gridDim.x - 1) / gridDim.x;
-    const int64_t beg = blk * k;
-    const int64_t end = min(beg + k, P.cap_desc);
-    for (int64_t q = beg + (tid - 64); q < end; q += 192) desc_other[q] = 0;
   }
   __syncthreads();
   int64_t run = s_excl + texcl;
@@ -1229,7 +1238,7 @@ int run_extract(ExtractParams& P, int elem, bool batch, int maxX, hipStream_t s,
   const unsigned k1_blocks = (unsigned)((P.ncols + 3) / 4);
   const unsigned k2_blocks = (unsigned)((P.n_tiles + kScanItems - 1) / kScanItems);
   const int64_t items = (P.n_tiles + 3) / 4;  // a workgroup is 4 waves, a wave takes tiles
-  const int64_t k3_max = (int64_t)cu_count() * kEmitGroupsPerCU;
+  const int64_t k3_max = (int64_t)cu_count() * g_tuning.emit_groups_per_cu;
   const unsigned k3_blocks = (unsigned)(items < k3_max ? items : k3_max);
 #define MCX_LAUNCH_SET(E, B)                                                                     \
   do {                                                                                           \
@@ -1275,7 +1284,7 @@ VolDesc make_vol(const void* d_vol, int X, int Y, int Z, float h, int16_t* ver_v
 }
 
 // Single-volume launch parameters: workspace layout, the volume, and the
-// column decomposition (R in-plane ranges x S slabs, about kColumnsPerCU
+// column decomposition (R in-plane ranges x S slabs, about columns_per_cu
 // waves per CU: several rounds of waves per SIMD, so that columns with much
 // surface do not leave SIMDs idle at the end).
 // Depends only on (X, Y, Z, ws_bytes), so a backward call sees the forward's
@@ -1291,7 +1300,7 @@ int prepare_single(ExtractParams& P, const void* d_vol, int X, int Y, int Z, flo
   P.nvol = 1;
   P.n_tiles = n_tiles;
   const int64_t R = P.vol0.R;
-  const int64_t target = (int64_t)cu_count() * kColumnsPerCU;
+  const int64_t target = (int64_t)cu_count() * g_tuning.columns_per_cu;
   int64_t S = (target + R - 1) / R;
   if (S > Z - 1) S = Z - 1;
   P.slab = (int)((Z - 1 + S - 1) / S);
@@ -1502,6 +1511,7 @@ struct mcx_batch {
   int elem = MCX_ELEM_SHORT2;
   int maxX = 0;
   bool wide = true;  // every volume suits the 128-bit count kernel
+  int slab = 64;     // planes per column
   int64_t n_tiles = 0;
   int64_t ncols = 0;
   int64_t cubes = 0;
@@ -1527,7 +1537,7 @@ extern "C" int mcx_batch_create(mcx_batch** out, int nvol, const int* dims_xyz, 
     if (nt > 0) {
       // one column per in-plane range, slabs of at most 64 planes
       const int R = vols[b].R;
-      const int slab = 64;
+      const int slab = g_tuning.batch_slab;
       for (int z0 = 0; z0 < Z - 1; z0 += slab)
         for (int r = 0; r < R; ++r) cols.push_back(Column{b, r, z0, std::min(z0 + slab, Z - 1)});
     }
@@ -1538,6 +1548,7 @@ extern "C" int mcx_batch_create(mcx_batch** out, int nvol, const int* dims_xyz, 
   }
   mcx_batch* p = new mcx_batch();
   p->wide = wide;
+  p->slab = g_tuning.batch_slab;
   p->nvol = nvol;
   p->elem = elem;
   p->maxX = maxX;
@@ -1591,7 +1602,7 @@ extern "C" int mcx_extract_batched(const mcx_batch* plan, mcx_vertex_voxel* d_vv
   P.nvol = plan->nvol;
   P.n_tiles = plan->n_tiles;
   P.ncols = plan->ncols;
-  P.slab_max = 64;
+  P.slab_max = plan->slab;
   P.wide = plan->wide;
   P.vv = d_vv;
   P.pos = d_pos;
@@ -1621,4 +1632,33 @@ extern "C" int mcx_extract_batched(const mcx_batch* plan, mcx_vertex_voxel* d_vv
     if (hh.count > cap) return MCX_ERR_CAPACITY;
   }
   return MCX_OK;
+}
+
+// ---- launch-shape tunables (no counterpart in the reference) ------------
+extern "C" int mcx_set_tuning(int key, int value) {
+  switch (key) {
+    case MCX_TUNE_COLUMNS_PER_CU:
+      if (value < 1 || value > 1024) return MCX_ERR_ARG;
+      g_tuning.columns_per_cu = value;
+      return MCX_OK;
+    case MCX_TUNE_EMIT_GROUPS_PER_CU:
+      if (value < 1 || value > 64) return MCX_ERR_ARG;
+      g_tuning.emit_groups_per_cu = value;
+      return MCX_OK;
+    case MCX_TUNE_BATCH_SLAB:
+      if (value < 1 || value > 4096) return MCX_ERR_ARG;
+      g_tuning.batch_slab = value;
+      return MCX_OK;
+    default:
+      return MCX_ERR_ARG;
+  }
+}
+
+extern "C" int mcx_get_tuning(int key) {
+  switch (key) {
+    case MCX_TUNE_COLUMNS_PER_CU: return g_tuning.columns_per_cu;
+    case MCX_TUNE_EMIT_GROUPS_PER_CU: return g_tuning.emit_groups_per_cu;
+    case MCX_TUNE_BATCH_SLAB: return g_tuning.batch_slab;
+    default: return -1;
+  }
 }

@@ -31,14 +31,14 @@ int fill_u32(void* d_ptr, uint32_t value, size_t n_words, hipStream_t s);
 int fill_bytes_zero(void* d_ptr, size_t n_bytes, hipStream_t s);
 
 // ------------------------------------------------------ workspace layout
-// [0, 256)   header: u32 dense_acc, u32 dense_last, u32 parity, u32 error, i64 count
-// [256, ...) u64 descriptors, two arrays of cap_tiles entries (by parity)
+// [0, 256)   header: u32 dense_acc, u32 dense_last, u32 (unused), u32 error, i64 count, ...
+// [256, ...) u64 scan look-back descriptors (zeroed by each call's count kernel)
 constexpr size_t kWsHeader = 256;
 struct WsHeader {
   uint32_t dense_acc;   // tiles of the running call that took the staged path
   uint32_t dense_last;  // the same count for the last completed call
-  uint32_t parity;
-  uint32_t error;
+  uint32_t unused;
+  uint32_t error;       // sticky: a look-back that timed out (until re-init)
   int64_t count;
   uint32_t nz_count;    // non-empty tiles appended by the running count kernel
   uint32_t nz_total;    // the same, frozen by the scan for the emit kernel

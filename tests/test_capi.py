@@ -79,3 +79,20 @@ def test_missing_library_fails_loudly(monkeypatch, tmp_path):
     monkeypatch.setattr(_capi, "LIB_PATH", str(tmp_path / "libmcx.so"))
     with pytest.raises(RuntimeError, match="no CPU fallback"):
         _capi.load()
+
+
+def test_tuning_set_get_and_bounds():
+    lib = _capi.load()
+    keys = (_capi.TUNE_COLUMNS_PER_CU, _capi.TUNE_EMIT_GROUPS_PER_CU, _capi.TUNE_BATCH_SLAB)
+    saved = [lib.mcx_get_tuning(k) for k in keys]
+    assert saved == [64, 8, 64]  # defaults
+    try:
+        assert lib.mcx_set_tuning(_capi.TUNE_COLUMNS_PER_CU, 32) == _capi.OK
+        assert lib.mcx_get_tuning(_capi.TUNE_COLUMNS_PER_CU) == 32
+        assert lib.mcx_set_tuning(_capi.TUNE_COLUMNS_PER_CU, 0) == _capi.ERR_ARG
+        assert lib.mcx_set_tuning(_capi.TUNE_EMIT_GROUPS_PER_CU, 65) == _capi.ERR_ARG
+        assert lib.mcx_set_tuning(99, 1) == _capi.ERR_ARG
+        assert lib.mcx_get_tuning(99) == -1
+    finally:
+        for k, v in zip(keys, saved):
+            lib.mcx_set_tuning(k, v)

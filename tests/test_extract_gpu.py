@@ -152,3 +152,47 @@ def test_normals_bit_exact(dev):
         n_pos = mesh.normals(v, h, torch.from_numpy(ref["pos"]).to(dev))
         assert n_vv.cpu().numpy().tobytes() == nref.tobytes()
         assert n_pos.cpu().numpy().tobytes() == nref.tobytes()
+
+
+def test_launch_shapes_do_not_change_output(dev):
+    """Every tuning candidate gives the same bytes (vertex_voxel, int3 map,
+    ver_voxel); the autotuner returns one of the candidates and applies it."""
+    from marching_cube_amd import autotune
+    print("[case] tunings x sphere 128 + noise", flush=True)
+    vols = [synth.sphere_volume(128)]
+    vols.append((synth.noise_volume((33, 40, 52), seed=5, p_zero_w=0.05), 0.02))
+    saved = autotune.get()
+    try:
+        for f, h in vols:
+            v = torch.from_numpy(f.view(np.int32)).to(dev)
+            ref = O.extract(f, h, want=("vv", "v2vox", "ver_voxel"))
+            for c, g in ((16, 4), (64, 8), (128, 16), (7, 1)):
+                autotune.set(c, g)
+                r = mesh.extract(v, h, want=("vv", "v2vox", "ver_voxel"))
+                assert r.count == ref["n"], (c, g)
+                assert r.vv.cpu().numpy().tobytes() == ref["vv"].view(np.int32).reshape(-1, 7).tobytes()
+                assert r.v2vox.cpu().numpy().tobytes() == ref["v2vox"].tobytes()
+                assert r.ver_voxel.cpu().numpy().tobytes() == ref["ver_voxel"].tobytes()
+        best = autotune.tune_extract(torch.from_numpy(vols[0][0].view(np.int32)).to(dev),
+                                     vols[0][1], columns=(32, 64), emit_groups=(4, 8), reps=2)
+        assert (best["columns_per_cu"], best["emit_groups_per_cu"]) in {(32, 4), (32, 8), (64, 4), (64, 8)}
+        assert autotune.get()["columns_per_cu"] == best["columns_per_cu"]
+    finally:
+        autotune.set(saved["columns_per_cu"], saved["emit_groups_per_cu"], saved["batch_slab"])
+
+
+def test_workspace_shared_across_volume_sizes(dev):
+    """One workspace, volumes of different sizes in turn (the scan's
+    look-back descriptors sit at size-dependent places): every call matches
+    the oracle, including a large volume after smaller ones."""
+    print("[case] one workspace, sizes 300/64/200/300/150", flush=True)
+    ws = mesh.Workspace(dev)
+    for n in (300, 64, 200, 300, 150):
+        f, h = synth.sphere_volume(n, radius=0.3 + 0.01 * (n % 7))
+        v = torch.from_numpy(f.view(np.int32)).to(dev)
+        ref = O.extract(f, h, want=("vv",), nthreads=8)
+        cnt = mesh.extract_into(v, h, cap=0, workspace=ws)
+        assert cnt == ref["n"], n
+        vv = torch.empty((cnt, 7), dtype=torch.int32, device=dev)
+        assert mesh.extract_into(v, h, vv=vv, cap=cnt, workspace=ws) == cnt
+        assert vv.cpu().numpy().tobytes() == ref["vv"].view(np.int32).reshape(-1, 7).tobytes(), n

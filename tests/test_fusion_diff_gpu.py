@@ -83,7 +83,7 @@ def gpu_loss_and_grad(dev, h, trunc, poses, depth, targets, n):
     pos = diff.extract_positions(tsdf, h)
     loss = fusion_diff.chamfer_loss(pos, torch.from_numpy(targets).to(dev))
     loss.backward()
-    return float(loss), d.grad.cpu().numpy().astype(np.float64)
+    return float(loss.detach()), d.grad.cpu().numpy().astype(np.float64)
 
 
 def f64_chain(h, trunc, poses, depth64, targets, n):
