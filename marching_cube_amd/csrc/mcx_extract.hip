@@ -853,45 +853,55 @@ __device__ __forceinline__ void make_vertex(const VolDesc& V, const uint64_t* s_
   if (write_vv && V.ver_voxel && inb) V.ver_voxel[li] = (int16_t)(tw & 0xffffu);
 }
 
-// Emit the vertices of up to 64 cubes held one per lane (record word 0 =
-// offset in tile << 8 | case, corner words cw; nv = 0 for lanes without a
-// cube), in lane order, to output indices out, out + 1, ...  Vertices are
-// staged 64 at a time in the wave's area s_out (64 x 7 words) and written
-// with consecutive lanes on consecutive words.  Returns the number of
-// vertices (wave-uniform).
+// Per-wave LDS areas of the emit kernel (32-bit words).
+constexpr int kWaveRec = 64 * kRecWords;  // 64 records (cube offset << 8 | case, 8 corners)
+constexpr int kWaveOut = 64 * 7;          // 64 staged vertex_voxel records
+constexpr int kWaveMap = 64 * 15 / 4;     // vertex -> record (u8), at most 15 vertices a record
+constexpr int kWaveEx = 64;               // first vertex of each record
+constexpr int kWaveWords = kWaveRec + kWaveOut + kWaveMap + kWaveEx;
+
+// Emit the vertices of the (up to 64) records in the wave's LDS area s_rec
+// (record r at s_rec[9r]; n of them) to output indices out, out + 1, ...:
+// the records' vertex counts are scanned across the wave, every record writes
+// its index into the vertex -> record map, and each lane then produces one
+// vertex of a round of 64, staged in s_out and written with consecutive
+// lanes on consecutive words.  p0 and z place the tile.  Returns the number
+// of vertices (wave-uniform).
 template <int ELEM>
-__device__ __forceinline__ int emit_lanes(const ExtractParams& P, const VolDesc& V,
-                                          const uint64_t* s_case, uint32_t w0,
-                                          const uint32_t cw[8], int nv, int z, int64_t p0,
-                                          int64_t out, uint32_t* s_out) {
+__device__ __forceinline__ int emit_records(const ExtractParams& P, const VolDesc& V,
+                                            const uint64_t* s_case, const uint8_t* s_nv,
+                                            uint32_t* s_w, int n, int z, int64_t p0,
+                                            int64_t out) {
+  uint32_t* s_rec = s_w;
+  uint32_t* s_out = s_w + kWaveRec;
+  uint8_t* s_map = reinterpret_cast<uint8_t*>(s_w + kWaveRec + kWaveOut);
+  uint32_t* s_ex = s_w + kWaveRec + kWaveOut + kWaveMap;
   const int lane = threadIdx.x & 63;
+  const int nv = lane < n ? s_nv[s_rec[lane * kRecWords] & 0xffu] : 0;
   const uint32_t incl = wave_incl_scan32((uint32_t)nv);
   const int ex = (int)incl - nv;
   const int tot = __shfl((int)incl, 63);
+  s_ex[lane] = (uint32_t)ex;
+  for (int q = 0; q < nv; ++q) s_map[ex + q] = (uint8_t)lane;
+  wave_sync();
   for (int vb = 0; vb < tot; vb += 64) {
-    const int j = vb + lane;
-    // the last lane whose first vertex is at or before j (lanes without
-    // vertices share their start with the next lane, so this one holds j)
-    int lo = 0;
-#pragma unroll
-    for (int step = 32; step > 0; step >>= 1) {
-      const int ex_mid = __shfl(ex, lo + step);
-      if (ex_mid <= j) lo += step;
-    }
-    const int q = j - __shfl(ex, lo);
-    const uint32_t rw = __shfl(w0, lo);
-    uint32_t c[8];
-#pragma unroll
-    for (int k = 0; k < 8; ++k) c[k] = __shfl(cw[k], lo);
     const int64_t vbase = out + vb;
-    const int n = min(64, tot - vb);
+    const int nr = min(64, tot - vb);
     const int64_t room = P.cap - vbase;
-    const int nw = room <= 0 ? 0 : (room < n ? (int)room : n);  // vertices below the capacity
+    const int nw = room <= 0 ? 0 : (room < nr ? (int)room : nr);  // vertices below the capacity
     if (lane < nw) {
+      const int j = vb + lane;
+      const int r = s_map[j];
+      const uint32_t* rc = s_rec + r * kRecWords;
+      const uint32_t rw = rc[0];
+      uint32_t c[8];
+#pragma unroll
+      for (int k = 0; k < 8; ++k) c[k] = rc[1 + k];
       const int64_t p = p0 + (rw >> 8);
       const uint32_t y = V.divX.div((uint32_t)p);
       const int x = (int)((uint32_t)p - y * (uint32_t)V.X);
-      make_vertex<ELEM>(V, s_case, c, x, (int)y, z, rw & 0xffu, q, true, s_out + 7 * lane);
+      make_vertex<ELEM>(V, s_case, c, x, (int)y, z, rw & 0xffu, j - (int)s_ex[r], true,
+                        s_out + 7 * lane);
     }
     wave_sync();
     if (P.vv) {
@@ -913,76 +923,110 @@ __device__ __forceinline__ int emit_lanes(const ExtractParams& P, const VolDesc&
 // Wave-level emission: wave w of the grid takes list entries w, w + nwaves,
 // ... (no atomics: one counter shared by every wave serialises).  A tile's
 // vertices are [voff[t], voff[t] + T), in cube order, so the output does not
-// depend on which wave takes which tile.  Records are read 64 at a time with
-// consecutive lanes on consecutive words into the wave's LDS area, then taken
-// one per lane; a tile whose records did not fit the scratch is re-read from
-// the volume, 64 cubes at a time, into the same per-lane form.
+// depend on which wave takes which tile.  The metadata of up to 64 of the
+// wave's tiles is loaded at once (one lane per tile) and broadcast; each
+// tile's first 64 records are fetched (coalesced, into registers) while the
+// previous tile is emitted.  A tile whose records did not fit the scratch is
+// re-read from the volume, 64 cubes at a time, into the same record form.
 template <int ELEM, bool BATCH>
 __global__ __launch_bounds__(kEmitThreads) void emit_kernel(ExtractParams P) {
   __shared__ uint64_t s_case[256];
   __shared__ uint8_t s_nv[256];
-  __shared__ uint32_t s_wrec[kEmitThreads / 64][64 * kRecWords];
-  __shared__ uint32_t s_wout[kEmitThreads / 64][64 * 7];
+  __shared__ uint32_t s_wave[kEmitThreads / 64][kWaveWords];
   s_case[threadIdx.x] = dCase[threadIdx.x];
   s_nv[threadIdx.x] = dNumVerts[threadIdx.x];
   __syncthreads();  // the only workgroup barrier: waves run independently below
   const int lane = threadIdx.x & 63;
   const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
-  uint32_t* const my_rec = s_wrec[wv];
-  uint32_t* const my_out = s_wout[wv];
+  uint32_t* const s_w = s_wave[wv];
   const uint32_t nz = P.hdr->nz_total;
   const uint32_t nwaves = gridDim.x * (kEmitThreads / 64);
   const uint32_t wave0 = blockIdx.x * (kEmitThreads / 64) + (uint32_t)wv;
-  for (uint32_t item = wave0; item < nz; item += nwaves) {
-    const int64_t t = P.nz_list[item];
-    const int nrec = P.tile_nrec[t];
-    int64_t out = P.voff[t];
-    const int vi = BATCH ? volume_of_tile(P, t) : 0;
-    const VolDesc& V = vol_at<BATCH>(P, vi);
-    const int64_t local = t - V.tile_begin;
-    const int z = (int)V.divR.div((uint32_t)local);
-    const int64_t p0 = (local - (int64_t)z * V.R) * kTile;
-    if (nrec >= 0) {
-      const uint32_t* rec = P.rec + P.tile_rec[t] * kRecWords;
-      for (int rb = 0; rb < nrec; rb += 64) {
-        const int n = min(64, nrec - rb);
-        const uint32_t* src = rec + (int64_t)rb * kRecWords;
-        for (int w = lane; w < n * kRecWords; w += 64) my_rec[w] = src[w];
-        wave_sync();
-        uint32_t w0 = 0, cw[8] = {};
-        int nv = 0;
-        if (lane < n) {
-          w0 = my_rec[lane * kRecWords];
+  // this wave's items: wave0 + i * nwaves, in batches of 64
+  for (uint32_t ib = 0; wave0 + ib * nwaves < nz; ib += 64) {
+    const uint32_t my_item = wave0 + (ib + (uint32_t)lane) * nwaves;
+    const bool have = my_item < nz;
+    const uint32_t nb = (uint32_t)__popcll(__ballot(have));
+    int64_t t_l = 0, off_l = 0, rs_l = 0;
+    int nrec_l = 0;
+    if (have) {
+      t_l = P.nz_list[my_item];
+      nrec_l = P.tile_nrec[t_l];
+      off_l = P.voff[t_l];
+      rs_l = P.tile_rec[t_l];
+    }
+    // first record block of tile j, 9 words per lane (coalesced)
+    auto fetch = [&](uint32_t j, uint32_t* pf) {
+      const int nr = __shfl(nrec_l, (int)j);
+      const int64_t rs = __shfl(rs_l, (int)j);
+      const int nw = min(nr, 64) * kRecWords;
+      const uint32_t* src = P.rec + rs * kRecWords;
 #pragma unroll
-          for (int q = 0; q < 8; ++q) cw[q] = my_rec[lane * kRecWords + 1 + q];
-          nv = s_nv[w0 & 0xffu];
-        }
-        wave_sync();
-        out += emit_lanes<ELEM>(P, V, s_case, w0, cw, nv, z, p0, out, my_out);
+      for (int q = 0; q < kRecWords; ++q) {
+        const int w = lane + 64 * q;
+        pf[q] = w < nw ? src[w] : 0u;
       }
-    } else {
-      const int X = V.X;
-      const int64_t XY = (int64_t)X * V.Y;
-      const uint32_t* vol = V.vol + (int64_t)z * XY;
-      for (int cb = 0; cb < kTile; cb += 64) {
-        const int o = cb + lane;
-        const int64_t p = p0 + o;
-        uint32_t w0 = 0, cw[8] = {};
-        int nv = 0;
-        if (p < XY) {
-          const uint32_t y = V.divX.div((uint32_t)p);
-          const uint32_t x = (uint32_t)p - y * (uint32_t)X;
-          if (x + 1 < (uint32_t)X && y + 1 < (uint32_t)V.Y) {
-            const uint32_t* a = vol + p;
-            const uint32_t* b = a + XY;
-            cw[0] = a[0]; cw[1] = a[1]; cw[2] = a[X + 1]; cw[3] = a[X];
-            cw[4] = b[0]; cw[5] = b[1]; cw[6] = b[X + 1]; cw[7] = b[X];
-            const uint32_t ci = cube_index<ELEM>(cw);
-            w0 = ((uint32_t)o << 8) | ci;
-            nv = s_nv[ci];
+    };
+    uint32_t pf[kRecWords];
+    fetch(0, pf);
+    for (uint32_t j = 0; j < nb; ++j) {
+      const int64_t t = __shfl(t_l, (int)j);
+      const int nrec = __shfl(nrec_l, (int)j);
+      int64_t out = __shfl(off_l, (int)j);
+      const int64_t rs = __shfl(rs_l, (int)j);
+      uint32_t cur[kRecWords];
+#pragma unroll
+      for (int q = 0; q < kRecWords; ++q) cur[q] = pf[q];
+      if (j + 1 < nb) fetch(j + 1, pf);  // in flight while tile j is emitted
+      const int vi = BATCH ? volume_of_tile(P, t) : 0;
+      const VolDesc& V = vol_at<BATCH>(P, vi);
+      const int64_t local = t - V.tile_begin;
+      const int z = (int)V.divR.div((uint32_t)local);
+      const int64_t p0 = (local - (int64_t)z * V.R) * kTile;
+      if (nrec >= 0) {
+        for (int rb = 0; rb < nrec; rb += 64) {
+          const int n = min(64, nrec - rb);
+          if (rb == 0) {
+#pragma unroll
+            for (int q = 0; q < kRecWords; ++q) s_w[lane + 64 * q] = cur[q];
+          } else {
+            const uint32_t* src = P.rec + (rs + rb) * kRecWords;
+            for (int w = lane; w < n * kRecWords; w += 64) s_w[w] = src[w];
           }
+          wave_sync();
+          out += emit_records<ELEM>(P, V, s_case, s_nv, s_w, n, z, p0, out);
         }
-        out += emit_lanes<ELEM>(P, V, s_case, w0, cw, nv, z, p0, out, my_out);
+      } else {
+        const int X = V.X;
+        const int64_t XY = (int64_t)X * V.Y;
+        const uint32_t* vol = V.vol + (int64_t)z * XY;
+        for (int cb = 0; cb < kTile; cb += 64) {
+          const int o = cb + lane;
+          const int64_t p = p0 + o;
+          uint32_t cw[8] = {}, ci = 0;
+          if (p < XY) {
+            const uint32_t y = V.divX.div((uint32_t)p);
+            const uint32_t x = (uint32_t)p - y * (uint32_t)X;
+            if (x + 1 < (uint32_t)X && y + 1 < (uint32_t)V.Y) {
+              const uint32_t* a = vol + p;
+              const uint32_t* b = a + XY;
+              cw[0] = a[0]; cw[1] = a[1]; cw[2] = a[X + 1]; cw[3] = a[X];
+              cw[4] = b[0]; cw[5] = b[1]; cw[6] = b[X + 1]; cw[7] = b[X];
+              ci = cube_index<ELEM>(cw);
+            }
+          }
+          // compact the cubes with vertices into the record area, in cube order
+          const uint64_t act = __ballot(s_nv[ci] != 0);
+          const int n = __popcll(act);
+          if ((act >> lane) & 1ull) {
+            uint32_t* rp = s_w + __popcll(act & ((1ull << lane) - 1ull)) * kRecWords;
+            rp[0] = ((uint32_t)o << 8) | ci;
+#pragma unroll
+            for (int q = 0; q < 8; ++q) rp[1 + q] = cw[q];
+          }
+          wave_sync();
+          if (n) out += emit_records<ELEM>(P, V, s_case, s_nv, s_w, n, z, p0, out);
+        }
       }
     }
   }
