@@ -84,6 +84,15 @@ int mcx_extract(const void* d_vol, int elem, int X, int Y, int Z, float h,
                 int16_t* d_ver_voxel, int64_t cap, int64_t* d_count,
                 int64_t* h_count, void* d_ws, size_t ws_bytes, void* stream);
 
+/* mcx_extract with one more optional output: d_edge_key (int64 per vertex) =
+ * the grid edge the vertex lies on, as (lower corner voxel linear id) * 3 +
+ * axis (0 x, 1 y, 2 z); the up to 4 cubes around an edge give it the same
+ * key (input of mcx_weld). */
+int mcx_extract_ex(const void* d_vol, int elem, int X, int Y, int Z, float h,
+                   mcx_vertex_voxel* d_vv, float* d_pos, int32_t* d_v2vox,
+                   int16_t* d_ver_voxel, int64_t* d_edge_key, int64_t cap, int64_t* d_count,
+                   int64_t* h_count, void* d_ws, size_t ws_bytes, void* stream);
+
 /* Per-cube classification (the per-cube part of OccupiedVoxels,
  * cu_marching_cubes_v1.cu:120-131): d_ci / d_nv (X*Y*Z bytes each, indexed by
  * the cube's min-corner voxel; 0 for cubes that do not exist). */
@@ -109,6 +118,21 @@ size_t mcx_batch_workspace_size(const mcx_batch* plan, int64_t records);
 int mcx_extract_batched(const mcx_batch* plan, mcx_vertex_voxel* d_vv, float* d_pos,
                         int32_t* d_v2vox, int64_t cap, int64_t* d_vbase, int64_t* d_count,
                         int64_t* h_count, void* d_ws, size_t ws_bytes, void* stream);
+
+/* ---- indexed mesh (replaces the OBJ writer's de-duplication by printed
+ *      coordinates, cu_kinect_fusion_v1.cu:3668-3719, with exact sharing by
+ *      grid edge).  Input: the V vertices of an extraction (positions with a
+ *      stride of stride_floats floats: 7 for vertex_voxel, 3 for float3) and
+ *      their edge keys (mcx_extract_ex).  Output: the unique vertices in order
+ *      of first occurrence (d_upos, float3; each takes the position of its
+ *      first occurrence), d_indices (int32 per input vertex: its unique
+ *      vertex; consecutive triples are the triangles) and the unique count
+ *      (d_nunique device int64 and/or h_nunique host, which synchronises).
+ *      Deterministic.  V < 2^31.  Workspace: mcx_weld_workspace_size(V). */
+size_t mcx_weld_workspace_size(int64_t V);
+int mcx_weld(const int64_t* d_edge_key, const float* d_pos, int64_t stride_floats, int64_t V,
+             float* d_upos, int32_t* d_indices, int64_t* d_nunique, int64_t* h_nunique,
+             void* d_ws, size_t ws_bytes, void* stream);
 
 /* ---- per-vertex normals (replaces d_voxel_extract_surface_normals,
  *      cu_kinect_fusion_v1.cu:3146-3238).  d_pos: n positions with a stride

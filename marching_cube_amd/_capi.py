@@ -58,6 +58,10 @@ SIGNATURES = {
                                               _vp, _sz, _vp]),
     "mcx_volume_init": (_i32, [_vp, _i32, _i32, _i32, _vp]),
     "mcx_set_tuning": (_i32, [_i32, _i32]),
+    "mcx_extract_ex": (_i32, [_vp, _i32, _i32, _i32, _i32, _f32, _vp, _vp, _vp, _vp, _vp, _i64,
+                              _vp, _vp, _vp, _sz, _vp]),
+    "mcx_weld_workspace_size": (_sz, [_i64]),
+    "mcx_weld": (_i32, [_vp, _vp, _i64, _i64, _vp, _vp, _vp, _vp, _vp, _sz, _vp]),
     "mcx_get_tuning": (_i32, [_i32]),
     "mcx_tsdf_init": (_i32, [_vp, _vp, _i64, _f32, _vp]),
     "mcx_integrate_tsdf": (_i32, [_vp, _vp, _i32, _i32, _i32, _f32, _f32, _i32, _vp, _vp, _vp,

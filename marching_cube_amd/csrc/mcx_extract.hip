@@ -116,6 +116,7 @@ struct ExtractParams {
   mcx_vertex_voxel* vv;
   float* pos;
   int32_t* v2vox;
+  int64_t* ekey;         // optional: grid edge of each vertex (min-corner voxel * 3 + axis)
   int64_t cap;
   int64_t* d_count;
   int64_t* vbase;
@@ -808,12 +809,22 @@ static_assert(sizeof(TileInfo) == 40, "tile info is 10 words");
 template <int ELEM>
 __device__ __forceinline__ void make_vertex(const VolDesc& V, const uint64_t* s_case,
                                             const uint32_t cw[8], int x, int y, int z,
-                                            uint32_t ci, int q, bool write_vv, uint32_t* o) {
+                                            uint32_t ci, int q, bool write_vv, uint32_t* o,
+                                            int64_t* key = nullptr) {
   // output slot q = 3*tri + r holds table corner 2 - r of triangle tri (:324-332)
   const int m = 3 * (q / 3) + 2 - (q % 3);
   const int e = (int)((s_case[ci] >> (4 * m)) & 0xfu);
   const int ca = (int)((kEdgeA >> (3 * e)) & 7u);
   const int cb = (int)((kEdgeB >> (3 * e)) & 7u);
+  if (key) {
+    // the grid edge: its lower corner and its axis (the two corners differ
+    // along exactly one axis); shared by the up to 4 cubes around the edge
+    const int ax = corner_dx(ca) != corner_dx(cb) ? 0 : (corner_dy(ca) != corner_dy(cb) ? 1 : 2);
+    const int mx = x + min(corner_dx(ca), corner_dx(cb));
+    const int my = y + min(corner_dy(ca), corner_dy(cb));
+    const int mz = z + min(corner_dz(ca), corner_dz(cb));
+    *key = (((int64_t)mz * V.Y + my) * V.X + mx) * 3 + ax;
+  }
   const float h = V.h;
   const float fa = unpack_f<ELEM>(pick8(cw, ca));
   const float fb = unpack_f<ELEM>(pick8(cw, cb));
@@ -900,8 +911,10 @@ __device__ __forceinline__ int emit_records(const ExtractParams& P, const VolDes
       const int64_t p = p0 + (rw >> 8);
       const uint32_t y = V.divX.div((uint32_t)p);
       const int x = (int)((uint32_t)p - y * (uint32_t)V.X);
+      int64_t key;
       make_vertex<ELEM>(V, s_case, c, x, (int)y, z, rw & 0xffu, j - (int)s_ex[r], true,
-                        s_out + 7 * lane);
+                        s_out + 7 * lane, P.ekey ? &key : nullptr);
+      if (P.ekey) P.ekey[vbase + lane] = key;
     }
     wave_sync();
     if (P.vv) {
@@ -1383,6 +1396,15 @@ extern "C" int mcx_extract(const void* d_vol, int elem, int X, int Y, int Z, flo
                            mcx_vertex_voxel* d_vv, float* d_pos, int32_t* d_v2vox,
                            int16_t* d_ver_voxel, int64_t cap, int64_t* d_count,
                            int64_t* h_count, void* d_ws, size_t ws_bytes, void* stream) {
+  return mcx_extract_ex(d_vol, elem, X, Y, Z, h, d_vv, d_pos, d_v2vox, d_ver_voxel, nullptr, cap,
+                        d_count, h_count, d_ws, ws_bytes, stream);
+}
+
+extern "C" int mcx_extract_ex(const void* d_vol, int elem, int X, int Y, int Z, float h,
+                              mcx_vertex_voxel* d_vv, float* d_pos, int32_t* d_v2vox,
+                              int16_t* d_ver_voxel, int64_t* d_edge_key, int64_t cap,
+                              int64_t* d_count, int64_t* h_count, void* d_ws, size_t ws_bytes,
+                              void* stream) {
   if (!d_vol || X < 1 || Y < 1 || Z < 1 || cap < 0) return MCX_ERR_ARG;
   if (elem != MCX_ELEM_SHORT2 && elem != MCX_ELEM_F32) return MCX_ERR_ARG;
   if (elem == MCX_ELEM_F32 && (d_vv || d_ver_voxel)) return MCX_ERR_ARG;
@@ -1412,6 +1434,7 @@ extern "C" int mcx_extract(const void* d_vol, int elem, int X, int Y, int Z, flo
   P.vv = d_vv;
   P.pos = d_pos;
   P.v2vox = d_v2vox;
+  P.ekey = d_edge_key;
   P.cap = cap;
   P.d_count = d_count;
   P.vbase = nullptr;

@@ -153,6 +153,33 @@ def extract(vol, h, *, want=("vv",), cap=None, workspace=None):
     return r
 
 
+def indexed_mesh(vol, h, workspace=None):
+    """Indexed mesh of the iso-0 surface: (vertices (U, 3) float32, faces
+    (F, 3) int32).  Vertices are shared exactly by grid edge (each the first
+    occurrence in the canonical extraction order), faces keep the canonical
+    triangle order and winding."""
+    elem, X, Y, Z = volume_kind(vol)
+    lib = _capi.load()
+    ws = workspace or default_workspace(vol.device)
+    ws.ensure(lib.mcx_extract_workspace_size(X, Y, Z))
+    n = extract_into(vol, h, cap=0, workspace=ws)
+    dev = vol.device
+    pos = torch.empty((max(n, 1), 3), dtype=torch.float32, device=dev)
+    key = torch.empty(max(n, 1), dtype=torch.int64, device=dev)
+    hc = ctypes.c_int64(0)
+    _capi.check(lib.mcx_extract_ex(_ptr(vol), elem, X, Y, Z, float(h), None, _ptr(pos), None,
+                                   None, _ptr(key), n, None, ctypes.byref(hc), ws.ptr,
+                                   ws.nbytes, _stream_handle(dev)), "mcx_extract_ex")
+    upos = torch.empty((max(n, 1), 3), dtype=torch.float32, device=dev)
+    idx = torch.empty(max(n, 1), dtype=torch.int32, device=dev)
+    wws = torch.empty(max(4096, lib.mcx_weld_workspace_size(n)), dtype=torch.uint8, device=dev)
+    nu = ctypes.c_int64(0)
+    _capi.check(lib.mcx_weld(_ptr(key), _ptr(pos), 3, n, _ptr(upos), _ptr(idx), None,
+                             ctypes.byref(nu), _ptr(wws), wws.numel(), _stream_handle(dev)),
+                "mcx_weld")
+    return upos[: nu.value], idx[:n].view(-1, 3)
+
+
 def classify(vol):
     """Per-cube (cube index, numVerts) as uint8 (Z, Y, X) tensors."""
     elem, X, Y, Z = volume_kind(vol)
