@@ -43,6 +43,8 @@ def parse():
                     help="volumes of the batched leg (0 = skip)")
     ap.add_argument("--graph-count", type=int, default=256,
                     help="volumes of the graph-captured differentiable fusion leg (0 = skip)")
+    ap.add_argument("--raycast-views", type=int, default=8,
+                    help="camera poses of the raycast leg on the fused volume (0 = skip)")
     ap.add_argument("--no-tune", action="store_true",
                     help="skip the launch-shape autotuner (default shapes)")
     ap.add_argument("--calib-copy", type=int, default=0,
@@ -181,6 +183,43 @@ def bench_batch(torch, dev, count, steps, warmup):
             "speedup": round(p_ms / b_ms, 2),
             "batched_gvoxel_per_s": round(nvox / (b_ms * 1e-3) / 1e9, 1),
             "batched_mtri_per_s": round(n / 3 / (b_ms * 1e-3) / 1e6, 1)}
+
+
+def bench_raycast(torch, dev, vol, h, cfg, views, steps, warmup, cpu_threads, no_cpu):
+    """Depth + normal maps (640x480, the frames' camera) ray-marched through
+    the fused volume from `views` poses of the orbit, one launch per view,
+    timed with HIP events; the CPU oracle renders one view for the baseline."""
+    import numpy as np
+    from marching_cube_amd import fusion, synth
+    W, H = cfg["W"], cfg["H"]
+    intr = [cfg["fx"], cfg["fy"], cfg["cx"], cfg["cy"]]
+    poses = synth.orbit_poses(views, cfg["extent"], seed=cfg["seed"] + 1)
+    depth = torch.empty((H, W), dtype=torch.float32, device=dev)
+    normals = torch.empty((H, W, 3), dtype=torch.float32, device=dev)
+
+    def cast(i):
+        fusion.raycast(vol, intr, poses[i % views], W, H, h=h, depth=depth, normals=normals)
+
+    avg, med = _event_ms(torch, cast, max(steps, views), warmup)
+    hits = 0
+    for k in range(views):
+        cast(k)
+        hits += int((depth > 0).sum().item())
+    out = {"workload": f"raycast{W}x{H}_on_fused_volume", "views": views,
+           "ms_per_view": round(avg, 4), "ms_per_view_median": round(med, 4),
+           "mrays_per_s": round(W * H / (avg * 1e-3) / 1e6, 1),
+           "hit_fraction": round(hits / (views * W * H), 3), "cpu_baseline": None}
+    if not no_cpu:
+        from oracle import oracle as O
+        host = vol.cpu().numpy().view(np.uint32)
+        nt = max(1, min(cpu_threads, os.cpu_count() or 1))
+        O.raycast(host, h, intr, poses[0], W, H, origin=0.5, nthreads=nt)
+        tc = time.perf_counter()
+        O.raycast(host, h, intr, poses[0], W, H, origin=0.5, nthreads=nt)
+        tcpu = time.perf_counter() - tc
+        out["cpu_baseline"] = {"ms_per_view": round(tcpu * 1e3, 1), "kind": "port",
+                               "cores": nt, "sample": "one 640x480 view (view 0), after 1 warm-up"}
+    return out
 
 
 def bench_fusion_graph(torch, dev, nvol, steps):
@@ -341,6 +380,12 @@ def main():
     if graph_leg:
         print(f"[bench] fusion graph {graph_leg}", file=sys.stderr, flush=True)
 
+    ray_leg = (bench_raycast(torch, dev, vols[0], h, cfg, args.raycast_views, args.steps,
+                             args.warmup, args.cpu_threads, args.no_cpu)
+               if args.raycast_views else None)
+    if ray_leg:
+        print(f"[bench] raycast {ray_leg}", file=sys.stderr, flush=True)
+
     # ---- CPU baseline: the oracle's extraction on the host cores
     cpu = None
     if not args.no_cpu:
@@ -404,6 +449,7 @@ def main():
         "fwd_bwd": diff_leg,
         "batched": batch_leg,
         "fusion_graph": graph_leg,
+        "raycast": ray_leg,
         "tuning": tuning,
     }
     print(json.dumps(out), flush=True)

@@ -225,6 +225,25 @@ int mcx_integrate_tsdf_backward(const float* d_grad_tsdf, const int32_t* d_weigh
                                 float depth_scale, int W, int H, int K, float* d_grad_depth,
                                 void* d_ws, size_t ws_bytes, void* stream);
 
+/* ---- raycast: depth and normal maps from a packed short2 volume
+ *      (replaces _cu_voxel_extract_depth_and_normal_map,
+ *      cu_kinect_fusion_v1.cu:972-986, and its kernels :551-970).
+ * Voxel i sits at world (i + origin) * h per axis: origin 0 is the
+ * reference's convention, 0.5 that of mcx_integrate's voxel centres.
+ * intr (host): {fx, fy, cx, cy}; pose (host): 12 floats, camera-to-world
+ * {R row-major, t} in metres.  Ray march as the reference: step 1 voxel,
+ * 0.25 inside the band |sdf| < 1, trilinear samples (+-1 at a corner holding
+ * +-32766), linear zero-crossing refinement, back faces end a ray.
+ * Outputs (device, row-major by pixel): d_depth[H*W] z-depth in metres,
+ * 0 where no surface is hit; d_normals[H*W*3] (may be NULL) camera-frame
+ * unit normals, 0 where no surface is hit or a corner is +-32766.  Colour is
+ * not produced.  MCX_ERR_ARG when K R^T is singular (|det| < FLT_EPSILON;
+ * the reference skips the launch).  Every ray ends: a march stops once it
+ * has left the volume for good, and after 2^20 steps. */
+int mcx_raycast(const void* d_vol, int X, int Y, int Z, float h, float origin,
+                const float* intr, const float* pose, int W, int H, float* d_depth,
+                float* d_normals, void* stream);
+
 /* ---- synthetic depth frames (test/benchmark input; no reference
  *      counterpart: the reference reads OpenNI frames).  Ray-casts nobj
  *      objects (8 floats each: kind 0 sphere {c, r} / 1 box {c, half sizes})

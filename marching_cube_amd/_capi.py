@@ -75,6 +75,8 @@ SIGNATURES = {
                              _i32, _i32, _vp, _sz, _vp]),
     "mcx_integrate_f32": (_i32, [_vp, _i32, _i32, _i32, _f32, _f32, _i32, _vp, _vp, _vp, _f32,
                                  _i32, _i32, _i32, _vp, _sz, _vp]),
+    "mcx_raycast": (_i32, [_vp, _i32, _i32, _i32, _f32, _f32, _vp, _vp, _i32, _i32, _vp, _vp,
+                           _vp]),
     "mcx_synth_depth": (_i32, [_vp, _i32, _i32, _i32, _vp, _vp, _vp, _i32, _f32, _f32, _u32,
                                _f32, _vp]),
 }

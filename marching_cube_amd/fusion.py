@@ -5,6 +5,8 @@ memory, calling libmcx through the C ABI.
       -> voxel_new(size, trunc, grid_s)   allocates and fills (-32766, 0)
   _cu_voxel_volumetric_integration(v, ci, ce, size, depth, ...)   :534-549
       -> integrate(volume, intr, poses, depth_frames)  K frames in one sweep
+  _cu_voxel_extract_depth_and_normal_map(v, ci, ce, size, d, n, c)   :972-986
+      -> raycast(volume, intr, pose, W, H)  depth + normal maps (no colour)
 Camera model (cu_geometry_functions.h:96-129): intrinsics {fx, fy, cx, cy};
 poses camera-to-world {R row-major, t}; q = R^T (p - t).
 """
@@ -64,6 +66,33 @@ def integrate(volume, intr, poses, depth, *, trunc, max_w=128, depth_scale=0.001
                 "mcx_integrate")
     del hi
     return volume
+
+
+def raycast(volume, intr, pose, W, H, *, h=None, origin=0.5, depth=None, normals=None):
+    """Ray-march a depth map (H, W) float32 (metres, 0 = no hit) and
+    camera-frame normals (H, W, 3) from a packed short2 volume seen from
+    `pose` (12 floats camera-to-world {R row-major, t}).  origin: voxel i at
+    world (i + origin) h -- 0.5 for volumes fused by `integrate`, 0 for the
+    reference's own convention.  Raises on a singular camera matrix."""
+    if isinstance(volume, CuVoxel):
+        vol, h = volume.data, volume.grid_s if h is None else h
+    else:
+        vol = volume
+    if h is None:
+        raise ValueError("grid size h is required")
+    Z, Y, X = vol.shape[:3]
+    dev = vol.device
+    if depth is None:
+        depth = torch.empty((H, W), dtype=torch.float32, device=dev)
+    if normals is None:
+        normals = torch.empty((H, W, 3), dtype=torch.float32, device=dev)
+    hi, hp = _host_f32(intr)
+    pi, pp = _host_f32(torch.as_tensor(pose, dtype=torch.float32).reshape(12))
+    _capi.check(_lib().mcx_raycast(_ptr(vol), X, Y, Z, float(h), float(origin), hp, pp, W, H,
+                                   _ptr(depth), _ptr(normals), _stream_handle(dev)),
+                "mcx_raycast")
+    del hi, pi
+    return depth, normals
 
 
 def synth_depth(intr, poses, objs, W, H, *, sigma=0.0015, dropout=0.02, seed=20261202,

@@ -93,6 +93,12 @@ void orc_integrate(uint32_t* vol, int X, int Y, int Z, float h, float trunc,
                    const uint16_t* depth, float depth_scale, int W, int H,
                    int nframes);
 
+/* Ray-marched depth and normal maps (see raycast_oracle.c). */
+int orc_raycast_setup(const float* intr, const float* pose, float* im);
+int orc_raycast(const uint32_t* vol, int X, int Y, int Z, float h, float origin,
+                const float* intr, const float* pose, int W, int H, float* depth,
+                float* normals, int nthreads);
+
 #ifdef __cplusplus
 }
 #endif

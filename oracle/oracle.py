@@ -55,6 +55,10 @@ def lib():
         L.orc_integrate.restype = None
         L.orc_integrate.argtypes = [vp, i32, i32, i32, f32, f32, i32, vp, vp, vp, f32,
                                     i32, i32, i32]
+        L.orc_raycast_setup.restype = i32
+        L.orc_raycast_setup.argtypes = [vp, vp, vp]
+        L.orc_raycast.restype = i32
+        L.orc_raycast.argtypes = [vp, i32, i32, i32, f32, f32, vp, vp, i32, i32, vp, vp, i32]
         for name in ("orc_tables_tri", "orc_tables_num_verts", "orc_tables_edge"):
             getattr(L, name).restype = ctypes.POINTER(ctypes.c_int)
             getattr(L, name).argtypes = []
@@ -207,3 +211,28 @@ def integrate_tsdf_f64(tsdf, w, h, trunc, max_w, intr, poses, depth, depth_scale
     lib().orc_integrate_tsdf_f64(_ptr(tsdf), _ptr(w), X, Y, Z, h, trunc, max_w, _ptr(intr),
                                  _ptr(poses), _ptr(depth), depth_scale, W, H, K)
     return tsdf, w
+
+
+def raycast(vol, h, intr, pose, W, H, origin=0.0, nthreads=8):
+    """Depth (H,W) and camera-frame normals (H,W,3) ray-marched through a
+    packed short2 volume (Z,Y,X uint32); None when K R^T is singular."""
+    vol = np.ascontiguousarray(vol).view(np.uint32)
+    Z, Y, X = vol.shape[:3]
+    intr = np.ascontiguousarray(intr, np.float32)
+    pose = np.ascontiguousarray(pose, np.float32).reshape(12)
+    depth = np.zeros((H, W), np.float32)
+    normals = np.zeros((H, W, 3), np.float32)
+    if lib().orc_raycast(_ptr(vol), X, Y, Z, h, origin, _ptr(intr), _ptr(pose), W, H,
+                         _ptr(depth), _ptr(normals), nthreads):
+        return None
+    return depth, normals
+
+
+def raycast_setup(intr, pose):
+    """IM = (K R^T)^-1 as the raycast computes it (row-major 3x3), or None."""
+    im = np.zeros(9, np.float32)
+    if lib().orc_raycast_setup(_ptr(np.ascontiguousarray(intr, np.float32)),
+                               _ptr(np.ascontiguousarray(pose, np.float32).reshape(12)),
+                               _ptr(im)):
+        return None
+    return im.reshape(3, 3)

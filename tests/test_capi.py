@@ -64,6 +64,17 @@ def test_argument_errors_without_device():
                                            1.0, W, H, 4, fake, fake, 16, None) \
         == _capi.ERR_WORKSPACE
 
+    # raycast: null pointers, a too-small volume, h <= 0, a singular camera
+    pose = (ctypes.c_float * 12)(1, 0, 0, 0, 1, 0, 0, 0, 1, 0.1, 0.1, 0.0)
+    assert lib.mcx_raycast(None, 8, 8, 8, 0.1, 0.0, intr, pose, W, H, fake, None,
+                           None) == _capi.ERR_ARG
+    assert lib.mcx_raycast(fake, 1, 8, 8, 0.1, 0.0, intr, pose, W, H, fake, None,
+                           None) == _capi.ERR_ARG
+    assert lib.mcx_raycast(fake, 8, 8, 8, 0.0, 0.0, intr, pose, W, H, fake, None,
+                           None) == _capi.ERR_ARG
+    sing = (ctypes.c_float * 4)(0.0, 60.0, 31.5, 23.5)
+    assert lib.mcx_raycast(fake, 8, 8, 8, 0.1, 0.0, sing, pose, W, H, fake, None,
+                           None) == _capi.ERR_ARG
 
 def test_workspace_size_grows_with_volume():
     lib = _capi.load()
