@@ -31,6 +31,7 @@ extern "C" {
 #define MCX_ERR_HIP 4         /* a HIP runtime call failed */
 #define MCX_ERR_INTERNAL 5    /* device-side consistency check tripped */
 #define MCX_ERR_UNSUPPORTED 6 /* combination not supported (e.g. too wide) */
+#define MCX_ERR_FORMAT 7      /* malformed input data (e.g. a .vol dump) */
 
 /* ---- voxel kinds ------------------------------------------------------- */
 /* cu_voxel_t / short2 {int16 tsdf; uint16 w}, [z][y][x], x fastest
@@ -243,6 +244,38 @@ int mcx_integrate_tsdf_backward(const float* d_grad_tsdf, const int32_t* d_weigh
 int mcx_raycast(const void* d_vol, int X, int Y, int Z, float h, float origin,
                 const float* intr, const float* pose, int W, int H, float* d_depth,
                 float* d_normals, void* stream);
+
+/* ---- `.vol` run-length codec (replaces the slice codec of
+ *      cu_voxel_run_length_encode / _decode, cu_kinect_fusion_v1.cu:1362-1600,
+ *      for the 4-byte voxel {int16 tsdf; uint16 w}).
+ * A dump is, per z slice, a uint32 byte count then the slice's records:
+ * u16 {n : 14 (low bits), v : 2 (high bits)}, v = 1 a run of (+32766, w 0),
+ * v = 2 a run of (-32766, w 0), v = 3 a run of the 4-byte voxel that follows
+ * (v = 0 decodes as v = 3).  Runs are maximal stretches of equal tsdf in
+ * [y][x] order (weights not compared; a run keeps its first voxel), at most
+ * 0x3fff voxels.  The file adds a 40-byte header in front: "VOLUME_TSDF_RGB\0",
+ * int32 size[3], float min_t, max_t, grid_s (:1303-1310).
+ * mcx_rle_encode: the dump of slices [z0, z0+nz) of d_vol (X, Y, Z) into
+ *   d_dump (2-byte aligned, dump_cap bytes; mcx_rle_encode_bound(X, Y, nz)
+ *   always suffices); the size goes to *d_dump_bytes and *h_dump_bytes (each
+ *   may be NULL).  MCX_ERR_CAPACITY (nothing written) if it exceeds dump_cap.
+ *   Workspace: mcx_rle_encode_workspace_size(X, Y, nz) bytes (16 per voxel:
+ *   encode large volumes in slabs).  Synchronises the stream.
+ * mcx_rle_decode: nz slices into d_vol[nz][Y][X]; slice j's uint32 byte
+ *   count sits at byte h_slice_offsets[j] (host array) of d_dump (2-byte
+ *   aligned, dump_bytes).  MCX_ERR_FORMAT for a malformed slice (offset
+ *   outside the dump, odd or zero size, a run of length 0, runs past the
+ *   slice, a truncated record, bytes left over); d_vol is then undefined.
+ *   Workspace: mcx_rle_decode_workspace_size(dump_bytes, X, Y, nz).
+ *   Synchronises the stream. */
+size_t mcx_rle_encode_bound(int X, int Y, int nz);
+size_t mcx_rle_encode_workspace_size(int X, int Y, int nz);
+int mcx_rle_encode(const void* d_vol, int X, int Y, int Z, int z0, int nz, void* d_dump,
+                   size_t dump_cap, int64_t* d_dump_bytes, int64_t* h_dump_bytes, void* d_ws,
+                   size_t ws_bytes, void* stream);
+size_t mcx_rle_decode_workspace_size(int64_t dump_bytes, int X, int Y, int nz);
+int mcx_rle_decode(const void* d_dump, int64_t dump_bytes, const int64_t* h_slice_offsets, int nz,
+                   int X, int Y, void* d_vol, void* d_ws, size_t ws_bytes, void* stream);
 
 /* ---- synthetic depth frames (test/benchmark input; no reference
  *      counterpart: the reference reads OpenNI frames).  Ray-casts nobj

@@ -18,6 +18,7 @@ ERR_WORKSPACE = 3
 ERR_HIP = 4
 ERR_INTERNAL = 5
 ERR_UNSUPPORTED = 6
+ERR_FORMAT = 7
 
 ELEM_SHORT2 = 0
 ELEM_F32 = 1
@@ -77,6 +78,12 @@ SIGNATURES = {
                                  _i32, _i32, _i32, _vp, _sz, _vp]),
     "mcx_raycast": (_i32, [_vp, _i32, _i32, _i32, _f32, _f32, _vp, _vp, _i32, _i32, _vp, _vp,
                            _vp]),
+    "mcx_rle_encode_bound": (_sz, [_i32, _i32, _i32]),
+    "mcx_rle_encode_workspace_size": (_sz, [_i32, _i32, _i32]),
+    "mcx_rle_encode": (_i32, [_vp, _i32, _i32, _i32, _i32, _i32, _vp, _sz, _vp, _vp, _vp, _sz,
+                              _vp]),
+    "mcx_rle_decode_workspace_size": (_sz, [_i64, _i32, _i32, _i32]),
+    "mcx_rle_decode": (_i32, [_vp, _i64, _vp, _i32, _i32, _i32, _vp, _vp, _sz, _vp]),
     "mcx_synth_depth": (_i32, [_vp, _i32, _i32, _i32, _vp, _vp, _vp, _i32, _f32, _f32, _u32,
                                _f32, _vp]),
 }

@@ -88,6 +88,7 @@ extern "C" const char* mcx_strerror(int status) {
     case MCX_ERR_HIP: return t_last_hip != hipSuccess ? hipGetErrorString(t_last_hip) : "HIP error";
     case MCX_ERR_INTERNAL: return "device-side consistency check failed";
     case MCX_ERR_UNSUPPORTED: return "unsupported size or combination";
+    case MCX_ERR_FORMAT: return "malformed input data";
     default: return "unknown status";
   }
 }

@@ -99,6 +99,13 @@ int orc_raycast(const uint32_t* vol, int X, int Y, int Z, float h, float origin,
                 const float* intr, const float* pose, int W, int H, float* depth,
                 float* normals, int nthreads);
 
+/* Run-length `.vol` codec (see rle_oracle.c). */
+int64_t orc_rle_encode_slice(const uint32_t* vox, int64_t n, uint8_t* out);
+int orc_rle_decode_slice(const uint8_t* in, int64_t nbytes, uint32_t* vox, int64_t n);
+int64_t orc_rle_encode_dump(const uint32_t* vol, int X, int Y, int z0, int nz, uint8_t* out);
+int orc_rle_decode_dump(const uint8_t* dump, int64_t nbytes, int X, int Y, int z0, int nz,
+                        uint32_t* out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -59,6 +59,14 @@ def lib():
         L.orc_raycast_setup.argtypes = [vp, vp, vp]
         L.orc_raycast.restype = i32
         L.orc_raycast.argtypes = [vp, i32, i32, i32, f32, f32, vp, vp, i32, i32, vp, vp, i32]
+        L.orc_rle_encode_slice.restype = i64
+        L.orc_rle_encode_slice.argtypes = [vp, i64, vp]
+        L.orc_rle_decode_slice.restype = i32
+        L.orc_rle_decode_slice.argtypes = [vp, i64, vp, i64]
+        L.orc_rle_encode_dump.restype = i64
+        L.orc_rle_encode_dump.argtypes = [vp, i32, i32, i32, i32, vp]
+        L.orc_rle_decode_dump.restype = i32
+        L.orc_rle_decode_dump.argtypes = [vp, i64, i32, i32, i32, i32, vp]
         for name in ("orc_tables_tri", "orc_tables_num_verts", "orc_tables_edge"):
             getattr(L, name).restype = ctypes.POINTER(ctypes.c_int)
             getattr(L, name).argtypes = []
@@ -236,3 +244,39 @@ def raycast_setup(intr, pose):
                                _ptr(im)):
         return None
     return im.reshape(3, 3)
+
+
+def rle_encode_slice(vox):
+    """Reference run-length bytes of one slice (uint32 voxels, any shape)."""
+    vox = np.ascontiguousarray(vox).view(np.uint32).reshape(-1)
+    out = np.zeros(6 * max(1, vox.size), np.uint8)
+    n = lib().orc_rle_encode_slice(_ptr(vox), vox.size, _ptr(out))
+    return out[:n].tobytes()
+
+
+def rle_decode_slice(data, n):
+    """Voxels (uint32, n) of one slice, or None for a malformed slice."""
+    buf = np.frombuffer(bytes(data), np.uint8).copy()
+    out = np.zeros(n, np.uint32)
+    if lib().orc_rle_decode_slice(_ptr(buf), buf.size, _ptr(out), n):
+        return None
+    return out
+
+
+def rle_encode_dump(vol, z0=0, nz=None):
+    """Dump (per slice: uint32 byte count, records) of slices z0.. of vol (Z,Y,X)."""
+    vol = np.ascontiguousarray(vol).view(np.uint32)
+    Z, Y, X = vol.shape
+    nz = Z - z0 if nz is None else nz
+    out = np.zeros(nz * (4 + 6 * X * Y), np.uint8)
+    n = lib().orc_rle_encode_dump(_ptr(vol), X, Y, z0, nz, _ptr(out))
+    return out[:n].tobytes()
+
+
+def rle_decode_dump(dump, X, Y, z0, nz):
+    """Slices z0..z0+nz-1 (nz,Y,X uint32) of a dump, or None if malformed."""
+    buf = np.frombuffer(bytes(dump), np.uint8).copy()
+    out = np.zeros((nz, Y, X), np.uint32)
+    if lib().orc_rle_decode_dump(_ptr(buf), buf.size, X, Y, z0, nz, _ptr(out)):
+        return None
+    return out

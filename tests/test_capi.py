@@ -30,7 +30,7 @@ def test_exported_symbols_are_c_linkage():
 def test_version_and_strerror():
     lib = _capi.load()
     assert lib.mcx_version() >= 10000
-    for st in range(7):
+    for st in range(8):
         assert lib.mcx_strerror(st)
     assert b"capacity" in lib.mcx_strerror(_capi.ERR_CAPACITY)
 
