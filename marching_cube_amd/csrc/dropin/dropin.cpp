@@ -4,6 +4,8 @@
 //   MarchingCubes::run            cu_marching_cubes_v1.cu:417-467
 //   cu_voxel_new / cu_voxel_delete       cu_kinect_fusion_v1.cu:381-412
 //   _cu_voxel_volumetric_integration     cu_kinect_fusion_v1.cu:534-549
+//   _cu_voxel_extract_depth_and_normal_map   cu_kinect_fusion_v1.cu:972-986
+//   cu_voxel_run_length_*            cu_kinect_fusion_v1.cu:1303-1600
 //   cu_voxel_to_mesh              cu_kinect_fusion_v1.cu:3511-3737
 //   cu_voxel_rlc_to_mesh          cu_kinect_fusion_v1.cu:3261-3435
 //   MyMarchingCube                MyMarchingCube.h:4-10 (declared only there)
@@ -219,6 +221,220 @@ void _cu_voxel_volumetric_integration(cu_voxel v, float_camera_intrinsic ci,
                               size.y, 1, ws.ptr(), ws.size(), nullptr),
             "mcx_integrate_f32");
   mcx_detail::check_hip(hipDeviceSynchronize(), "_cu_voxel_volumetric_integration");
+}
+
+// ---------------------------------------------------------------- raycast
+void _cu_voxel_extract_depth_and_normal_map(cu_voxel v, float_camera_intrinsic ci,
+                                            float_camera_extrinsic ce, int2 size, float* d_ptr,
+                                            float3* n_ptr, uchar3* c_ptr) {
+  const float intr[4] = {ci.fc.x, ci.fc.y, ci.cc.x, ci.cc.y};
+  const float3* c = ce.R.p;
+  const float pose[12] = {c[0].x, c[1].x, c[2].x, c[0].y, c[1].y, c[2].y,
+                          c[0].z, c[1].z, c[2].z, ce.t.x, ce.t.y, ce.t.z};
+  const int st = mcx_raycast(v.data, v.size.x, v.size.y, v.size.z, v.grid_s, 0.5f, intr, pose,
+                             size.x, size.y, d_ptr, reinterpret_cast<float*>(n_ptr), nullptr);
+  if (st == MCX_ERR_ARG) return;  // e.g. a singular camera: the reference launches nothing
+  check_mcx(st, "mcx_raycast");
+  if (c_ptr)
+    mcx_detail::check_hip(hipMemset(c_ptr, 0xff, (size_t)size.x * size.y * sizeof(uchar3)),
+                          "_cu_voxel_extract_depth_and_normal_map");
+  mcx_detail::check_hip(hipDeviceSynchronize(), "_cu_voxel_extract_depth_and_normal_map");
+}
+
+// ---------------------------------------------------------------- .vol files
+namespace {
+
+const char kVolumeFormat[16] = "VOLUME_TSDF_RGB";
+
+// Dump (per slice: uint32 byte count, records) of slices [z0, z0 + nz) of a
+// device volume, encoded in slabs of at most 2^24 voxels.
+std::vector<char> rle_encode(const cu_voxel_t* d_vol, int X, int Y, int Z, int z0, int nz) {
+  std::vector<char> out;
+  const long long XY = (long long)X * Y;
+  const int per = (int)std::max(1LL, std::min<long long>(nz, (1LL << 24) / XY));
+  cu_array<unsigned char> ws(mcx_rle_encode_workspace_size(X, Y, per));
+  cu_array<unsigned char> buf(mcx_rle_encode_bound(X, Y, per));
+  for (int z = z0; z < z0 + nz; z += per) {
+    const int k = std::min(per, z0 + nz - z);
+    int64_t total = 0;
+    check_mcx(mcx_rle_encode(d_vol, X, Y, Z, z, k, buf.ptr(), buf.size(), nullptr, &total,
+                             ws.ptr(), ws.size(), nullptr),
+              "mcx_rle_encode");
+    const size_t at = out.size();
+    out.resize(at + (size_t)total);
+    mcx_detail::check_hip(hipMemcpy(out.data() + at, buf.ptr(), (size_t)total,
+                                    hipMemcpyDeviceToHost),
+                          "cu_voxel_run_length_encode");
+  }
+  return out;
+}
+
+// Byte offsets of the nz slice counts of a dump; false if it is too short.
+bool slice_offsets(const std::vector<char>& dump, int nz, std::vector<int64_t>& offs) {
+  offs.resize((size_t)nz + 1);
+  int64_t off = 0;
+  for (int z = 0; z < nz; ++z) {
+    if (off + 4 > (int64_t)dump.size()) return false;
+    offs[z] = off;
+    uint32_t b;
+    std::memcpy(&b, dump.data() + off, 4);
+    off += 4 + (int64_t)b;
+  }
+  offs[nz] = off;
+  return off <= (int64_t)dump.size();
+}
+
+// Decodes slices [z0, z0 + nz) of a host dump into a device [nz][Y][X] buffer.
+void rle_decode(const std::vector<char>& dump, const std::vector<int64_t>& offs, int X, int Y,
+                int z0, int nz, cu_voxel_t* d_out) {
+  cu_array<char> d(dump.size());
+  d.upload(dump.data(), dump.size());
+  cu_array<unsigned char> ws(mcx_rle_decode_workspace_size((int64_t)dump.size(), X, Y, nz));
+  check_mcx(mcx_rle_decode(d.ptr(), (int64_t)dump.size(), offs.data() + z0, nz, X, Y, d_out,
+                           ws.ptr(), ws.size(), nullptr),
+            "mcx_rle_decode");
+}
+
+bool read_file(const char* path, cu_voxel* header, std::vector<char>& dump) {
+  FILE* fp = std::fopen(path, "rb");
+  if (!fp) return false;
+  cu_voxel_run_length_read_header(header, fp);
+  std::vector<char> rest;
+  char chunk[1 << 16];
+  size_t got;
+  while ((got = std::fread(chunk, 1, sizeof(chunk), fp)) > 0) rest.insert(rest.end(), chunk, chunk + got);
+  std::fclose(fp);
+  dump.swap(rest);
+  return true;
+}
+
+bool region_ok(const cu_voxel& local, int3 bl, const cu_voxel& global, int3 bg, int3 sz) {
+  const int l[3] = {local.size.x, local.size.y, local.size.z};
+  const int g[3] = {global.size.x, global.size.y, global.size.z};
+  const int b1[3] = {bl.x, bl.y, bl.z}, b2[3] = {bg.x, bg.y, bg.z}, s3[3] = {sz.x, sz.y, sz.z};
+  for (int a = 0; a < 3; ++a)
+    if (b1[a] < 0 || b2[a] < 0 || s3[a] < 0 || b1[a] + s3[a] > l[a] || b2[a] + s3[a] > g[a])
+      return false;
+  return local.grid_s == global.grid_s;
+}
+
+// Copies a size.x x size.y x size.z box between two [z][y][x] device volumes.
+void copy_box(cu_voxel_t* dst, int dX, int dY, int3 dbeg, const cu_voxel_t* src, int sX, int sY,
+              int3 sbeg, int3 size) {
+  for (int z = 0; z < size.z; ++z) {
+    cu_voxel_t* d = dst + ((size_t)(z + dbeg.z) * dY + dbeg.y) * dX + dbeg.x;
+    const cu_voxel_t* s = src + ((size_t)(z + sbeg.z) * sY + sbeg.y) * sX + sbeg.x;
+    mcx_detail::check_hip(hipMemcpy2D(d, (size_t)dX * sizeof(cu_voxel_t), s,
+                                      (size_t)sX * sizeof(cu_voxel_t),
+                                      (size_t)size.x * sizeof(cu_voxel_t), (size_t)size.y,
+                                      hipMemcpyDeviceToDevice),
+                          "copy_box");
+  }
+}
+
+}  // namespace
+
+void cu_voxel_run_length_write_header(cu_voxel* v, FILE* fp) {
+  std::fwrite(kVolumeFormat, sizeof(char), 16, fp);
+  std::fwrite(&v->size, sizeof(int), 3, fp);
+  std::fwrite(&v->min_t, sizeof(float), 1, fp);
+  std::fwrite(&v->max_t, sizeof(float), 1, fp);
+  std::fwrite(&v->grid_s, sizeof(float), 1, fp);
+}
+
+void cu_voxel_run_length_read_header(cu_voxel* v, FILE* fp) {
+  char format[16] = {0};
+  if (std::fread(format, sizeof(char), 16, fp) == 16 &&
+      std::strncmp(format, kVolumeFormat, 16) == 0 && std::fread(&v->size, sizeof(int), 3, fp) == 3 &&
+      std::fread(&v->min_t, sizeof(float), 1, fp) == 1 &&
+      std::fread(&v->max_t, sizeof(float), 1, fp) == 1 &&
+      std::fread(&v->grid_s, sizeof(float), 1, fp) == 1) {
+    v->size_xy = v->size.x * v->size.y;
+    return;
+  }
+  std::printf("cu_voxel_run_length_decode_header - invalid format.\n");
+  v->size = make_int3(0, 0, 0);
+  v->size_xy = 0;
+}
+
+void cu_voxel_run_length_encode(cu_voxel* v_local, char* file_name, int3 begin_local,
+                                int3 begin_global, int3 size) {
+  cu_voxel g;
+  std::vector<char> dump;
+  if (!read_file(file_name, &g, dump)) {
+    std::printf("cu_voxel_run_length_encode - saving...\n");
+    const std::vector<char> d = rle_encode(v_local->data, v_local->size.x, v_local->size.y,
+                                           v_local->size.z, 0, v_local->size.z);
+    FILE* fp = std::fopen(file_name, "wb");
+    if (!fp) {
+      std::printf("cu_voxel_run_length_encode - fail to open %s\n", file_name);
+      return;
+    }
+    cu_voxel_run_length_write_header(v_local, fp);
+    std::fwrite(d.data(), 1, d.size(), fp);
+    std::fclose(fp);
+    return;
+  }
+  std::vector<int64_t> offs;
+  if (g.size_xy == 0 || !region_ok(*v_local, begin_local, g, begin_global, size) ||
+      !slice_offsets(dump, g.size.z, offs)) {
+    std::printf("cu_voxel_run_length_encode - out of bound.\n");
+    return;
+  }
+  std::printf("cu_voxel_run_length_encode - updating...\n");
+  if (size.x > 0 && size.y > 0 && size.z > 0) {
+    const int X = g.size.x, Y = g.size.y, gz = begin_global.z, nz = size.z;
+    cu_array<cu_voxel_t> part((size_t)X * Y * nz);
+    rle_decode(dump, offs, X, Y, gz, nz, part.ptr());
+    copy_box(part.ptr(), X, Y, make_int3(begin_global.x, begin_global.y, 0), v_local->data,
+             v_local->size.x, v_local->size.y, begin_local, size);
+    const std::vector<char> mid = rle_encode(part.ptr(), X, Y, nz, 0, nz);
+    std::vector<char> nd(dump.begin(), dump.begin() + offs[gz]);
+    nd.insert(nd.end(), mid.begin(), mid.end());
+    nd.insert(nd.end(), dump.begin() + offs[gz + nz], dump.end());
+    dump.swap(nd);
+  }
+  FILE* fp = std::fopen(file_name, "wb");
+  if (!fp) return;
+  cu_voxel_run_length_write_header(&g, fp);
+  std::fwrite(dump.data(), 1, dump.size(), fp);
+  std::fclose(fp);
+}
+
+void cu_voxel_run_length_decode(cu_voxel* v_local, char* file_name, int3 begin_local,
+                                int3 begin_global, int3 size) {
+  cu_voxel g;
+  std::vector<char> dump;
+  if (!read_file(file_name, &g, dump)) {
+    std::printf("cu_voxel_run_length_decode - fail to open %s file.\n", file_name);
+    return;
+  }
+  std::vector<int64_t> offs;
+  if (g.size_xy == 0 || !slice_offsets(dump, g.size.z, offs)) {
+    std::printf("cu_voxel_run_length_decode - invalid file.\n");
+    return;
+  }
+  if (v_local->data == nullptr) {
+    *v_local = g;
+    v_local->data = nullptr;
+    mcx_detail::check_hip(
+        hipMalloc(&v_local->data, sizeof(cu_voxel_t) * (size_t)g.size_xy * g.size.z),
+        "cu_voxel_run_length_decode");
+    rle_decode(dump, offs, g.size.x, g.size.y, 0, g.size.z, v_local->data);
+    return;
+  }
+  if (!region_ok(*v_local, begin_local, g, begin_global, size)) {
+    std::printf("cu_voxel_run_length_decode - out of bound.\n");
+    return;
+  }
+  if (size.x > 0 && size.y > 0 && size.z > 0) {
+    const int X = g.size.x, Y = g.size.y;
+    cu_array<cu_voxel_t> part((size_t)X * Y * size.z);
+    rle_decode(dump, offs, X, Y, begin_global.z, size.z, part.ptr());
+    copy_box(v_local->data, v_local->size.x, v_local->size.y, begin_local, part.ptr(), X, Y,
+             make_int3(begin_global.x, begin_global.y, 0), size);
+  }
+  mcx_detail::check_hip(hipDeviceSynchronize(), "cu_voxel_run_length_decode");
 }
 
 // ---------------------------------------------------------------- mesh files

@@ -1,6 +1,7 @@
 // Drives the C++ drop-in API (MarchingCubes, cu_voxel_to_mesh,
 // cu_voxel_rlc_to_mesh, MyMarchingCube, cu_voxel_new +
-// _cu_voxel_volumetric_integration) on inputs written by
+// _cu_voxel_volumetric_integration, _cu_voxel_extract_depth_and_normal_map,
+// cu_voxel_run_length_encode / _decode) on inputs written by
 // tests/test_dropin_gpu.py, which checks the files this program writes.
 //   test_dropin <volume.bin> X Y Z h <outdir> <rlc.txt> <rlc_dim>
 #include <hip/hip_runtime_api.h>
@@ -93,7 +94,52 @@ int main(int argc, char** argv) {
   std::vector<cu_voxel_t> fused(16 * 16 * 16);
   if (hipMemcpy(fused.data(), tv.data, fused.size() * sizeof(cu_voxel_t), hipMemcpyDeviceToHost) != hipSuccess) return 3;
   dump(out + "/fused.bin", fused.data(), fused.size() * sizeof(cu_voxel_t));
+
+  // 6. ray-cast the fused volume from the same camera
+  cu_array<float> rdepth(32 * 24);
+  cu_array<float3> rnormal(32 * 24);
+  cu_array<uchar3> rcolor(32 * 24);
+  _cu_voxel_extract_depth_and_normal_map(tv, ci, ce, make_int2(32, 24), rdepth.ptr(),
+                                         rnormal.ptr(), rcolor.ptr());
+  std::vector<float> hd(32 * 24);
+  std::vector<float3> hn(32 * 24);
+  std::vector<uchar3> hc(32 * 24);
+  rdepth.download(hd.data());
+  rnormal.download(hn.data());
+  rcolor.download(hc.data());
+  dump(out + "/ray_depth.bin", hd.data(), hd.size() * sizeof(float));
+  dump(out + "/ray_normal.bin", hn.data(), hn.size() * sizeof(float3));
+  dump(out + "/ray_color.bin", hc.data(), hc.size() * sizeof(uchar3));
   cu_voxel_delete(&tv);
+
+  // 7. .vol files: the sphere volume whole, loaded back, a region replaced
+  // by the original voxels, and a region loaded into a zeroed volume
+  const std::string vf = out + "/sphere.vol";
+  vol.min_t = 0.03f;
+  vol.max_t = 0.04f;
+  cu_voxel_run_length_encode(&vol, const_cast<char*>(vf.c_str()), make_int3(0, 0, 0),
+                             make_int3(0, 0, 0), make_int3(0, 0, 0));
+  cu_voxel lv;
+  lv.data = nullptr;
+  cu_voxel_run_length_decode(&lv, const_cast<char*>(vf.c_str()), make_int3(0, 0, 0),
+                             make_int3(0, 0, 0), make_int3(0, 0, 0));
+  std::printf("vol_size %dx%dx%d\n", lv.size.x, lv.size.y, lv.size.z);
+  std::vector<cu_voxel_t> hl(n);
+  if (hipMemcpy(hl.data(), lv.data, n * sizeof(cu_voxel_t), hipMemcpyDeviceToHost) != hipSuccess) return 3;
+  dump(out + "/vol_loaded.bin", hl.data(), n * sizeof(cu_voxel_t));
+  cu_voxel_delete(&lv);
+  cu_voxel_run_length_encode(&vol, const_cast<char*>(vf.c_str()), make_int3(3, 5, 7),
+                             make_int3(20, 10, 30), make_int3(31, 22, 17));
+  cu_voxel rv;
+  rv.data = nullptr;
+  cu_voxel_new(&rv, make_int3(40, 30, 20), 0.0f, 0.0f, h);
+  if (hipMemset(rv.data, 0, (size_t)40 * 30 * 20 * sizeof(cu_voxel_t)) != hipSuccess) return 3;
+  cu_voxel_run_length_decode(&rv, const_cast<char*>(vf.c_str()), make_int3(1, 2, 3),
+                             make_int3(10, 20, 30), make_int3(35, 25, 15));
+  std::vector<cu_voxel_t> hr(40 * 30 * 20);
+  if (hipMemcpy(hr.data(), rv.data, hr.size() * sizeof(cu_voxel_t), hipMemcpyDeviceToHost) != hipSuccess) return 3;
+  dump(out + "/vol_region.bin", hr.data(), hr.size() * sizeof(cu_voxel_t));
+  cu_voxel_delete(&rv);
   std::printf("done\n");
   return 0;
 }

@@ -1,7 +1,8 @@
 """The C++ drop-in API (include/marching_cube_amd/*.h over libmcx) driven by
 tests/cpp/test_dropin.cpp: MarchingCubes::run, cu_voxel_to_mesh (PLY, OBJ),
-MyMarchingCube, cu_voxel_rlc_to_mesh and cu_voxel_new +
-_cu_voxel_volumetric_integration, each checked against the oracle."""
+MyMarchingCube, cu_voxel_rlc_to_mesh, cu_voxel_new +
+_cu_voxel_volumetric_integration, _cu_voxel_extract_depth_and_normal_map and
+cu_voxel_run_length_encode / _decode, each checked against the oracle."""
 import os
 import re
 import struct
@@ -95,3 +96,30 @@ def test_dropin_api(dev, tmp_path):
     fused = np.fromfile(tmp_path / "fused.bin", dtype=np.uint32).reshape(16, 16, 16)
     assert (synth.unpack_short2(ref_vol)[1] > 0).any()
     assert fused.tobytes() == ref_vol.tobytes()
+
+    # raycast of the fused volume from the integrating camera
+    intr = np.array([40, 40, 15.5, 11.5], np.float32)
+    rd, rn = O.raycast(ref_vol, 0.1, intr, pose, 32, 24, origin=0.5)
+    gd = np.fromfile(tmp_path / "ray_depth.bin", dtype=np.float32).reshape(24, 32)
+    gn = np.fromfile(tmp_path / "ray_normal.bin", dtype=np.float32).reshape(24, 32, 3)
+    assert gd.tobytes() == rd.tobytes() and gn.tobytes() == rn.tobytes()
+    assert (gd > 0).mean() > 0.5
+    assert (np.fromfile(tmp_path / "ray_color.bin", dtype=np.uint8) == 255).all()
+
+    # .vol files
+    assert out["vol_size"] == "64x64x64"
+    canon = O.rle_decode_dump(O.rle_encode_dump(vol), 64, 64, 0, 64)
+    loaded = np.fromfile(tmp_path / "vol_loaded.bin", dtype=np.uint32).reshape(64, 64, 64)
+    assert (loaded == canon).all()
+    E = canon.copy()
+    E[30:47, 10:32, 20:51] = vol.view(np.uint32)[7:24, 5:27, 3:34]
+    raw = open(tmp_path / "sphere.vol", "rb").read()
+    assert raw[:16] == b"VOLUME_TSDF_RGB\x00"
+    assert struct.unpack_from("<3i3f", raw, 16)[:3] == (64, 64, 64)
+    assert np.float32(struct.unpack_from("<3f", raw, 28)[1]) == np.float32(0.04)
+    assert raw[40:] == O.rle_encode_dump(E)
+    E = O.rle_decode_dump(raw[40:], 64, 64, 0, 64)
+    reg = np.fromfile(tmp_path / "vol_region.bin", dtype=np.uint32).reshape(20, 30, 40)
+    expect = np.zeros_like(reg)
+    expect[3:18, 2:27, 1:36] = E[30:45, 20:45, 10:45]
+    assert (reg == expect).all()
