@@ -43,6 +43,8 @@ def parse():
                     help="volumes of the batched leg (0 = skip)")
     ap.add_argument("--graph-count", type=int, default=256,
                     help="volumes of the graph-captured differentiable fusion leg (0 = skip)")
+    ap.add_argument("--no-mesh-io", dest="mesh_io", action="store_false",
+                    help="skip the indexed-mesh and .vol codec leg")
     ap.add_argument("--raycast-views", type=int, default=8,
                     help="camera poses of the raycast leg on the fused volume (0 = skip)")
     ap.add_argument("--no-tune", action="store_true",
@@ -222,6 +224,86 @@ def bench_raycast(torch, dev, vol, h, cfg, views, steps, warmup, cpu_threads, no
     return out
 
 
+def bench_mesh_io(torch, dev, vol, h, steps):
+    """On the fused volume: indexed mesh (extraction with edge keys + weld),
+    and the .vol codec (device volume -> device dump -> device volume)."""
+    import ctypes
+    import numpy as np
+    from marching_cube_amd import _capi, mesh, volio
+    lib = _capi.load()
+    elem, X, Y, Z = mesh.volume_kind(vol)
+    ws = mesh.Workspace(dev)
+    n = mesh.extract_into(vol, h, cap=0, workspace=ws)
+    pos = torch.empty((n, 3), dtype=torch.float32, device=dev)
+    key = torch.empty(n, dtype=torch.int64, device=dev)
+    upos = torch.empty((n, 3), dtype=torch.float32, device=dev)
+    idx = torch.empty(n, dtype=torch.int32, device=dev)
+    wws = torch.empty(lib.mcx_weld_workspace_size(n), dtype=torch.uint8, device=dev)
+    st = mesh._stream_handle(dev)
+
+    def extract_keys(i):
+        _capi.check(lib.mcx_extract_ex(mesh._ptr(vol), elem, X, Y, Z, float(h), None,
+                                       mesh._ptr(pos), None, None, mesh._ptr(key), n, None, None,
+                                       ws.ptr, ws.nbytes, st), "mcx_extract_ex")
+
+    def weld(i):
+        _capi.check(lib.mcx_weld(mesh._ptr(key), mesh._ptr(pos), 3, n, mesh._ptr(upos),
+                                 mesh._ptr(idx), None, None, mesh._ptr(wws), wws.numel(), st),
+                    "mcx_weld")
+
+    ek_ms, _ = _event_ms(torch, extract_keys, steps, 2)
+    weld_ms, _ = _event_ms(torch, weld, steps, 2)
+    nu = ctypes.c_int64(0)
+    _capi.check(lib.mcx_weld(mesh._ptr(key), mesh._ptr(pos), 3, n, mesh._ptr(upos),
+                             mesh._ptr(idx), None, ctypes.byref(nu), mesh._ptr(wws), wws.numel(),
+                             st), "mcx_weld")
+    # codec: whole volume in one call each way (the calls synchronise)
+    ews = torch.empty(lib.mcx_rle_encode_workspace_size(X, Y, Z), dtype=torch.uint8, device=dev)
+    dump = torch.empty(lib.mcx_rle_encode_bound(X, Y, Z), dtype=torch.uint8, device=dev)
+    total = ctypes.c_int64(0)
+
+    def encode():
+        _capi.check(lib.mcx_rle_encode(mesh._ptr(vol), X, Y, Z, 0, Z, mesh._ptr(dump),
+                                       dump.numel(), None, ctypes.byref(total), mesh._ptr(ews),
+                                       ews.numel(), st), "mcx_rle_encode")
+
+    encode()
+    reps = max(3, steps // 4)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        encode()
+    enc_ms = (time.perf_counter() - t0) * 1e3 / reps
+    del ews
+    nb = total.value
+    d = dump[:nb]
+    offs = volio.slice_offsets(d.cpu().numpy().tobytes(), Z)
+    out = torch.empty_like(vol)
+    dws = torch.empty(lib.mcx_rle_decode_workspace_size(nb, X, Y, Z), dtype=torch.uint8,
+                      device=dev)
+
+    def decode():
+        _capi.check(lib.mcx_rle_decode(mesh._ptr(d), nb, offs.ctypes.data, Z, X, Y,
+                                       mesh._ptr(out), mesh._ptr(dws), dws.numel(), st),
+                    "mcx_rle_decode")
+
+    decode()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        decode()
+    dec_ms = (time.perf_counter() - t0) * 1e3 / reps
+    # the round trip keeps every tsdf (weights of saturated voxels become 0)
+    same_t = bool(((out & 0xFFFF) == (vol & 0xFFFF)).all().item())
+    res = {"vertices": n, "unique_vertices": nu.value,
+           "extract_with_keys_ms": round(ek_ms, 4), "weld_ms": round(weld_ms, 4),
+           "vol_dump_bytes": nb, "vol_ratio": round(4 * X * Y * Z / nb, 2),
+           "encode_ms": round(enc_ms, 3), "decode_ms": round(dec_ms, 3),
+           "encode_gvoxel_per_s": round(X * Y * Z / (enc_ms * 1e-3) / 1e9, 2),
+           "decode_gvoxel_per_s": round(X * Y * Z / (dec_ms * 1e-3) / 1e9, 2),
+           "round_trip_tsdf_exact": same_t}
+    del dump, dws, out
+    return res
+
+
 def bench_fusion_graph(torch, dev, nvol, steps):
     """256 volumes with sides uniform in [64, 192], each fused from 16
     synthetic 320x240 uint16 depth frames (3 random spheres/boxes in front of
@@ -380,6 +462,9 @@ def main():
     if graph_leg:
         print(f"[bench] fusion graph {graph_leg}", file=sys.stderr, flush=True)
 
+    io_leg = bench_mesh_io(torch, dev, vols[0], h, args.steps) if args.mesh_io else None
+    if io_leg:
+        print(f"[bench] mesh io {io_leg}", file=sys.stderr, flush=True)
     ray_leg = (bench_raycast(torch, dev, vols[0], h, cfg, args.raycast_views, args.steps,
                              args.warmup, args.cpu_threads, args.no_cpu)
                if args.raycast_views else None)
@@ -450,6 +535,7 @@ def main():
         "batched": batch_leg,
         "fusion_graph": graph_leg,
         "raycast": ray_leg,
+        "mesh_io": io_leg,
         "tuning": tuning,
     }
     print(json.dumps(out), flush=True)
