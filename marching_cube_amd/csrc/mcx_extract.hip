@@ -135,10 +135,16 @@ struct ExtractParams {
   int64_t rec_cap;
 };
 
+// tsdf / 32766 correctly rounded for every int16 tsdf: a multiply by the
+// rounded reciprocal and one fused correction step (equal to IEEE division
+// over [-32768, 32767], checked exhaustively).
 template <int ELEM>
 __device__ __forceinline__ float unpack_f(uint32_t v) {
   if constexpr (ELEM == MCX_ELEM_SHORT2) {
-    return (float)(int16_t)(v & 0xffffu) / 32766.0f;
+    constexpr float r = 1.0f / 32766.0f;
+    const float x = (float)(int16_t)(v & 0xffffu);
+    const float q = x * r;
+    return __fmaf_rn(__fmaf_rn(-q, 32766.0f, x), r, q);
   } else {
     return __uint_as_float(v);
   }
@@ -454,11 +460,38 @@ constexpr int kDppWaveShl1 = 0x130;
 struct Plane4 {
   u32x4 u[2], w[2];  // rows y (u) and y + 1 (w) of chunks 0 and 1
   uint32_t xu, xw;   // word 512 of the two rows (wave-uniform; neighbour of chunk 1's lane 63)
+  // per chunk, byte k = 0xff / 0x00: the tsdf at position k of row u / w
+  // negative (set once per plane, used by two steps)
+  uint32_t su[2], sw[2];
 };
+
+// v_perm_b32 selectors: byte 8 / 10 = the sign of bits 15 / 47 of {src0, src1}
+// (src1 = the lower word) as 0xff or 0x00, byte 12 = 0x00.
+constexpr uint32_t kSignSel01 = 0x0C0C0A08u;  // bytes 0, 1 <- signs of src1, src0
+constexpr uint32_t kSignSel23 = 0x0A080C0Cu;  // bytes 2, 3 <- signs of src1, src0
+
+// Bits of m from a, the others from b (v_bfi_b32).
+__device__ __forceinline__ uint32_t bfi(uint32_t m, uint32_t a, uint32_t b) {
+  return (m & a) | (~m & b);
+}
+
+// Byte k = 0xff when the tsdf of voxel position k is negative.
+__device__ __forceinline__ uint32_t sign_bytes(const u32x4& v) {
+  return __builtin_amdgcn_perm(v[1], v[0], kSignSel01) | __builtin_amdgcn_perm(v[3], v[2], kSignSel23);
+}
+
+__device__ __forceinline__ void plane_signs(Plane4& L) {
+#pragma unroll
+  for (int c = 0; c < 2; ++c) {
+    L.su[c] = sign_bytes(L.u[c]);
+    L.sw[c] = sign_bytes(L.w[c]);
+  }
+}
 typedef const __attribute__((address_space(4))) uint32_t const_u32;  // scalar loads
 
 template <bool BATCH>
-__global__ __launch_bounds__(256) void count4_kernel(ExtractParams P) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4)))
+void count4_kernel(ExtractParams P) {
   __shared__ uint8_t s_nv[256];
   __shared__ uint32_t s_rec[4][256 * kRecWords];  // per wave: one chunk's records
   s_nv[threadIdx.x] = dNumVerts[threadIdx.x];
@@ -508,93 +541,143 @@ __global__ __launch_bounds__(256) void count4_kernel(ExtractParams P) {
     lane_ok[c] = ok ? 0xffffffffu : 0u;
     lane_mid[c] = end ? 0u : 0xffffffffu;
   }
+  // byte k of cube_mask[c]: cube k of the lane exists (lane_ok; cube 3 also
+  // needs the next position in the row)
+  uint32_t cube_mask[2];
+#pragma unroll
+  for (int c = 0; c < 2; ++c) cube_mask[c] = lane_ok[c] & (lane_mid[c] | 0x00ffffffu);
 
-  // Offsets in the VGPR offset (range-checked against the plane size; a
-  // 16-byte load is either wholly inside the plane or wholly outside).
+  // One buffer resource over the planes the column reads (z0 .. z1); the
+  // plane offset is part of the VGPR offset, which the hardware range-checks
+  // (the SGPR offset is not checked): planes past z1 read as zero.  Voxels of
+  // a tile past the end of its plane read the next plane's words, which only
+  // lanes without cubes see (they are masked).  The host keeps
+  // (z1 - z0 + 4) planes under 2^31 bytes.
   const uint32_t plane_bytes = (uint32_t)(XY * 4);
+  const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
+      const_cast<uint32_t*>(V.vol + (int64_t)z0 * XY), (short)0,
+      (int)((uint32_t)(z1 - z0 + 1) * plane_bytes), kBufferWord3);
   const uint32_t voff = (uint32_t)(p0 + 4 * lane) * 4u;
   const uint32_t voffW = voff + 4u * (uint32_t)X;
-  // the word after the tile in each row (wave-uniform scalar loads; 0 past the plane)
-  const bool has_xu = p0 + kTile < XY, has_xw = p0 + kTile + X < XY;
-  auto load_plane = [&](int q, bool need, Plane4& L) {
-    const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
-        const_cast<uint32_t*>(V.vol + (need ? (int64_t)q * XY : 0)), (short)0,
-        need ? (int)plane_bytes : 0, kBufferWord3);
-    L.u[0] = __builtin_amdgcn_raw_buffer_load_b128(rs, voff, 0, 0);
-    L.u[1] = __builtin_amdgcn_raw_buffer_load_b128(rs, voff + 1024u, 0, 0);
-    L.w[0] = __builtin_amdgcn_raw_buffer_load_b128(rs, voffW, 0, 0);
-    L.w[1] = __builtin_amdgcn_raw_buffer_load_b128(rs, voffW + 1024u, 0, 0);
-    const_u32* row = (const_u32*)(V.vol) + (int64_t)q * XY + p0 + kTile;
-    L.xu = need && has_xu ? row[0] : 0u;
-    L.xw = need && has_xw ? row[X] : 0u;
+  // the word after the tile in each row (the same for all lanes; an offset
+  // past the resource, hence 0, when it lies outside the plane)
+  const uint32_t xoffU = p0 + kTile < XY ? (uint32_t)(p0 + kTile) * 4u : 0x80000000u;
+  const uint32_t xoffW = p0 + kTile + X < XY ? (uint32_t)(p0 + kTile + X) * 4u : 0x80000000u;
+  auto load_plane = [&](int q, Plane4& L) {
+    const uint32_t po = (uint32_t)(q - z0) * plane_bytes;
+    L.u[0] = __builtin_amdgcn_raw_buffer_load_b128(rs, voff + po, 0, 0);
+    L.u[1] = __builtin_amdgcn_raw_buffer_load_b128(rs, voff + po + 1024u, 0, 0);
+    L.w[0] = __builtin_amdgcn_raw_buffer_load_b128(rs, voffW + po, 0, 0);
+    L.w[1] = __builtin_amdgcn_raw_buffer_load_b128(rs, voffW + po + 1024u, 0, 0);
+    L.xu = __builtin_amdgcn_readfirstlane(__builtin_amdgcn_raw_buffer_load_b32(rs, xoffU + po, 0, 0));
+    L.xw = __builtin_amdgcn_readfirstlane(__builtin_amdgcn_raw_buffer_load_b32(rs, xoffW + po, 0, 0));
   };
   // Four plane buffers whose roles rotate over an unrolled loop of four steps
   // (no register copies, which would wait for in-flight loads): the step for
   // cubes between planes z and z + 1 loads plane z + 3.
   Plane4 A, B, C, D;
-  load_plane(z0, true, A);
-  load_plane(z0 + 1, true, B);
-  load_plane(z0 + 2, z0 + 1 < z1, C);
+  load_plane(z0, A);
+  plane_signs(A);
+  load_plane(z0 + 1, B);
+  load_plane(z0 + 2, C);
 
-  int64_t used = 0;
+  int32_t used = 0;  // records of the column's quota taken
   uint64_t nzmask = 0;  // planes zb + i of this column whose tile has vertices
   int zb = z0;
+  const uint32_t quota = (uint32_t)P.quota;
+  uint32_t t = (uint32_t)(V.tile_begin + (int64_t)z0 * V.R + r);  // this step's tile
+  const uint32_t tR = (uint32_t)V.R;
   auto step = [&](Plane4& A, Plane4& B, Plane4& L, int z) __attribute__((always_inline)) {
-    load_plane(z + 3, z + 2 < z1, L);
-    const int64_t t = V.tile_begin + (int64_t)z * V.R + r;
+    load_plane(z + 3, L);
 
-    // ---- pass 1: active masks m[c][k] (bit L: cube 256c + 4L + k).  Per lane
-    // and voxel position, over the four corner rows: OR and AND of the words
-    // (bit 15: some / every tsdf negative) and the minimum word (below 0x10000:
-    // some weight zero; forced to 0 on lanes without cubes).  A cube combines
-    // positions k and k + 1; position 4 is the next lane's position 0 (a DPP
-    // lane shift; lane 63 takes the next chunk's lane 0, or the words after
-    // the tile).
+    // ---- pass 1: active masks m[c][k] (bit L: cube 256c + 4L + k).  Signs
+    // first, four cubes per lane at once: byte k of O / N = some / every tsdf
+    // of the four corner rows at position k negative (the planes' row pairs
+    // combined); the bytes of positions k + 1 come from shifting O / N down
+    // one byte and taking the next lane's byte 0 (a DPP lane shift; lane 63
+    // takes the next chunk's lane 0, or the words after the tile).  A cube
+    // has both signs where OR & ~AND; lanes without cubes and position 4 at a
+    // row end are masked.  Only chunks with such cubes test the weights:
+    // the minimum word over the eight corners below 0x10000 means some
+    // weight is zero.
+    plane_signs(B);
     uint64_t m[2][4];
     uint32_t nxo, nxn, nxm;  // position 0 of the next chunk's lane 0 (wave-uniform)
     {
-      nxo = A.xu | A.xw | B.xu | B.xw;
-      nxn = A.xu & A.xw & B.xu & B.xw;
+      const uint32_t xo = A.xu | A.xw | B.xu | B.xw, xa = A.xu & A.xw & B.xu & B.xw;
+      nxo = (xo & 0x8000u) ? 0xffu : 0u;
+      nxn = (xa & 0x8000u) ? 0xffu : 0u;
       nxm = min(min(A.xu, A.xw), min(B.xu, B.xw));
     }
 #pragma unroll
     for (int c = 1; c >= 0; --c) {
-      uint32_t o[5], n[5], mn[5];
+      const uint32_t O = A.su[c] | A.sw[c] | B.su[c] | B.sw[c];
+      const uint32_t N = A.su[c] & A.sw[c] & B.su[c] & B.sw[c];
+      const uint32_t O2 = __builtin_amdgcn_alignbyte(
+          __builtin_amdgcn_update_dpp(nxo, O, kDppWaveShl1, 0xf, 0xf, false), O, 1);
+      const uint32_t N2 = __builtin_amdgcn_alignbyte(
+          __builtin_amdgcn_update_dpp(nxn, N, kDppWaveShl1, 0xf, 0xf, false), N, 1);
+      const uint32_t M = (O | O2) & ~(N & N2) & cube_mask[c];
+      uint64_t sg[4];
+      sg[0] = __ballot((int8_t)M < 0);
+      sg[1] = __ballot((int16_t)M < 0);
+      sg[2] = __ballot((int8_t)(M >> 16) < 0);
+      sg[3] = __ballot((int32_t)M < 0);
+      uint32_t mn0 = 0;
+      if (sg[0] | sg[1] | sg[2] | sg[3]) {
+        uint32_t mn[5];
 #pragma unroll
-      for (int pos = 0; pos < 4; ++pos) {
-        const uint32_t a = A.u[c][pos], b = A.w[c][pos], cc = B.u[c][pos], d = B.w[c][pos];
-        o[pos] = a | b | cc | d;
-        n[pos] = a & b & cc & d;
-        mn[pos] = min(min(min(min(a, b), cc), d), lane_ok[c]);  // two v_min3
-      }
-      o[4] = __builtin_amdgcn_update_dpp(nxo, o[0], kDppWaveShl1, 0xf, 0xf, false);
-      n[4] = __builtin_amdgcn_update_dpp(nxn, n[0], kDppWaveShl1, 0xf, 0xf, false);
-      mn[4] = min(__builtin_amdgcn_update_dpp(nxm, mn[0], kDppWaveShl1, 0xf, 0xf, false),
-                  lane_mid[c]);
+        for (int pos = 0; pos < 4; ++pos)
+          mn[pos] = min(min(min(A.u[c][pos], A.w[c][pos]), B.u[c][pos]), B.w[c][pos]);
+        mn[4] = __builtin_amdgcn_update_dpp(nxm, mn[0], kDppWaveShl1, 0xf, 0xf, false);
+        mn0 = mn[0];
 #pragma unroll
-      for (int k = 0; k < 4; ++k) {
-        const uint32_t mixed = (o[k] | o[k + 1]) & ~(n[k] & n[k + 1]);
-        // all ones when bit 15 of mixed is set (both signs among the corners)
-        const uint32_t both = (uint32_t)((int32_t)(mixed << 16) >> 31);
-        m[c][k] = __ballot((min(mn[k], mn[k + 1]) & both) >= 0x10000u);
+        for (int k = 0; k < 4; ++k) m[c][k] = sg[k] & __ballot(min(mn[k], mn[k + 1]) >= 0x10000u);
+      } else {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) m[c][k] = 0;
+        if (c == 1)
+          mn0 = min(min(min(A.u[1][0], A.w[1][0]), B.u[1][0]), B.w[1][0]);
       }
       if (c == 1) {
-        nxo = __builtin_amdgcn_readlane(o[0], 0);
-        nxn = __builtin_amdgcn_readlane(n[0], 0);
-        nxm = __builtin_amdgcn_readlane(mn[0], 0);
+        nxo = __builtin_amdgcn_readlane(O, 0) & 0xffu;
+        nxn = __builtin_amdgcn_readlane(N, 0) & 0xffu;
+        nxm = __builtin_amdgcn_readlane(mn0, 0);
       }
     }
-    int64_t nrec = 0;
+    const uint64_t any_c0 = m[0][0] | m[0][1] | m[0][2] | m[0][3];
+    const uint64_t any_c1 = m[1][0] | m[1][1] | m[1][2] | m[1][3];
+    if ((any_c0 | any_c1) == 0) {
+      // no active cube: only the tile's vertex count (0) is read later
+      if (lane == 0) P.tileT[t] = 0;
+      t += tR;
+      if (z - zb == 63 || z + 1 == z1) {
+        if (nzmask) {
+          const uint32_t n = (uint32_t)__popcll(nzmask);
+          uint32_t base = 0;
+          if (lane == 0) base = atomicAdd(&P.hdr->nz_count, n);
+          base = __shfl(base, 0);
+          if ((nzmask >> lane) & 1ull) {
+            const uint32_t i = (uint32_t)__popcll(nzmask & ((1ull << lane) - 1ull));
+            P.nz_list[base + i] = (uint32_t)(V.tile_begin + (int64_t)(zb + lane) * V.R + r);
+          }
+        }
+        nzmask = 0;
+        zb = z + 1;
+      }
+      return;
+    }
+    int nrec32 = 0;  // at most 512
 #pragma unroll
     for (int c = 0; c < 2; ++c)
 #pragma unroll
-      for (int k = 0; k < 4; ++k) nrec += __popcll(m[c][k]);
+      for (int k = 0; k < 4; ++k) nrec32 += __popcll(m[c][k]);
+    const int64_t nrec = nrec32;
 
     // ---- destination of the tile's records
-    const int64_t room = P.quota - used;
     int64_t rbase = rec_base + used;
     bool dense = false, in_quota = true;
-    if (nrec > room) {
+    if ((uint32_t)nrec32 > quota - (uint32_t)used) {
       in_quota = false;
       unsigned long long ob = 0;
       if (lane == 0) ob = atomicAdd(reinterpret_cast<unsigned long long*>(&P.hdr->ovf_next),
@@ -605,9 +688,11 @@ __global__ __launch_bounds__(256) void count4_kernel(ExtractParams P) {
     }
 
     // ---- pass 2: the active cubes of a chunk are staged in the wave's LDS
-    // area in cube order (offset and 8 corner words), then taken 64 at a
-    // time, one per lane, for their case and vertex count, and written to the
-    // scratch with consecutive lanes on consecutive words.
+    // area in cube order (offset | case, 8 corner words) and written to the
+    // scratch with consecutive lanes on consecutive words.  The cases of a
+    // lane's four cubes come from the rows' sign bytes at positions k and
+    // k + 1 (active cubes have no zero weight, so the case is the corner
+    // signs), one bit per corner in corner order.
     uint32_t vsum = 0;
     int64_t cbase = 0;
 #pragma unroll
@@ -632,12 +717,44 @@ __global__ __launch_bounds__(256) void count4_kernel(ExtractParams P) {
         nbu = __builtin_amdgcn_update_dpp(lbu, B.u[c][0], kDppWaveShl1, 0xf, 0xf, false);
         nbw = __builtin_amdgcn_update_dpp(lbw, B.w[c][0], kDppWaveShl1, 0xf, 0xf, false);
       }
+      uint32_t cases;
+      {
+        // the rows' sign bytes at positions k + 1: down one byte, the next
+        // lane's byte 0 on top (lane 63: the next chunk's lane 0, or the
+        // words after the tile)
+        uint32_t x0, x1, x2, x3;
+        if (c == 0) {
+          x0 = __builtin_amdgcn_readlane(A.su[1], 0);
+          x1 = __builtin_amdgcn_readlane(A.sw[1], 0);
+          x2 = __builtin_amdgcn_readlane(B.su[1], 0);
+          x3 = __builtin_amdgcn_readlane(B.sw[1], 0);
+        } else {
+          x0 = (A.xu & 0x8000u) ? 0xffu : 0u;
+          x1 = (A.xw & 0x8000u) ? 0xffu : 0u;
+          x2 = (B.xu & 0x8000u) ? 0xffu : 0u;
+          x3 = (B.xw & 0x8000u) ? 0xffu : 0u;
+        }
+        auto next = [&](uint32_t v, uint32_t x) {
+          return __builtin_amdgcn_alignbyte(
+              __builtin_amdgcn_update_dpp(x, v, kDppWaveShl1, 0xf, 0xf, false), v, 1);
+        };
+        const uint32_t au1 = next(A.su[c], x0), aw1 = next(A.sw[c], x1);
+        const uint32_t bu1 = next(B.su[c], x2), bw1 = next(B.sw[c], x3);
+        cases = bfi(0x01010101u, A.su[c],
+                bfi(0x02020202u, au1,
+                bfi(0x04040404u, aw1,
+                bfi(0x08080808u, A.sw[c],
+                bfi(0x10101010u, B.su[c],
+                bfi(0x20202020u, bu1, bfi(0x40404040u, bw1, B.sw[c])))))));
+      }
       uint32_t own = 0;
 #pragma unroll
       for (int k = 0; k < 4; ++k) {
         if ((m[c][k] >> lane) & 1ull) {
           uint32_t* rp = my_rec + (pre + own) * kRecWords;
-          rp[0] = (uint32_t)(256 * c + 4 * lane + k) << 8;
+          const uint32_t ci = (cases >> (8 * k)) & 0xffu;
+          rp[0] = ((uint32_t)(256 * c + 4 * lane + k) << 8) | ci;
+          vsum += s_nv[ci];
           rp[1] = A.u[c][k];
           rp[2] = k < 3 ? A.u[c][(k + 1) & 3] : nau;
           rp[3] = k < 3 ? A.w[c][(k + 1) & 3] : naw;
@@ -650,16 +767,6 @@ __global__ __launch_bounds__(256) void count4_kernel(ExtractParams P) {
         }
       }
       const int nc = __popcll(m[c][0]) + __popcll(m[c][1]) + __popcll(m[c][2]) + __popcll(m[c][3]);
-      wave_sync();
-      for (int i = lane; i < nc; i += 64) {
-        uint32_t* rp = my_rec + i * kRecWords;
-        // active cubes have no zero weight: the case is the corner signs
-        uint32_t ci = 0;
-#pragma unroll
-        for (int q = 0; q < 8; ++q) ci |= ((rp[1 + q] >> 15) & 1u) << q;
-        rp[0] |= ci;
-        vsum += s_nv[ci];
-      }
       wave_sync();
       if (!dense) {
         // the chunk's records are consecutive in scratch
@@ -677,7 +784,8 @@ __global__ __launch_bounds__(256) void count4_kernel(ExtractParams P) {
       P.tile_rec[t] = rbase;
       if (dense) atomicAdd(&P.hdr->dense_acc, 1u);
     }
-    if (in_quota) used += nrec;
+    if (in_quota) used += nrec32;
+    t += tR;
     // append the column's non-empty tiles to the list, 64 planes at a time
     if (vsum > 0) nzmask |= 1ull << (z - zb);
     if (z - zb == 63 || z + 1 == z1) {
@@ -869,7 +977,8 @@ constexpr int kWaveRec = 64 * kRecWords;  // 64 records (cube offset << 8 | case
 constexpr int kWaveOut = 64 * 7;          // 64 staged vertex_voxel records
 constexpr int kWaveMap = 64 * 15 / 4;     // vertex -> record (u8), at most 15 vertices a record
 constexpr int kWaveEx = 64;               // first vertex of each record
-constexpr int kWaveWords = kWaveRec + kWaveOut + kWaveMap + kWaveEx;
+constexpr int kWaveMeta = 64 * 4;         // packed blocks: output index (2 words), p, z
+constexpr int kWaveWords = kWaveRec + kWaveOut + kWaveMap + kWaveEx + kWaveMeta;
 
 // Emit the vertices of the (up to 64) records in the wave's LDS area s_rec
 // (record r at s_rec[9r]; n of them) to output indices out, out + 1, ...:
@@ -933,6 +1042,71 @@ __device__ __forceinline__ int emit_records(const ExtractParams& P, const VolDes
   return tot;
 }
 
+// Emit the vertices of a packed block: nb records (up to 64, possibly of
+// several tiles) in the wave's record area, record r with its output index
+// (of its first vertex) and cube position (p = in-plane cube index, z) in
+// s_meta[4r..4r+3].  The records' vertex counts are scanned across the wave,
+// every record writes its index into the vertex -> record map, and each lane
+// produces one vertex of a round of 64 and stores its record fields at its
+// output index (the vertex's own slot; consecutive lanes mostly write
+// consecutive records).
+template <int ELEM>
+__device__ __forceinline__ void emit_block(const ExtractParams& P, const VolDesc& V,
+                                           const uint64_t* s_case, const uint8_t* s_nv,
+                                           uint32_t* s_w, int nb) {
+  const uint32_t* s_rec = s_w;
+  uint8_t* s_map = reinterpret_cast<uint8_t*>(s_w + kWaveRec + kWaveOut);
+  uint32_t* s_ex = s_w + kWaveRec + kWaveOut + kWaveMap;
+  const uint32_t* s_meta = s_w + kWaveRec + kWaveOut + kWaveMap + kWaveEx;
+  const int lane = threadIdx.x & 63;
+  wave_sync();
+  const int nv = lane < nb ? s_nv[s_rec[lane * kRecWords] & 0xffu] : 0;
+  const uint32_t incl = wave_incl_scan32((uint32_t)nv);
+  const int ex = (int)incl - nv;
+  const int tot = (int)__builtin_amdgcn_readlane(incl, 63);
+  s_ex[lane] = (uint32_t)ex;
+  for (int q = 0; q < nv; ++q) s_map[ex + q] = (uint8_t)lane;
+  wave_sync();
+  uint32_t* vv = reinterpret_cast<uint32_t*>(P.vv);
+  for (int vb = 0; vb < tot; vb += 64) {
+    const int j = vb + lane;
+    if (j < tot) {
+      const int r = s_map[j];
+      const uint32_t* rc = s_rec + r * kRecWords;
+      const uint32_t rw = rc[0];
+      uint32_t c[8];
+#pragma unroll
+      for (int k = 0; k < 8; ++k) c[k] = rc[1 + k];
+      const uint32_t* mt = s_meta + 4 * r;
+      const int slot = j - (int)s_ex[r];
+      const int64_t out = (int64_t)(((uint64_t)mt[1] << 32) | mt[0]) + slot;
+      if (out < P.cap) {
+        const uint32_t p = mt[2];
+        const uint32_t y = V.divX.div(p);
+        const int x = (int)(p - y * (uint32_t)V.X);
+        uint32_t o[7];
+        int64_t key;
+        make_vertex<ELEM>(V, s_case, c, x, (int)y, (int)mt[3], rw & 0xffu, slot, true, o,
+                          P.ekey ? &key : nullptr);
+        if (vv) {
+#pragma unroll
+          for (int f = 0; f < 7; ++f) vv[7 * out + f] = o[f];
+        }
+        if (P.pos) {
+#pragma unroll
+          for (int f = 0; f < 3; ++f) P.pos[3 * out + f] = __uint_as_float(o[f]);
+        }
+        if (P.v2vox) {
+#pragma unroll
+          for (int f = 0; f < 3; ++f) P.v2vox[3 * out + f] = (int32_t)o[3 + f];
+        }
+        if (P.ekey) P.ekey[out] = key;
+      }
+    }
+  }
+  wave_sync();
+}
+
 // Wave-level emission: wave w of the grid takes list entries w, w + nwaves,
 // ... (no atomics: one counter shared by every wave serialises).  A tile's
 // vertices are [voff[t], voff[t] + T), in cube order, so the output does not
@@ -955,6 +1129,7 @@ __global__ __launch_bounds__(kEmitThreads) void emit_kernel(ExtractParams P) {
   const uint32_t nz = P.hdr->nz_total;
   const uint32_t nwaves = gridDim.x * (kEmitThreads / 64);
   const uint32_t wave0 = blockIdx.x * (kEmitThreads / 64) + (uint32_t)wv;
+  int nblk = 0;  // records in the packed block (single volumes)
   // this wave's items: wave0 + i * nwaves, in batches of 64
   for (uint32_t ib = 0; wave0 + ib * nwaves < nz; ib += 64) {
     const uint32_t my_item = wave0 + (ib + (uint32_t)lane) * nwaves;
@@ -996,7 +1171,43 @@ __global__ __launch_bounds__(kEmitThreads) void emit_kernel(ExtractParams P) {
       const int64_t local = t - V.tile_begin;
       const int z = (int)V.divR.div((uint32_t)local);
       const int64_t p0 = (local - (int64_t)z * V.R) * kTile;
-      if (nrec >= 0) {
+      if (nrec >= 0 && !BATCH) {
+        // append the tile's records to the packed block (emitted when full)
+        uint32_t* s_meta = s_w + kWaveRec + kWaveOut + kWaveMap + kWaveEx;
+        for (int rb = 0; rb < nrec; rb += 64) {
+          const int n = min(64, nrec - rb);
+          if (nblk + n > 64) {
+            emit_block<ELEM>(P, V, s_case, s_nv, s_w, nblk);
+            nblk = 0;
+          }
+          uint32_t* dst = s_w + nblk * kRecWords;
+          if (rb == 0) {
+#pragma unroll
+            for (int q = 0; q < kRecWords; ++q)
+              if (lane + 64 * q < n * kRecWords) dst[lane + 64 * q] = cur[q];
+          } else {
+            const uint32_t* src = P.rec + (rs + rb) * kRecWords;
+            for (int w = lane; w < n * kRecWords; w += 64) dst[w] = src[w];
+          }
+          wave_sync();
+          uint32_t nv = 0, rw = 0;
+          if (lane < n) {
+            rw = dst[lane * kRecWords];
+            nv = s_nv[rw & 0xffu];
+          }
+          const uint32_t incl = wave_incl_scan32(nv);
+          if (lane < n) {
+            const uint64_t o = (uint64_t)(out + (int64_t)(incl - nv));
+            uint32_t* mt = s_meta + 4 * (nblk + lane);
+            mt[0] = (uint32_t)o;
+            mt[1] = (uint32_t)(o >> 32);
+            mt[2] = (uint32_t)(p0 + (rw >> 8));
+            mt[3] = (uint32_t)z;
+          }
+          out += (int64_t)__builtin_amdgcn_readlane(incl, 63);
+          nblk += n;
+        }
+      } else if (nrec >= 0) {
         for (int rb = 0; rb < nrec; rb += 64) {
           const int n = min(64, nrec - rb);
           if (rb == 0) {
@@ -1010,6 +1221,10 @@ __global__ __launch_bounds__(kEmitThreads) void emit_kernel(ExtractParams P) {
           out += emit_records<ELEM>(P, V, s_case, s_nv, s_w, n, z, p0, out);
         }
       } else {
+        if (nblk) {
+          emit_block<ELEM>(P, V, s_case, s_nv, s_w, nblk);
+          nblk = 0;
+        }
         const int X = V.X;
         const int64_t XY = (int64_t)X * V.Y;
         const uint32_t* vol = V.vol + (int64_t)z * XY;
@@ -1042,6 +1257,9 @@ __global__ __launch_bounds__(kEmitThreads) void emit_kernel(ExtractParams P) {
         }
       }
     }
+  }
+  if constexpr (!BATCH) {
+    if (nblk) emit_block<ELEM>(P, vol_at<false>(P, 0), s_case, s_nv, s_w, nblk);
   }
 }
 
@@ -1221,6 +1439,10 @@ size_t al256(size_t b) { return (b + 255) & ~(size_t)255; }
 // stay below 2^32.
 bool plane_ok(int X, int Y) { return (int64_t)X * Y + X + 1024 < ((int64_t)1 << 30); }
 
+// The 128-bit count kernel addresses a column's planes (slab + 1, plus the
+// prefetched ones past it) through one buffer resource with 32-bit offsets.
+bool span_ok(int64_t XY, int slab) { return ((int64_t)slab + 4) * XY * 4 <= ((int64_t)1 << 31); }
+
 int64_t tiles_per_plane(int X, int Y) { return ((int64_t)X * Y + kTile - 1) / kTile; }
 
 int64_t tiles_of(int X, int Y, int Z) {
@@ -1364,7 +1586,8 @@ int prepare_single(ExtractParams& P, const void* d_vol, int X, int Y, int Z, flo
   S = (Z - 1 + P.slab - 1) / P.slab;
   P.ncols = R * S;
   P.slab_max = P.slab;
-  P.wide = X % 4 == 0 && (reinterpret_cast<uintptr_t>(d_vol) & 15u) == 0;
+  P.wide = X % 4 == 0 && (reinterpret_cast<uintptr_t>(d_vol) & 15u) == 0 &&
+           span_ok((int64_t)X * Y, P.slab);
   return MCX_OK;
 }
 
@@ -1611,7 +1834,8 @@ extern "C" int mcx_batch_create(mcx_batch** out, int nvol, const int* dims_xyz, 
     tb += nt;
     cubes += cubes_of(X, Y, Z);
     maxX = std::max(maxX, X);
-    wide = wide && X % 4 == 0 && (reinterpret_cast<uintptr_t>(d_vols[b]) & 15u) == 0;
+    wide = wide && X % 4 == 0 && (reinterpret_cast<uintptr_t>(d_vols[b]) & 15u) == 0 &&
+           span_ok((int64_t)X * Y, g_tuning.batch_slab);
   }
   mcx_batch* p = new mcx_batch();
   p->wide = wide;
