@@ -1107,6 +1107,162 @@ __device__ __forceinline__ void emit_block(const ExtractParams& P, const VolDesc
   wave_sync();
 }
 
+// Inclusive max over lanes 0..lane.
+__device__ __forceinline__ uint32_t wave_incl_max32(uint32_t v) {
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const uint32_t n = __shfl_up(v, d, 64);
+    if (lane >= d) v = max(v, n);
+  }
+  return v;
+}
+
+// Single volumes: the wave's tiles (list entries wave0, wave0 + nwaves, ...)
+// are taken in runs whose records fit one 64-record block: lane i loads
+// record i of the run (all loads in flight at once; lane i finds its tile
+// by a max-scan over the tiles' first-record marks), the records' vertex
+// counts are scanned, and each record's output index is its tile's first
+// vertex plus the vertices of the tile's earlier records.  A tile with more
+// than 64 records goes in blocks of 64 of its own; a tile whose records did
+// not fit the scratch is re-read from the volume.
+template <int ELEM>
+__global__ __launch_bounds__(kEmitThreads) void emit_packed_kernel(ExtractParams P) {
+  __shared__ uint64_t s_case[256];
+  __shared__ uint8_t s_nv[256];
+  __shared__ uint32_t s_wave[kEmitThreads / 64][kWaveWords];
+  __shared__ uint32_t s_mark[kEmitThreads / 64][64];
+  s_case[threadIdx.x] = dCase[threadIdx.x];
+  s_nv[threadIdx.x] = dNumVerts[threadIdx.x];
+  const int lane = threadIdx.x & 63;
+  const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  s_mark[wv][lane] = 0;
+  __syncthreads();  // the only workgroup barrier
+  uint32_t* const s_w = s_wave[wv];
+  uint32_t* const mark = s_mark[wv];
+  uint32_t* const s_meta = s_w + kWaveRec + kWaveOut + kWaveMap + kWaveEx;
+  const VolDesc& V = P.vol0;
+  const uint32_t nz = P.hdr->nz_total;
+  const uint32_t nwaves = gridDim.x * (kEmitThreads / 64);
+  const uint32_t wave0 = blockIdx.x * (kEmitThreads / 64) + (uint32_t)wv;
+  for (uint32_t ib = 0; wave0 + ib * nwaves < nz; ib += 64) {
+    const uint32_t my_item = wave0 + (ib + (uint32_t)lane) * nwaves;
+    const bool have = my_item < nz;
+    const int nb = __popcll(__ballot(have));
+    int64_t off_l = 0, rs_l = 0;
+    int nrec_l = 0;
+    uint32_t p0_l = 0, z_l = 0;
+    if (have) {
+      const uint32_t t = P.nz_list[my_item];
+      nrec_l = P.tile_nrec[t];
+      off_l = P.voff[t];
+      rs_l = P.tile_rec[t];
+      const uint32_t local = (uint32_t)(t - V.tile_begin);
+      z_l = V.divR.div(local);
+      p0_l = (local - z_l * (uint32_t)V.R) * (uint32_t)kTile;
+    }
+    int a = 0;
+    while (a < nb) {
+      // the run [a, b): whole tiles of at most 64 records, 64 in all
+      const bool small = nrec_l >= 0 && nrec_l <= 64;
+      const uint32_t v = lane < a ? 0u : (lane < nb && small ? (uint32_t)nrec_l : 65u);
+      const uint32_t incl = wave_incl_scan32(v);
+      const uint64_t over = __ballot(lane >= a && incl > 64u);
+      const int b = over ? __ffsll((unsigned long long)over) - 1 : 64;
+      if (b == a) {
+        // one tile on its own: more than 64 records, or re-read from the volume
+        const int nrec = __shfl(nrec_l, a);
+        int64_t out = __shfl(off_l, a);
+        const int64_t rs = __shfl(rs_l, a);
+        const int z = (int)__shfl(z_l, a);
+        const int64_t p0 = (int64_t)__shfl(p0_l, a);
+        if (nrec >= 0) {
+          for (int rb = 0; rb < nrec; rb += 64) {
+            const int n = min(64, nrec - rb);
+            const uint32_t* src = P.rec + (rs + rb) * kRecWords;
+            for (int w = lane; w < n * kRecWords; w += 64) s_w[w] = src[w];
+            wave_sync();
+            out += emit_records<ELEM>(P, V, s_case, s_nv, s_w, n, z, p0, out);
+          }
+        } else {
+          const int X = V.X;
+          const int64_t XY = (int64_t)X * V.Y;
+          const uint32_t* vol = V.vol + (int64_t)z * XY;
+          for (int cb = 0; cb < kTile; cb += 64) {
+            const int o = cb + lane;
+            const int64_t p = p0 + o;
+            uint32_t cw[8] = {}, ci = 0;
+            if (p < XY) {
+              const uint32_t y = V.divX.div((uint32_t)p);
+              const uint32_t x = (uint32_t)p - y * (uint32_t)X;
+              if (x + 1 < (uint32_t)X && y + 1 < (uint32_t)V.Y) {
+                const uint32_t* pa = vol + p;
+                const uint32_t* pb = pa + XY;
+                cw[0] = pa[0]; cw[1] = pa[1]; cw[2] = pa[X + 1]; cw[3] = pa[X];
+                cw[4] = pb[0]; cw[5] = pb[1]; cw[6] = pb[X + 1]; cw[7] = pb[X];
+                ci = cube_index<ELEM>(cw);
+              }
+            }
+            const uint64_t act = __ballot(s_nv[ci] != 0);
+            const int n = __popcll(act);
+            if ((act >> lane) & 1ull) {
+              uint32_t* rp = s_w + __popcll(act & ((1ull << lane) - 1ull)) * kRecWords;
+              rp[0] = ((uint32_t)o << 8) | ci;
+#pragma unroll
+              for (int q = 0; q < 8; ++q) rp[1 + q] = cw[q];
+            }
+            wave_sync();
+            if (n) out += emit_records<ELEM>(P, V, s_case, s_nv, s_w, n, z, p0, out);
+          }
+        }
+        ++a;
+        continue;
+      }
+      const int R = (int)__builtin_amdgcn_readlane(incl, b - 1);
+      // lane i < R: record i of the run belongs to tile k (lane k's item)
+      if (lane >= a && lane < b) mark[incl - v] = (uint32_t)lane + 1u;
+      wave_sync();
+      uint32_t mk = 0;
+      if (lane < R) {
+        mk = mark[lane];
+        mark[lane] = 0;
+      }
+      const int k = (int)wave_incl_max32(mk) - 1;
+      const int kk = lane < R ? k : a;
+      const int first = (int)__shfl((int)(incl - v), kk);  // the tile's first record in the run
+      const int64_t rs = __shfl(rs_l, kk);
+      uint32_t rec[kRecWords];
+      if (lane < R) {
+        const uint32_t* src = P.rec + (rs + (lane - first)) * kRecWords;
+#pragma unroll
+        for (int q = 0; q < kRecWords; ++q) rec[q] = src[q];
+      } else {
+#pragma unroll
+        for (int q = 0; q < kRecWords; ++q) rec[q] = 0;
+      }
+      const uint32_t nv = lane < R ? s_nv[rec[0] & 0xffu] : 0u;
+      const uint32_t vincl = wave_incl_scan32(nv);
+      const uint32_t vex = vincl - nv;
+      const uint32_t vfirst = (uint32_t)__shfl((int)vex, first);
+      const int64_t off = __shfl(off_l, kk);
+      const uint32_t p0 = (uint32_t)__shfl((int)p0_l, kk);
+      const uint32_t z = (uint32_t)__shfl((int)z_l, kk);
+      if (lane < R) {
+#pragma unroll
+        for (int q = 0; q < kRecWords; ++q) s_w[lane * kRecWords + q] = rec[q];
+        const uint64_t o = (uint64_t)(off + (int64_t)(vex - vfirst));
+        uint32_t* mt = s_meta + 4 * lane;
+        mt[0] = (uint32_t)o;
+        mt[1] = (uint32_t)(o >> 32);
+        mt[2] = p0 + (rec[0] >> 8);
+        mt[3] = z;
+      }
+      emit_block<ELEM>(P, V, s_case, s_nv, s_w, R);
+      a = b;
+    }
+  }
+}
+
 // Wave-level emission: wave w of the grid takes list entries w, w + nwaves,
 // ... (no atomics: one counter shared by every wave serialises).  A tile's
 // vertices are [voff[t], voff[t] + T), in cube order, so the output does not
@@ -1532,7 +1688,8 @@ int run_extract(ExtractParams& P, int elem, bool batch, int maxX, hipStream_t s,
       MCX_HIP(hipGetLastError());                                                                \
     }                                                                                            \
     if (!emit) break;                                                                            \
-    hipLaunchKernelGGL((emit_kernel<E, B>), dim3(k3_blocks), dim3(kEmitThreads), 0, s, P);      \
+    if (B) hipLaunchKernelGGL((emit_kernel<E, B>), dim3(k3_blocks), dim3(kEmitThreads), 0, s, P); \
+    else hipLaunchKernelGGL(emit_packed_kernel<E>, dim3(k3_blocks), dim3(kEmitThreads), 0, s, P); \
     MCX_HIP(hipGetLastError());                                                                  \
   } while (0)
   if (elem == MCX_ELEM_SHORT2) {
