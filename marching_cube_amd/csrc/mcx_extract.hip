@@ -367,8 +367,7 @@ __global__ __launch_bounds__(256) void count_kernel(ExtractParams P) {
       nrec += __popcll(m);
     }
     const bool dense = nrec > room;
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) vsum += __shfl_xor(vsum, o);
+    vsum = wave_sum32(vsum);
     if (lane == 0) {
       P.tileT[t] = (int32_t)vsum;
       P.tile_nrec[t] = dense ? -1 : (int32_t)nrec;
@@ -776,8 +775,7 @@ void count4_kernel(ExtractParams P) {
       }
       cbase += nc;
     }
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) vsum += __shfl_xor(vsum, o);
+    vsum = wave_sum32(vsum);
     if (lane == 0) {
       P.tileT[t] = (int32_t)vsum;
       P.tile_nrec[t] = dense ? -1 : (int32_t)nrec;
@@ -954,12 +952,12 @@ __device__ __forceinline__ void make_vertex(const VolDesc& V, const uint64_t* s_
   const int vx = (int)floorf(cx), vy = (int)floorf(cy), vz = (int)floorf(cz);
   const bool inb = vx >= 0 && vy >= 0 && vz >= 0 && vx < V.X && vy < V.Y && vz < V.Z;
   uint32_t tw = 0;
-  int64_t li = 0;
+  auto lin = [&]() { return ((int64_t)vz * V.Y + vy) * V.X + vx; };
   if (inb) {
-    li = ((int64_t)vz * V.Y + vy) * V.X + vx;
     if constexpr (ELEM == MCX_ELEM_SHORT2) {
+      // the voxel is almost always one of the cube's corners
       const unsigned dx = (unsigned)(vx - x), dy = (unsigned)(vy - y), dz = (unsigned)(vz - z);
-      tw = (dx | dy | dz) <= 1u ? pick8(cw, corner_of((int)dx, (int)dy, (int)dz)) : V.vol[li];
+      tw = (dx | dy | dz) <= 1u ? pick8(cw, corner_of((int)dx, (int)dy, (int)dz)) : V.vol[lin()];
     }
   }
   o[0] = __float_as_uint(px);
@@ -969,7 +967,7 @@ __device__ __forceinline__ void make_vertex(const VolDesc& V, const uint64_t* s_
   o[4] = (uint32_t)vy;
   o[5] = (uint32_t)vz;
   o[6] = tw;
-  if (write_vv && V.ver_voxel && inb) V.ver_voxel[li] = (int16_t)(tw & 0xffffu);
+  if (write_vv && V.ver_voxel && inb) V.ver_voxel[lin()] = (int16_t)(tw & 0xffffu);
 }
 
 // Per-wave LDS areas of the emit kernel (32-bit words).
@@ -1107,17 +1105,6 @@ __device__ __forceinline__ void emit_block(const ExtractParams& P, const VolDesc
   wave_sync();
 }
 
-// Inclusive max over lanes 0..lane.
-__device__ __forceinline__ uint32_t wave_incl_max32(uint32_t v) {
-  const int lane = threadIdx.x & 63;
-#pragma unroll
-  for (int d = 1; d < 64; d <<= 1) {
-    const uint32_t n = __shfl_up(v, d, 64);
-    if (lane >= d) v = max(v, n);
-  }
-  return v;
-}
-
 // Single volumes: the wave's tiles (list entries wave0, wave0 + nwaves, ...)
 // are taken in runs whose records fit one 64-record block: lane i loads
 // record i of the run (all loads in flight at once; lane i finds its tile
@@ -1177,12 +1164,32 @@ __global__ __launch_bounds__(kEmitThreads) void emit_packed_kernel(ExtractParams
         const int z = (int)__shfl(z_l, a);
         const int64_t p0 = (int64_t)__shfl(p0_l, a);
         if (nrec >= 0) {
+          // 64 records at a time, each block emitted like a run
           for (int rb = 0; rb < nrec; rb += 64) {
             const int n = min(64, nrec - rb);
-            const uint32_t* src = P.rec + (rs + rb) * kRecWords;
-            for (int w = lane; w < n * kRecWords; w += 64) s_w[w] = src[w];
-            wave_sync();
-            out += emit_records<ELEM>(P, V, s_case, s_nv, s_w, n, z, p0, out);
+            uint32_t rec[kRecWords];
+            if (lane < n) {
+              const uint32_t* src = P.rec + (rs + rb + lane) * kRecWords;
+#pragma unroll
+              for (int q = 0; q < kRecWords; ++q) rec[q] = src[q];
+            } else {
+#pragma unroll
+              for (int q = 0; q < kRecWords; ++q) rec[q] = 0;
+            }
+            const uint32_t nv = lane < n ? s_nv[rec[0] & 0xffu] : 0u;
+            const uint32_t vincl = wave_incl_scan32(nv);
+            if (lane < n) {
+#pragma unroll
+              for (int q = 0; q < kRecWords; ++q) s_w[lane * kRecWords + q] = rec[q];
+              const uint64_t o = (uint64_t)(out + (int64_t)(vincl - nv));
+              uint32_t* mt = s_meta + 4 * lane;
+              mt[0] = (uint32_t)o;
+              mt[1] = (uint32_t)(o >> 32);
+              mt[2] = (uint32_t)p0 + (rec[0] >> 8);
+              mt[3] = (uint32_t)z;
+            }
+            out += (int64_t)__builtin_amdgcn_readlane(vincl, 63);
+            emit_block<ELEM>(P, V, s_case, s_nv, s_w, n);
           }
         } else {
           const int X = V.X;

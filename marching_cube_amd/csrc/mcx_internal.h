@@ -91,24 +91,58 @@ __device__ __forceinline__ uint64_t shfl_xor_u64(uint64_t v, int m) {
   return ((uint64_t)hi << 32) | lo;
 }
 
-// Inclusive prefix sum over the 64 lanes of a wave.
-__device__ __forceinline__ uint64_t wave_incl_scan(uint64_t v) {
-  const int lane = lane_id();
-#pragma unroll
-  for (int d = 1; d < kWave; d <<= 1) {
-    uint64_t n = shfl_up_u64(v, d);
-    if (lane >= d) v += n;
-  }
+
+// DPP controls (gfx9 wave64): row_shr:n shifts within rows of 16 lanes,
+// row_bcast:15 / :31 pass lane 15 of each row / lane 31 on to the next
+// row(s); a lane without a source takes `old` (0 below).
+constexpr int kDppRowShr1 = 0x111, kDppRowShr2 = 0x112, kDppRowShr4 = 0x114,
+              kDppRowShr8 = 0x118, kDppRowBcast15 = 0x142, kDppRowBcast31 = 0x143;
+
+// Inclusive prefix sum over the 64 lanes of a wave in registers: four row
+// shifts give each row of 16 its own scan, then lane 15's total goes to
+// rows 1 and 3 and lane 31's to rows 2 and 3.
+__device__ __forceinline__ uint32_t wave_incl_scan32(uint32_t v) {
+  v += __builtin_amdgcn_update_dpp(0u, v, kDppRowShr1, 0xf, 0xf, false);
+  v += __builtin_amdgcn_update_dpp(0u, v, kDppRowShr2, 0xf, 0xf, false);
+  v += __builtin_amdgcn_update_dpp(0u, v, kDppRowShr4, 0xf, 0xf, false);
+  v += __builtin_amdgcn_update_dpp(0u, v, kDppRowShr8, 0xf, 0xf, false);
+  v += __builtin_amdgcn_update_dpp(0u, v, kDppRowBcast15, 0xa, 0xf, false);
+  v += __builtin_amdgcn_update_dpp(0u, v, kDppRowBcast31, 0xc, 0xf, false);
   return v;
 }
-__device__ __forceinline__ uint32_t wave_incl_scan32(uint32_t v) {
-  const int lane = lane_id();
-#pragma unroll
-  for (int d = 1; d < kWave; d <<= 1) {
-    uint32_t n = __shfl_up(v, d, kWave);
-    if (lane >= d) v += n;
-  }
+
+// 64-bit inclusive prefix sum over the lanes of a wave (the same DPP
+// pattern on both halves, a 64-bit add per step).
+template <int CTRL, int ROWS>
+__device__ __forceinline__ uint64_t dpp_u64(uint64_t v) {
+  const uint32_t lo = __builtin_amdgcn_update_dpp(0u, (uint32_t)v, CTRL, ROWS, 0xf, false);
+  const uint32_t hi = __builtin_amdgcn_update_dpp(0u, (uint32_t)(v >> 32), CTRL, ROWS, 0xf, false);
+  return ((uint64_t)hi << 32) | lo;
+}
+__device__ __forceinline__ uint64_t wave_incl_scan(uint64_t v) {
+  v += dpp_u64<kDppRowShr1, 0xf>(v);
+  v += dpp_u64<kDppRowShr2, 0xf>(v);
+  v += dpp_u64<kDppRowShr4, 0xf>(v);
+  v += dpp_u64<kDppRowShr8, 0xf>(v);
+  v += dpp_u64<kDppRowBcast15, 0xa>(v);
+  v += dpp_u64<kDppRowBcast31, 0xc>(v);
   return v;
+}
+
+// Inclusive maximum over the lanes of a wave (same pattern).
+__device__ __forceinline__ uint32_t wave_incl_max32(uint32_t v) {
+  v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0u, v, kDppRowShr1, 0xf, 0xf, false));
+  v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0u, v, kDppRowShr2, 0xf, 0xf, false));
+  v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0u, v, kDppRowShr4, 0xf, 0xf, false));
+  v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0u, v, kDppRowShr8, 0xf, 0xf, false));
+  v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0u, v, kDppRowBcast15, 0xa, 0xf, false));
+  v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0u, v, kDppRowBcast31, 0xc, 0xf, false));
+  return v;
+}
+
+// Sum over the 64 lanes, the same in every lane.
+__device__ __forceinline__ uint32_t wave_sum32(uint32_t v) {
+  return __builtin_amdgcn_readlane(wave_incl_scan32(v), 63);
 }
 __device__ __forceinline__ uint64_t wave_sum(uint64_t v) {
 #pragma unroll
