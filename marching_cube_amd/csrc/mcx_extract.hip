@@ -458,7 +458,7 @@ constexpr int kDppWaveShl1 = 0x130;
 
 struct Plane4 {
   u32x4 u[2], w[2];  // rows y (u) and y + 1 (w) of chunks 0 and 1
-  uint32_t xu, xw;   // word 512 of the two rows (wave-uniform; neighbour of chunk 1's lane 63)
+  uint32_t xu, xw;   // word 512 of the two rows (the same in all lanes; neighbour of chunk 1's lane 63)
   // per chunk, byte k = 0xff / 0x00: the tsdf at position k of row u / w
   // negative (set once per plane, used by two steps)
   uint32_t su[2], sw[2];
@@ -553,32 +553,57 @@ void count4_kernel(ExtractParams P) {
   // lanes without cubes see (they are masked).  The host keeps
   // (z1 - z0 + 4) planes under 2^31 bytes.
   const uint32_t plane_bytes = (uint32_t)(XY * 4);
-  const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
-      const_cast<uint32_t*>(V.vol + (int64_t)z0 * XY), (short)0,
-      (int)((uint32_t)(z1 - z0 + 1) * plane_bytes), kBufferWord3);
   const uint32_t voff = (uint32_t)(p0 + 4 * lane) * 4u;
   const uint32_t voffW = voff + 4u * (uint32_t)X;
+  // The plane loads are issued as inline assembly so that the compiler's
+  // wait insertion does not see them: after a step's variable number of
+  // record stores it could only wait for every outstanding load (including
+  // the prefetched plane).  Each step instead waits for all but the newest
+  // six vector memory operations (vector memory operations of a wave retire
+  // in issue order), tied to the registers it is about to read.
+  u32x4 rsv;  // buffer resource: base, stride 0, size, format word
+  {
+    const uint64_t base = reinterpret_cast<uint64_t>(V.vol + (int64_t)z0 * XY);
+    rsv[0] = (uint32_t)base;
+    rsv[1] = (uint32_t)(base >> 32) & 0xffffu;
+    rsv[2] = (uint32_t)(z1 - z0 + 1) * plane_bytes;
+    rsv[3] = (uint32_t)kBufferWord3;
+    rsv[0] = __builtin_amdgcn_readfirstlane(rsv[0]);
+    rsv[1] = __builtin_amdgcn_readfirstlane(rsv[1]);
+    rsv[2] = __builtin_amdgcn_readfirstlane(rsv[2]);
+    rsv[3] = __builtin_amdgcn_readfirstlane(rsv[3]);
+  }
   // the word after the tile in each row (the same for all lanes; an offset
   // past the resource, hence 0, when it lies outside the plane)
   const uint32_t xoffU = p0 + kTile < XY ? (uint32_t)(p0 + kTile) * 4u : 0x80000000u;
   const uint32_t xoffW = p0 + kTile + X < XY ? (uint32_t)(p0 + kTile + X) * 4u : 0x80000000u;
   auto load_plane = [&](int q, Plane4& L) {
     const uint32_t po = (uint32_t)(q - z0) * plane_bytes;
-    L.u[0] = __builtin_amdgcn_raw_buffer_load_b128(rs, voff + po, 0, 0);
-    L.u[1] = __builtin_amdgcn_raw_buffer_load_b128(rs, voff + po + 1024u, 0, 0);
-    L.w[0] = __builtin_amdgcn_raw_buffer_load_b128(rs, voffW + po, 0, 0);
-    L.w[1] = __builtin_amdgcn_raw_buffer_load_b128(rs, voffW + po + 1024u, 0, 0);
-    L.xu = __builtin_amdgcn_readfirstlane(__builtin_amdgcn_raw_buffer_load_b32(rs, xoffU + po, 0, 0));
-    L.xw = __builtin_amdgcn_readfirstlane(__builtin_amdgcn_raw_buffer_load_b32(rs, xoffW + po, 0, 0));
+    const uint32_t a = voff + po, b = voffW + po;
+    asm volatile("buffer_load_dwordx4 %0, %1, %2, 0 offen" : "=v"(L.u[0]) : "v"(a), "s"(rsv) : "memory");
+    asm volatile("buffer_load_dwordx4 %0, %1, %2, 0 offen offset:1024" : "=v"(L.u[1]) : "v"(a), "s"(rsv) : "memory");
+    asm volatile("buffer_load_dwordx4 %0, %1, %2, 0 offen" : "=v"(L.w[0]) : "v"(b), "s"(rsv) : "memory");
+    asm volatile("buffer_load_dwordx4 %0, %1, %2, 0 offen offset:1024" : "=v"(L.w[1]) : "v"(b), "s"(rsv) : "memory");
+    const uint32_t xa = xoffU + po, xb = xoffW + po;
+    asm volatile("buffer_load_dword %0, %1, %2, 0 offen" : "=v"(L.xu) : "v"(xa), "s"(rsv) : "memory");
+    asm volatile("buffer_load_dword %0, %1, %2, 0 offen" : "=v"(L.xw) : "v"(xb), "s"(rsv) : "memory");
+  };
+  auto wait_plane = [&](Plane4& L, int n) {
+    // all but the newest n vector memory operations done; ties L's registers
+    if (n == 6)
+      asm volatile("s_waitcnt vmcnt(6)" : "+v"(L.u[0]), "+v"(L.u[1]), "+v"(L.w[0]), "+v"(L.w[1]), "+v"(L.xu), "+v"(L.xw) :: "memory");
+    else
+      asm volatile("s_waitcnt vmcnt(0)" : "+v"(L.u[0]), "+v"(L.u[1]), "+v"(L.w[0]), "+v"(L.w[1]), "+v"(L.xu), "+v"(L.xw) :: "memory");
   };
   // Four plane buffers whose roles rotate over an unrolled loop of four steps
   // (no register copies, which would wait for in-flight loads): the step for
   // cubes between planes z and z + 1 loads plane z + 3.
   Plane4 A, B, C, D;
   load_plane(z0, A);
-  plane_signs(A);
   load_plane(z0 + 1, B);
   load_plane(z0 + 2, C);
+  asm volatile("s_waitcnt vmcnt(12)" : "+v"(A.u[0]), "+v"(A.u[1]), "+v"(A.w[0]), "+v"(A.w[1]), "+v"(A.xu), "+v"(A.xw) :: "memory");
+  plane_signs(A);
 
   int32_t used = 0;  // records of the column's quota taken
   uint64_t nzmask = 0;  // planes zb + i of this column whose tile has vertices
@@ -586,8 +611,10 @@ void count4_kernel(ExtractParams P) {
   const uint32_t quota = (uint32_t)P.quota;
   uint32_t t = (uint32_t)(V.tile_begin + (int64_t)z0 * V.R + r);  // this step's tile
   const uint32_t tR = (uint32_t)V.R;
+  // The plane two steps ahead is loaded at the end of a step, after the
+  // step's record stores: the data a step waits for is then always older
+  // than the newest loads (a fixed count), however many stores were issued.
   auto step = [&](Plane4& A, Plane4& B, Plane4& L, int z) __attribute__((always_inline)) {
-    load_plane(z + 3, L);
 
     // ---- pass 1: active masks m[c][k] (bit L: cube 256c + 4L + k).  Signs
     // first, four cubes per lane at once: byte k of O / N = some / every tsdf
@@ -599,6 +626,7 @@ void count4_kernel(ExtractParams P) {
     // row end are masked.  Only chunks with such cubes test the weights:
     // the minimum word over the eight corners below 0x10000 means some
     // weight is zero.
+    wait_plane(B, 6);
     plane_signs(B);
     uint64_t m[2][4];
     uint32_t nxo, nxn, nxm;  // position 0 of the next chunk's lane 0 (wave-uniform)
@@ -649,6 +677,7 @@ void count4_kernel(ExtractParams P) {
     if ((any_c0 | any_c1) == 0) {
       // no active cube: only the tile's vertex count (0) is read later
       if (lane == 0) P.tileT[t] = 0;
+      load_plane(z + 3, L);
       t += tR;
       if (z - zb == 63 || z + 1 == z1) {
         if (nzmask) {
@@ -783,6 +812,7 @@ void count4_kernel(ExtractParams P) {
       if (dense) atomicAdd(&P.hdr->dense_acc, 1u);
     }
     if (in_quota) used += nrec32;
+    load_plane(z + 3, L);
     t += tR;
     // append the column's non-empty tiles to the list, 64 planes at a time
     if (vsum > 0) nzmask |= 1ull << (z - zb);
@@ -912,11 +942,16 @@ static_assert(sizeof(TileInfo) == 40, "tile info is 10 words");
 // One output vertex: slot q of cube (x, y, z) with case ci and corner words
 // cw[8] in the reference corner order.  The 7 words of its vertex_voxel
 // record go to the wave's staging area o[0..6]; ver_voxel is written here.
+// node_off (optional): for every grid index i, floor((i + 0.5) h / h - 0.5) - i
+// in the same fp32 operations; along the two axes the edge does not run on
+// the vertex sits exactly on its corners' node coordinate, so only the edge
+// axis needs the division.
 template <int ELEM>
 __device__ __forceinline__ void make_vertex(const VolDesc& V, const uint64_t* s_case,
                                             const uint32_t cw[8], int x, int y, int z,
                                             uint32_t ci, int q, bool write_vv, uint32_t* o,
-                                            int64_t* key = nullptr) {
+                                            int64_t* key = nullptr,
+                                            const int8_t* node_off = nullptr) {
   // output slot q = 3*tri + r holds table corner 2 - r of triangle tri (:324-332)
   const int m = 3 * (q / 3) + 2 - (q % 3);
   const int e = (int)((s_case[ci] >> (4 * m)) & 0xfu);
@@ -945,11 +980,25 @@ __device__ __forceinline__ void make_vertex(const VolDesc& V, const uint64_t* s_
   const float py = __fmaf_rn(tt, pby - pay, pay);
   const float pz = __fmaf_rn(tt, pbz - paz, paz);
   // record fields (store_point :336-367)
-  float cx = px / h, cy = py / h, cz = pz / h;
-  cx = cx - 0.5f;
-  cy = cy - 0.5f;
-  cz = cz - 0.5f;
-  const int vx = (int)floorf(cx), vy = (int)floorf(cy), vz = (int)floorf(cz);
+  int vx, vy, vz;
+  if (node_off) {
+    const int ax = corner_dx(ca) != corner_dx(cb) ? 0 : (corner_dy(ca) != corner_dy(cb) ? 1 : 2);
+    float ce = (ax == 0 ? px : (ax == 1 ? py : pz)) / h;
+    ce = ce - 0.5f;
+    const int ve = (int)floorf(ce);
+    const int ix = x + corner_dx(ca), iy = y + corner_dy(ca), iz = z + corner_dz(ca);
+    vx = ax == 0 ? ve : ix + node_off[ix];
+    vy = ax == 1 ? ve : iy + node_off[iy];
+    vz = ax == 2 ? ve : iz + node_off[iz];
+  } else {
+    float cx = px / h, cy = py / h, cz = pz / h;
+    cx = cx - 0.5f;
+    cy = cy - 0.5f;
+    cz = cz - 0.5f;
+    vx = (int)floorf(cx);
+    vy = (int)floorf(cy);
+    vz = (int)floorf(cz);
+  }
   const bool inb = vx >= 0 && vy >= 0 && vz >= 0 && vx < V.X && vy < V.Y && vz < V.Z;
   uint32_t tw = 0;
   auto lin = [&]() { return ((int64_t)vz * V.Y + vy) * V.X + vx; };
@@ -975,8 +1024,14 @@ constexpr int kWaveRec = 64 * kRecWords;  // 64 records (cube offset << 8 | case
 constexpr int kWaveOut = 64 * 7;          // 64 staged vertex_voxel records
 constexpr int kWaveMap = 64 * 15 / 4;     // vertex -> record (u8), at most 15 vertices a record
 constexpr int kWaveEx = 64;               // first vertex of each record
-constexpr int kWaveMeta = 64 * 4;         // packed blocks: output index (2 words), p, z
-constexpr int kWaveWords = kWaveRec + kWaveOut + kWaveMap + kWaveEx + kWaveMeta;
+constexpr int kWaveWords = kWaveRec + kWaveOut + kWaveMap + kWaveEx;
+// Packed emission (single volumes): records, vertex -> record map, first
+// vertex of each record, and per record its output index (2 words), p and z.
+constexpr int kWaveMeta = 64 * 4;
+constexpr int kPackedMap = kWaveRec, kPackedEx = kPackedMap + kWaveMap;
+constexpr int kPackedMeta = kPackedEx + kWaveEx;
+constexpr int kPackedWords = kPackedMeta + kWaveMeta;
+constexpr int kNodeTable = 4096;  // node offsets for grids up to 4096 per axis
 
 // Emit the vertices of the (up to 64) records in the wave's LDS area s_rec
 // (record r at s_rec[9r]; n of them) to output indices out, out + 1, ...:
@@ -1051,11 +1106,12 @@ __device__ __forceinline__ int emit_records(const ExtractParams& P, const VolDes
 template <int ELEM>
 __device__ __forceinline__ void emit_block(const ExtractParams& P, const VolDesc& V,
                                            const uint64_t* s_case, const uint8_t* s_nv,
-                                           uint32_t* s_w, int nb) {
+                                           uint32_t* s_w, int nb,
+                                           const int8_t* node_off = nullptr) {
   const uint32_t* s_rec = s_w;
-  uint8_t* s_map = reinterpret_cast<uint8_t*>(s_w + kWaveRec + kWaveOut);
-  uint32_t* s_ex = s_w + kWaveRec + kWaveOut + kWaveMap;
-  const uint32_t* s_meta = s_w + kWaveRec + kWaveOut + kWaveMap + kWaveEx;
+  uint8_t* s_map = reinterpret_cast<uint8_t*>(s_w + kPackedMap);
+  uint32_t* s_ex = s_w + kPackedEx;
+  const uint32_t* s_meta = s_w + kPackedMeta;
   const int lane = threadIdx.x & 63;
   wave_sync();
   const int nv = lane < nb ? s_nv[s_rec[lane * kRecWords] & 0xffu] : 0;
@@ -1085,7 +1141,7 @@ __device__ __forceinline__ void emit_block(const ExtractParams& P, const VolDesc
         uint32_t o[7];
         int64_t key;
         make_vertex<ELEM>(V, s_case, c, x, (int)y, (int)mt[3], rw & 0xffu, slot, true, o,
-                          P.ekey ? &key : nullptr);
+                          P.ekey ? &key : nullptr, node_off);
         if (vv) {
 #pragma unroll
           for (int f = 0; f < 7; ++f) vv[7 * out + f] = o[f];
@@ -1117,18 +1173,36 @@ template <int ELEM>
 __global__ __launch_bounds__(kEmitThreads) void emit_packed_kernel(ExtractParams P) {
   __shared__ uint64_t s_case[256];
   __shared__ uint8_t s_nv[256];
-  __shared__ uint32_t s_wave[kEmitThreads / 64][kWaveWords];
+  __shared__ uint32_t s_wave[kEmitThreads / 64][kPackedWords];
   __shared__ uint32_t s_mark[kEmitThreads / 64][64];
+  __shared__ int8_t s_node[kNodeTable];
+  __shared__ int s_node_ok;
   s_case[threadIdx.x] = dCase[threadIdx.x];
   s_nv[threadIdx.x] = dNumVerts[threadIdx.x];
   const int lane = threadIdx.x & 63;
   const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
   s_mark[wv][lane] = 0;
-  __syncthreads();  // the only workgroup barrier
+  const VolDesc& V = P.vol0;
+  const int mdim = max(V.X, max(V.Y, V.Z));
+  if (threadIdx.x == 0) s_node_ok = mdim <= kNodeTable;
+  __syncthreads();
+  if (mdim <= kNodeTable) {
+    for (int i = threadIdx.x; i < mdim; i += kEmitThreads) {
+      const float n = ((float)i + 0.5f) * V.h;
+      float c = n / V.h;
+      c = c - 0.5f;
+      const float d = floorf(c) - (float)i;  // 0 or -1 for any usable grid size
+      if (d >= -2.0f && d <= 2.0f)
+        s_node[i] = (int8_t)(int)d;
+      else
+        s_node_ok = 0;
+    }
+  }
+  __syncthreads();  // the last workgroup barrier
+  const int8_t* node_off = s_node_ok ? s_node : nullptr;
   uint32_t* const s_w = s_wave[wv];
   uint32_t* const mark = s_mark[wv];
-  uint32_t* const s_meta = s_w + kWaveRec + kWaveOut + kWaveMap + kWaveEx;
-  const VolDesc& V = P.vol0;
+  uint32_t* const s_meta = s_w + kPackedMeta;
   const uint32_t nz = P.hdr->nz_total;
   const uint32_t nwaves = gridDim.x * (kEmitThreads / 64);
   const uint32_t wave0 = blockIdx.x * (kEmitThreads / 64) + (uint32_t)wv;
@@ -1189,7 +1263,7 @@ __global__ __launch_bounds__(kEmitThreads) void emit_packed_kernel(ExtractParams
               mt[3] = (uint32_t)z;
             }
             out += (int64_t)__builtin_amdgcn_readlane(vincl, 63);
-            emit_block<ELEM>(P, V, s_case, s_nv, s_w, n);
+            emit_block<ELEM>(P, V, s_case, s_nv, s_w, n, node_off);
           }
         } else {
           const int X = V.X;
@@ -1210,16 +1284,25 @@ __global__ __launch_bounds__(kEmitThreads) void emit_packed_kernel(ExtractParams
                 ci = cube_index<ELEM>(cw);
               }
             }
-            const uint64_t act = __ballot(s_nv[ci] != 0);
+            const uint32_t nvc = s_nv[ci];
+            const uint64_t act = __ballot(nvc != 0);
             const int n = __popcll(act);
+            const uint32_t vincl = wave_incl_scan32(nvc);  // lane order = record order
             if ((act >> lane) & 1ull) {
-              uint32_t* rp = s_w + __popcll(act & ((1ull << lane) - 1ull)) * kRecWords;
+              const int rk = __popcll(act & ((1ull << lane) - 1ull));
+              uint32_t* rp = s_w + rk * kRecWords;
               rp[0] = ((uint32_t)o << 8) | ci;
 #pragma unroll
               for (int q = 0; q < 8; ++q) rp[1 + q] = cw[q];
+              const uint64_t ob = (uint64_t)(out + (int64_t)(vincl - nvc));
+              uint32_t* mt = s_meta + 4 * rk;
+              mt[0] = (uint32_t)ob;
+              mt[1] = (uint32_t)(ob >> 32);
+              mt[2] = (uint32_t)p;
+              mt[3] = (uint32_t)z;
             }
-            wave_sync();
-            if (n) out += emit_records<ELEM>(P, V, s_case, s_nv, s_w, n, z, p0, out);
+            out += (int64_t)__builtin_amdgcn_readlane(vincl, 63);
+            if (n) emit_block<ELEM>(P, V, s_case, s_nv, s_w, n, node_off);
           }
         }
         ++a;
@@ -1264,7 +1347,7 @@ __global__ __launch_bounds__(kEmitThreads) void emit_packed_kernel(ExtractParams
         mt[2] = p0 + (rec[0] >> 8);
         mt[3] = z;
       }
-      emit_block<ELEM>(P, V, s_case, s_nv, s_w, R);
+      emit_block<ELEM>(P, V, s_case, s_nv, s_w, R, node_off);
       a = b;
     }
   }
@@ -1292,7 +1375,6 @@ __global__ __launch_bounds__(kEmitThreads) void emit_kernel(ExtractParams P) {
   const uint32_t nz = P.hdr->nz_total;
   const uint32_t nwaves = gridDim.x * (kEmitThreads / 64);
   const uint32_t wave0 = blockIdx.x * (kEmitThreads / 64) + (uint32_t)wv;
-  int nblk = 0;  // records in the packed block (single volumes)
   // this wave's items: wave0 + i * nwaves, in batches of 64
   for (uint32_t ib = 0; wave0 + ib * nwaves < nz; ib += 64) {
     const uint32_t my_item = wave0 + (ib + (uint32_t)lane) * nwaves;
@@ -1334,43 +1416,7 @@ __global__ __launch_bounds__(kEmitThreads) void emit_kernel(ExtractParams P) {
       const int64_t local = t - V.tile_begin;
       const int z = (int)V.divR.div((uint32_t)local);
       const int64_t p0 = (local - (int64_t)z * V.R) * kTile;
-      if (nrec >= 0 && !BATCH) {
-        // append the tile's records to the packed block (emitted when full)
-        uint32_t* s_meta = s_w + kWaveRec + kWaveOut + kWaveMap + kWaveEx;
-        for (int rb = 0; rb < nrec; rb += 64) {
-          const int n = min(64, nrec - rb);
-          if (nblk + n > 64) {
-            emit_block<ELEM>(P, V, s_case, s_nv, s_w, nblk);
-            nblk = 0;
-          }
-          uint32_t* dst = s_w + nblk * kRecWords;
-          if (rb == 0) {
-#pragma unroll
-            for (int q = 0; q < kRecWords; ++q)
-              if (lane + 64 * q < n * kRecWords) dst[lane + 64 * q] = cur[q];
-          } else {
-            const uint32_t* src = P.rec + (rs + rb) * kRecWords;
-            for (int w = lane; w < n * kRecWords; w += 64) dst[w] = src[w];
-          }
-          wave_sync();
-          uint32_t nv = 0, rw = 0;
-          if (lane < n) {
-            rw = dst[lane * kRecWords];
-            nv = s_nv[rw & 0xffu];
-          }
-          const uint32_t incl = wave_incl_scan32(nv);
-          if (lane < n) {
-            const uint64_t o = (uint64_t)(out + (int64_t)(incl - nv));
-            uint32_t* mt = s_meta + 4 * (nblk + lane);
-            mt[0] = (uint32_t)o;
-            mt[1] = (uint32_t)(o >> 32);
-            mt[2] = (uint32_t)(p0 + (rw >> 8));
-            mt[3] = (uint32_t)z;
-          }
-          out += (int64_t)__builtin_amdgcn_readlane(incl, 63);
-          nblk += n;
-        }
-      } else if (nrec >= 0) {
+      if (nrec >= 0) {
         for (int rb = 0; rb < nrec; rb += 64) {
           const int n = min(64, nrec - rb);
           if (rb == 0) {
@@ -1384,10 +1430,6 @@ __global__ __launch_bounds__(kEmitThreads) void emit_kernel(ExtractParams P) {
           out += emit_records<ELEM>(P, V, s_case, s_nv, s_w, n, z, p0, out);
         }
       } else {
-        if (nblk) {
-          emit_block<ELEM>(P, V, s_case, s_nv, s_w, nblk);
-          nblk = 0;
-        }
         const int X = V.X;
         const int64_t XY = (int64_t)X * V.Y;
         const uint32_t* vol = V.vol + (int64_t)z * XY;
@@ -1420,9 +1462,6 @@ __global__ __launch_bounds__(kEmitThreads) void emit_kernel(ExtractParams P) {
         }
       }
     }
-  }
-  if constexpr (!BATCH) {
-    if (nblk) emit_block<ELEM>(P, vol_at<false>(P, 0), s_case, s_nv, s_w, nblk);
   }
 }
 
