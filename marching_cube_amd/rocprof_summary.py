@@ -1,7 +1,12 @@
 """Summaries of rocprofv3 output for the profiles/ folder.
 
     python -m marching_cube_amd.rocprof_summary stats  <kernel_stats.csv> [--match extract]
+    python -m marching_cube_amd.rocprof_summary window <kernel_trace.csv> --first F --count N
     python -m marching_cube_amd.rocprof_summary traffic <fetch_dir> <write_dir> --out X.json
+
+`window` averages, per extraction kernel, the dispatches F .. F+N-1 in
+launch order (a bench's timed region: it skips the sizing, checking and
+warm-up calls before it and the calls after it).
 
 `traffic` reads two `rocprofv3 --pmc FETCH_SIZE` / `--pmc WRITE_SIZE` runs of
 the same command (``pm_counter_collection.csv``), scales each counter by what
@@ -16,7 +21,8 @@ import json
 import os
 import sys
 
-EXTRACT_KERNELS = ("count4_kernel", "count_kernel", "scan_kernel", "emit_kernel")
+EXTRACT_KERNELS = ("count4_kernel", "count_kernel", "scan_kernel", "emit_kernel",
+                   "emit_packed_kernel")
 COPY_BYTES = 512 << 20
 
 
@@ -30,6 +36,27 @@ def kernel_stats(path, match=None):
             rows.append((name, int(r["Calls"]), float(r["AverageNs"]) / 1e3,
                          float(r["TotalDurationNs"]) / 1e3))
     return rows
+
+
+def window(path, first, count, match=EXTRACT_KERNELS):
+    """Per kernel: durations (us) of dispatches first .. first+count-1 in
+    start order."""
+    per = collections.defaultdict(list)
+    with open(path) as f:
+        for r in csv.DictReader(f):
+            name = r["Kernel_Name"]
+            if any(m in name for m in match):
+                per[name].append((int(r["Start_Timestamp"]), int(r["End_Timestamp"])))
+    out = []
+    for name, ds in per.items():
+        ds.sort()
+        us = [(e - b) / 1e3 for b, e in ds[first:first + count]]
+        if us:
+            us_s = sorted(us)
+            out.append((name, len(ds), len(us), sum(us) / len(us), us_s[len(us_s) // 2],
+                        us_s[0], us_s[-1], ds[first][0]))
+    out.sort(key=lambda r: r[-1])
+    return out
 
 
 def _per_kernel(path, counter):
@@ -79,8 +106,8 @@ def traffic(fetch_dir, write_dir):
             continue
         fv = fetch.get(name, [])
         wv = write.get(name, [])
-        fb = (sum(fv) / len(fv)) * 1024.0 * f_scale if fv else 0.0
-        wb = (sum(wv) / len(wv)) * 1024.0 * w_scale if wv else 0.0
+        fb = sorted(fv)[len(fv) // 2] * 1024.0 * f_scale if fv else 0.0  # median dispatch
+        wb = sorted(wv)[len(wv) // 2] * 1024.0 * w_scale if wv else 0.0
         out["kernels"][short] = {"dispatches": len(fv), "fetch_bytes": round(fb),
                                  "write_bytes": round(wb)}
         total += fb + wb
@@ -94,6 +121,10 @@ def main(argv=None):
     a = sub.add_parser("stats")
     a.add_argument("csv")
     a.add_argument("--match", nargs="*")
+    w = sub.add_parser("window")
+    w.add_argument("csv")
+    w.add_argument("--first", type=int, required=True)
+    w.add_argument("--count", type=int, required=True)
     b = sub.add_parser("traffic")
     b.add_argument("fetch_dir")
     b.add_argument("write_dir")
@@ -105,6 +136,16 @@ def main(argv=None):
         print("|---|---|---|---|")
         for name, calls, avg, tot in kernel_stats(args.csv, args.match):
             print(f"| {_short(name)} | {calls} | {avg:.2f} | {tot:.1f} |")
+        return 0
+    if args.cmd == "window":
+        rows = window(args.csv, args.first, args.count)
+        print("| kernel | dispatches in run | in window | avg us | median us | min us | max us |")
+        print("|---|---|---|---|---|---|---|")
+        tot = 0.0
+        for name, n, k, avg, med, lo, hi, _ in rows:
+            print(f"| {_short(name)} | {n} | {k} | {avg:.2f} | {med:.2f} | {lo:.2f} | {hi:.2f} |")
+            tot += avg
+        print(f"\nsum of the window averages: {tot:.2f} us")
         return 0
     res = traffic(args.fetch_dir, args.write_dir)
     res.update(json.loads(args.meta))
