@@ -54,7 +54,8 @@ __constant__ unsigned char dNumVerts[256] = MCX_NUM_VERTS_INIT;
 constexpr int kCPW = 8;                // 64-cube chunks per wave and plane
 constexpr int kTile = 64 * kCPW;       // cubes per tile (one plane of a column)
 constexpr int kRecWords = 9;           // record: (offset << 8 | case), 8 corner words
-constexpr int kScanItems = 4096;       // tiles per scan block (256 threads x 16)
+constexpr int kScanPer = 4;            // tiles per scan thread
+constexpr int kScanItems = 256 * kScanPer;  // tiles per scan block
 constexpr int kEmitThreads = 256;
 // Launch-shape tunables (process-wide, set by mcx_set_tuning / the
 // autotuner; read by every launch on the host): K1 columns (waves) per CU for
@@ -130,6 +131,14 @@ struct ExtractParams {
   int64_t* voff;         // exclusive vertex offset per tile
   uint64_t* masks;       // active-cube bits, kCPW words per tile
   uint32_t* nz_list;     // ids of the tiles with vertices (in append order)
+  // emission in units of 64 records (single volumes): the scan also scans
+  // the records of the list's tiles in list order
+  bool list_scan;
+  uint32_t* clist;       // the list's tiles with records, in list order
+  int64_t* lpre;         // [n + 1] first record of clist entry j (lpre[n]: all records)
+  int64_t* lrs;          // tile_rec of clist entry j
+  uint32_t* units;       // clist entry holding record 64 k, for every unit k
+  uint32_t* dense_list;  // the list's tiles that took the staged path
   bool write_masks;
   uint32_t* rec;
   int64_t rec_cap;
@@ -209,9 +218,13 @@ __device__ __forceinline__ int volume_of_tile(const ExtractParams& P, int64_t t)
 // workspace depends on the volume size, so a workspace reused with other
 // sizes holds stale flags there.  Workgroup 0 of the count kernel (which
 // completes before the scan starts) zeroes them.
+// Zeroes both look-back descriptor sets (tiles, list) and the staged-tile
+// list count; the previous call's emit kernel has finished (same stream).
 __device__ __forceinline__ void clear_scan_descriptors(const ExtractParams& P) {
-  if (blockIdx.x == 0)
-    for (int64_t i = threadIdx.x; i < P.cap_desc; i += blockDim.x) P.desc[i] = 0;
+  if (blockIdx.x == 0) {
+    for (int64_t i = threadIdx.x; i < 3 * P.cap_desc; i += blockDim.x) P.desc[i] = 0;
+    if (threadIdx.x == 0) P.hdr->dense_n = 0;
+  }
 }
 
 // ============================================================ K1: count
@@ -840,44 +853,117 @@ void count4_kernel(ExtractParams P) {
 }
 
 // ============================================================ K2: scan
+// Exclusive scan of the tile vertex counts (tile order) into voff, single
+// pass with decoupled look-back over blocks of 1024 tiles; a block publishes
+// its tile aggregate as soon as its counts are in.  With list_scan, the
+// blocks then split the non-empty-tile list evenly (contiguous ranges, list
+// order): tiles that took the staged path move to dense_list, the others
+// are compacted into clist with their first record (lrs) and the exclusive
+// scan of their record counts (lpre), and units[k] is set to the clist entry
+// holding record 64 k.  The three scans have descriptor sets of their own,
+// looked back by waves 0, 1 and 2.
 __global__ __launch_bounds__(256) void scan_kernel(ExtractParams P) {
-  __shared__ int64_t s_wave[4];
-  __shared__ int64_t s_excl;
+  __shared__ int64_t s_wave[3][4];
+  __shared__ int64_t s_excl[3];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int64_t blk = blockIdx.x;
-  const int64_t i0 = blk * kScanItems + (int64_t)tid * 16;
-  int32_t v[16];
+  const int64_t i0 = blk * kScanItems + (int64_t)tid * kScanPer;
+  // read before this block publishes anything: the last block resets the
+  // counter once every block has published
+  const uint32_t nz = P.hdr->nz_count;
+
+  // ---- tiles: one 128-bit load of the thread's 4 counts (i0 is a multiple
+  // of 4, the array is 256-byte aligned), element loads at the end
+  static_assert(kScanPer == 4, "one int4 per thread");
+  int32_t v[kScanPer];
+  if (i0 + kScanPer <= P.n_tiles) {
+    const int4 w = *reinterpret_cast<const int4*>(P.tileT + i0);
+    v[0] = w.x;
+    v[1] = w.y;
+    v[2] = w.z;
+    v[3] = w.w;
+  } else {
 #pragma unroll
-  for (int k = 0; k < 16; ++k) v[k] = i0 + k < P.n_tiles ? P.tileT[i0 + k] : 0;
-  int64_t s = 0;
+    for (int k = 0; k < kScanPer; ++k) v[k] = i0 + k < P.n_tiles ? P.tileT[i0 + k] : 0;
+  }
+  int64_t st = 0;
 #pragma unroll
-  for (int k = 0; k < 16; ++k) s += v[k];
-  const uint64_t incl = wave_incl_scan((uint64_t)s);
-  if (lane == 63) s_wave[wave] = (int64_t)incl;
+  for (int k = 0; k < kScanPer; ++k) st += v[k];
+  const uint64_t incl0 = wave_incl_scan((uint64_t)st);
+  if (lane == 63) s_wave[0][wave] = (int64_t)incl0;
   __syncthreads();
-  int64_t wpre = 0, total = 0;
+  int64_t texcl0 = (int64_t)incl0 - st, total0 = 0;
 #pragma unroll
   for (int w = 0; w < 4; ++w) {
-    if (w < wave) wpre += s_wave[w];
-    total += s_wave[w];
+    if (w < wave) texcl0 += s_wave[0][w];
+    total0 += s_wave[0][w];
   }
-  const int64_t texcl = (int64_t)incl - s + wpre;
   // the descriptors were zeroed by the count kernel of this call
-  uint64_t* desc = P.desc;
-  if (wave == 0) {
-    int64_t excl = 0;
-    if (blk == 0) {
-      if (lane == 0) st_agent(desc, kStatInc | (uint64_t)total);
-    } else {
-      if (lane == 0) st_agent(desc + blk, kStatAgg | (uint64_t)total);
-      excl = lookback<1>(desc, blk, &P.hdr->error);
-      if (lane == 0) st_agent(desc + blk, kStatInc | (uint64_t)(excl + total));
+  if (wave == 0 && lane == 0)
+    st_agent(P.desc + blk, (blk == 0 ? kStatInc : kStatAgg) | (uint64_t)total0);
+
+  // ---- list: this block's contiguous range, c entries per thread
+  const bool lst = P.list_scan && nz > 0;
+  const int64_t L = lst ? ((int64_t)nz + gridDim.x - 1) / gridDim.x : 0;  // <= kScanItems
+  const int c = (int)((L + 255) / 256);                                     // <= kScanPer
+  const int64_t lb = min(blk * L, (int64_t)nz), le = min(lb + L, (int64_t)nz);
+  const int64_t j0 = lb + (int64_t)tid * c;
+  uint32_t tl[kScanPer];
+  int32_t n[kScanPer];
+  int64_t rsl[kScanPer];
+  uint32_t keep = 0;  // bit k: entry j0 + k has records
+  int64_t sn = 0, se = 0;
+  if (lst) {
+    // every load issued before any is waited for: clamped indices, the
+    // entries outside the range read a valid one and are dropped
+#pragma unroll
+    for (int k = 0; k < kScanPer; ++k) tl[k] = P.nz_list[min(j0 + k, (int64_t)nz - 1)];
+#pragma unroll
+    for (int k = 0; k < kScanPer; ++k) {
+      n[k] = P.tile_nrec[tl[k]];
+      rsl[k] = P.tile_rec[tl[k]];
     }
+#pragma unroll
+    for (int k = 0; k < kScanPer; ++k) {
+      if (k < c && j0 + k < le) {
+        if (n[k] < 0) {  // the staged path (rare): off the list
+          P.dense_list[atomicAdd(&P.hdr->dense_n, 1u)] = tl[k];
+          n[k] = 0;
+        } else {
+          keep |= 1u << k;
+        }
+      } else {
+        n[k] = 0;
+      }
+      sn += n[k];
+    }
+    se = __popc(keep);
+  }
+  const uint64_t incl1 = wave_incl_scan((uint64_t)sn);
+  const uint64_t incl2 = wave_incl_scan((uint64_t)se);
+  if (lane == 63) {
+    s_wave[1][wave] = (int64_t)incl1;
+    s_wave[2][wave] = (int64_t)incl2;
+  }
+  __syncthreads();
+  int64_t texcl1 = (int64_t)incl1 - sn, texcl2 = (int64_t)incl2 - se, total1 = 0, total2 = 0;
+#pragma unroll
+  for (int w = 0; w < 4; ++w) {
+    if (w < wave) {
+      texcl1 += s_wave[1][w];
+      texcl2 += s_wave[2][w];
+    }
+    total1 += s_wave[1][w];
+    total2 += s_wave[2][w];
+  }
+  if (wave == 0) {
+    const int64_t excl = blk == 0 ? 0 : lookback<1>(P.desc, blk, &P.hdr->error);
     if (lane == 0) {
-      s_excl = excl;
+      if (blk > 0) st_agent(P.desc + blk, kStatInc | (uint64_t)(excl + total0));
+      s_excl[0] = excl;
       if (blk == (int64_t)gridDim.x - 1) {
-        P.hdr->count = excl + total;
-        if (P.d_count) *P.d_count = excl + total;
+        P.hdr->count = excl + total0;
+        if (P.d_count) *P.d_count = excl + total0;
         P.hdr->dense_last = P.hdr->dense_acc;  // K1 has finished: hand the count over
         P.hdr->dense_acc = 0;
         P.hdr->nz_total = P.hdr->nz_count;     // the emit kernel's work list
@@ -885,13 +971,44 @@ __global__ __launch_bounds__(256) void scan_kernel(ExtractParams P) {
         P.hdr->ovf_next = 0;
       }
     }
+  } else if (lst && wave < 3) {
+    uint64_t* desc = P.desc + wave * P.cap_desc;
+    const int64_t tot = wave == 1 ? total1 : total2;
+    int64_t excl = 0;
+    if (blk == 0) {
+      if (lane == 0) st_agent(desc, kStatInc | (uint64_t)tot);
+    } else {
+      if (lane == 0) st_agent(desc + blk, kStatAgg | (uint64_t)tot);
+      excl = lookback<1>(desc, blk, &P.hdr->error);
+      if (lane == 0) st_agent(desc + blk, kStatInc | (uint64_t)(excl + tot));
+    }
+    if (lane == 0) s_excl[wave] = excl;
   }
   __syncthreads();
-  int64_t run = s_excl + texcl;
+  int64_t run = s_excl[0] + texcl0;
 #pragma unroll
-  for (int k = 0; k < 16; ++k) {
+  for (int k = 0; k < kScanPer; ++k) {
     if (i0 + k < P.n_tiles) P.voff[i0 + k] = run;
     run += v[k];
+  }
+  if (lst) {
+    int64_t r = s_excl[1] + texcl1;
+    int64_t e = s_excl[2] + texcl2;
+#pragma unroll
+    for (int k = 0; k < kScanPer; ++k) {
+      if ((keep >> k) & 1u) {
+        P.clist[e] = tl[k];
+        P.lpre[e] = r;
+        P.lrs[e] = rsl[k];
+        for (int64_t u = (r + 63) >> 6; (u << 6) < r + n[k]; ++u) P.units[u] = (uint32_t)e;
+        r += n[k];
+        ++e;
+      }
+      if (k < c && j0 + k == (int64_t)nz - 1) {  // the list's last entry: its totals
+        P.lpre[e] = r;
+        P.hdr->list_n = (uint32_t)e;
+      }
+    }
   }
 }
 
@@ -1161,16 +1278,72 @@ __device__ __forceinline__ void emit_block(const ExtractParams& P, const VolDesc
   wave_sync();
 }
 
-// Single volumes: the wave's tiles (list entries wave0, wave0 + nwaves, ...)
-// are taken in runs whose records fit one 64-record block: lane i loads
-// record i of the run (all loads in flight at once; lane i finds its tile
-// by a max-scan over the tiles' first-record marks), the records' vertex
-// counts are scanned, and each record's output index is its tile's first
-// vertex plus the vertices of the tile's earlier records.  A tile with more
-// than 64 records goes in blocks of 64 of its own; a tile whose records did
-// not fit the scratch is re-read from the volume.
+// A tile whose records did not fit the scratch, re-read from the volume 64
+// cubes at a time (cube order) into the record form and emitted from voff[t].
 template <int ELEM>
-__global__ __launch_bounds__(kEmitThreads) void emit_packed_kernel(ExtractParams P) {
+__device__ __forceinline__ void emit_staged_tile(const ExtractParams& P, const VolDesc& V,
+                                                 const uint64_t* s_case, const uint8_t* s_nv,
+                                                 uint32_t* s_w, uint32_t t,
+                                                 const int8_t* node_off) {
+  const int lane = threadIdx.x & 63;
+  uint32_t* const s_meta = s_w + kPackedMeta;
+  const uint32_t local = (uint32_t)(t - V.tile_begin);
+  const int z = (int)V.divR.div(local);
+  const int64_t p0 = (int64_t)(local - (uint32_t)z * (uint32_t)V.R) * kTile;
+  int64_t out = P.voff[t];
+  const int X = V.X;
+  const int64_t XY = (int64_t)X * V.Y;
+  const uint32_t* vol = V.vol + (int64_t)z * XY;
+  for (int cb = 0; cb < kTile; cb += 64) {
+    const int o = cb + lane;
+    const int64_t p = p0 + o;
+    uint32_t cw[8] = {}, ci = 0;
+    if (p < XY) {
+      const uint32_t y = V.divX.div((uint32_t)p);
+      const uint32_t x = (uint32_t)p - y * (uint32_t)X;
+      if (x + 1 < (uint32_t)X && y + 1 < (uint32_t)V.Y) {
+        const uint32_t* pa = vol + p;
+        const uint32_t* pb = pa + XY;
+        cw[0] = pa[0]; cw[1] = pa[1]; cw[2] = pa[X + 1]; cw[3] = pa[X];
+        cw[4] = pb[0]; cw[5] = pb[1]; cw[6] = pb[X + 1]; cw[7] = pb[X];
+        ci = cube_index<ELEM>(cw);
+      }
+    }
+    const uint32_t nvc = s_nv[ci];
+    const uint64_t act = __ballot(nvc != 0);
+    const int n = __popcll(act);
+    const uint32_t vincl = wave_incl_scan32(nvc);  // lane order = record order
+    if ((act >> lane) & 1ull) {
+      const int rk = __popcll(act & ((1ull << lane) - 1ull));
+      uint32_t* rp = s_w + rk * kRecWords;
+      rp[0] = ((uint32_t)o << 8) | ci;
+#pragma unroll
+      for (int q = 0; q < 8; ++q) rp[1 + q] = cw[q];
+      const uint64_t ob = (uint64_t)(out + (int64_t)(vincl - nvc));
+      uint32_t* mt = s_meta + 4 * rk;
+      mt[0] = (uint32_t)ob;
+      mt[1] = (uint32_t)(ob >> 32);
+      mt[2] = (uint32_t)p;
+      mt[3] = (uint32_t)z;
+    }
+    out += (int64_t)__builtin_amdgcn_readlane(vincl, 63);
+    if (n) emit_block<ELEM>(P, V, s_case, s_nv, s_w, n, node_off);
+  }
+}
+
+// Single volumes, balanced by records: the records of the list's tiles, in
+// list order, are cut into units of 64 (the scan's clist / lpre / units
+// tables) and wave w takes units w, w + nwaves, ..., then the staged-path
+// tiles.  In a unit, lane l holds clist entry units[k] + l (every entry has
+// records, so the at most 64 entries overlapping the unit are consecutive;
+// the first may start before it); each entry marks its
+// first record in the unit and a max-scan gives every record lane its entry.
+// A record's output index is its tile's first vertex, plus the vertices of
+// the tile's records before it: those in the unit come from a scan of the
+// unit's vertex counts, those before the unit (first entry only) are summed
+// from their case bytes.
+template <int ELEM>
+__global__ __launch_bounds__(kEmitThreads) void emit_units_kernel(ExtractParams P) {
   __shared__ uint64_t s_case[256];
   __shared__ uint8_t s_nv[256];
   __shared__ uint32_t s_wave[kEmitThreads / 64][kPackedWords];
@@ -1188,8 +1361,8 @@ __global__ __launch_bounds__(kEmitThreads) void emit_packed_kernel(ExtractParams
   __syncthreads();
   if (mdim <= kNodeTable) {
     for (int i = threadIdx.x; i < mdim; i += kEmitThreads) {
-      const float n = ((float)i + 0.5f) * V.h;
-      float c = n / V.h;
+      const float nd = ((float)i + 0.5f) * V.h;
+      float c = nd / V.h;
       c = c - 0.5f;
       const float d = floorf(c) - (float)i;  // 0 or -1 for any usable grid size
       if (d >= -2.0f && d <= 2.0f)
@@ -1203,153 +1376,89 @@ __global__ __launch_bounds__(kEmitThreads) void emit_packed_kernel(ExtractParams
   uint32_t* const s_w = s_wave[wv];
   uint32_t* const mark = s_mark[wv];
   uint32_t* const s_meta = s_w + kPackedMeta;
-  const uint32_t nz = P.hdr->nz_total;
-  const uint32_t nwaves = gridDim.x * (kEmitThreads / 64);
-  const uint32_t wave0 = blockIdx.x * (kEmitThreads / 64) + (uint32_t)wv;
-  for (uint32_t ib = 0; wave0 + ib * nwaves < nz; ib += 64) {
-    const uint32_t my_item = wave0 + (ib + (uint32_t)lane) * nwaves;
-    const bool have = my_item < nz;
-    const int nb = __popcll(__ballot(have));
-    int64_t off_l = 0, rs_l = 0;
-    int nrec_l = 0;
-    uint32_t p0_l = 0, z_l = 0;
-    if (have) {
-      const uint32_t t = P.nz_list[my_item];
-      nrec_l = P.tile_nrec[t];
-      off_l = P.voff[t];
-      rs_l = P.tile_rec[t];
+  const uint32_t nl = P.hdr->nz_total ? P.hdr->list_n : 0u;  // clist entries
+  const int64_t total = nl ? P.lpre[nl] : 0;  // records of the list
+  const int64_t n_units = (total + 63) >> 6;
+  const int64_t n_work = n_units + (int64_t)P.hdr->dense_n;
+  const int64_t nwaves = (int64_t)gridDim.x * (kEmitThreads / 64);
+  for (int64_t k = (int64_t)blockIdx.x * (kEmitThreads / 64) + wv; k < n_work; k += nwaves) {
+    if (k >= n_units) {
+      emit_staged_tile<ELEM>(P, V, s_case, s_nv, s_w, P.dense_list[k - n_units], node_off);
+      continue;
+    }
+    const int64_t g0 = k << 6;
+    const int R = (int)min((int64_t)64, total - g0);
+    // lane l: clist entry units[k] + l, its records [pre, pre + nrec)
+    const uint32_t i = P.units[k] + (uint32_t)lane;
+    const uint32_t ic = min(i, nl);
+    const int64_t pre = P.lpre[ic];
+    const int64_t nrec = i < nl ? P.lpre[ic + 1] - pre : 0;
+    const uint32_t t = i < nl ? P.clist[i] : 0u;
+    const int64_t trec = i < nl ? P.lrs[i] : 0;  // the tile's first record
+    const bool inu = nrec > 0 && pre < g0 + R;  // the entry has records in the unit
+    const int first = (int)max(pre - g0, (int64_t)0);  // its first record in the unit
+    int64_t rs = 0, off = 0;
+    uint32_t p0 = 0, z = 0;
+    if (inu) {
+      rs = trec + (g0 + first - pre);  // the record of unit position `first`
+      off = P.voff[t];
       const uint32_t local = (uint32_t)(t - V.tile_begin);
-      z_l = V.divR.div(local);
-      p0_l = (local - z_l * (uint32_t)V.R) * (uint32_t)kTile;
+      z = V.divR.div(local);
+      p0 = (local - z * (uint32_t)V.R) * (uint32_t)kTile;
+      mark[first] = (uint32_t)lane + 1u;
     }
-    int a = 0;
-    while (a < nb) {
-      // the run [a, b): whole tiles of at most 64 records, 64 in all
-      const bool small = nrec_l >= 0 && nrec_l <= 64;
-      const uint32_t v = lane < a ? 0u : (lane < nb && small ? (uint32_t)nrec_l : 65u);
-      const uint32_t incl = wave_incl_scan32(v);
-      const uint64_t over = __ballot(lane >= a && incl > 64u);
-      const int b = over ? __ffsll((unsigned long long)over) - 1 : 64;
-      if (b == a) {
-        // one tile on its own: more than 64 records, or re-read from the volume
-        const int nrec = __shfl(nrec_l, a);
-        int64_t out = __shfl(off_l, a);
-        const int64_t rs = __shfl(rs_l, a);
-        const int z = (int)__shfl(z_l, a);
-        const int64_t p0 = (int64_t)__shfl(p0_l, a);
-        if (nrec >= 0) {
-          // 64 records at a time, each block emitted like a run
-          for (int rb = 0; rb < nrec; rb += 64) {
-            const int n = min(64, nrec - rb);
-            uint32_t rec[kRecWords];
-            if (lane < n) {
-              const uint32_t* src = P.rec + (rs + rb + lane) * kRecWords;
-#pragma unroll
-              for (int q = 0; q < kRecWords; ++q) rec[q] = src[q];
-            } else {
-#pragma unroll
-              for (int q = 0; q < kRecWords; ++q) rec[q] = 0;
-            }
-            const uint32_t nv = lane < n ? s_nv[rec[0] & 0xffu] : 0u;
-            const uint32_t vincl = wave_incl_scan32(nv);
-            if (lane < n) {
-#pragma unroll
-              for (int q = 0; q < kRecWords; ++q) s_w[lane * kRecWords + q] = rec[q];
-              const uint64_t o = (uint64_t)(out + (int64_t)(vincl - nv));
-              uint32_t* mt = s_meta + 4 * lane;
-              mt[0] = (uint32_t)o;
-              mt[1] = (uint32_t)(o >> 32);
-              mt[2] = (uint32_t)p0 + (rec[0] >> 8);
-              mt[3] = (uint32_t)z;
-            }
-            out += (int64_t)__builtin_amdgcn_readlane(vincl, 63);
-            emit_block<ELEM>(P, V, s_case, s_nv, s_w, n, node_off);
-          }
-        } else {
-          const int X = V.X;
-          const int64_t XY = (int64_t)X * V.Y;
-          const uint32_t* vol = V.vol + (int64_t)z * XY;
-          for (int cb = 0; cb < kTile; cb += 64) {
-            const int o = cb + lane;
-            const int64_t p = p0 + o;
-            uint32_t cw[8] = {}, ci = 0;
-            if (p < XY) {
-              const uint32_t y = V.divX.div((uint32_t)p);
-              const uint32_t x = (uint32_t)p - y * (uint32_t)X;
-              if (x + 1 < (uint32_t)X && y + 1 < (uint32_t)V.Y) {
-                const uint32_t* pa = vol + p;
-                const uint32_t* pb = pa + XY;
-                cw[0] = pa[0]; cw[1] = pa[1]; cw[2] = pa[X + 1]; cw[3] = pa[X];
-                cw[4] = pb[0]; cw[5] = pb[1]; cw[6] = pb[X + 1]; cw[7] = pb[X];
-                ci = cube_index<ELEM>(cw);
-              }
-            }
-            const uint32_t nvc = s_nv[ci];
-            const uint64_t act = __ballot(nvc != 0);
-            const int n = __popcll(act);
-            const uint32_t vincl = wave_incl_scan32(nvc);  // lane order = record order
-            if ((act >> lane) & 1ull) {
-              const int rk = __popcll(act & ((1ull << lane) - 1ull));
-              uint32_t* rp = s_w + rk * kRecWords;
-              rp[0] = ((uint32_t)o << 8) | ci;
-#pragma unroll
-              for (int q = 0; q < 8; ++q) rp[1 + q] = cw[q];
-              const uint64_t ob = (uint64_t)(out + (int64_t)(vincl - nvc));
-              uint32_t* mt = s_meta + 4 * rk;
-              mt[0] = (uint32_t)ob;
-              mt[1] = (uint32_t)(ob >> 32);
-              mt[2] = (uint32_t)p;
-              mt[3] = (uint32_t)z;
-            }
-            out += (int64_t)__builtin_amdgcn_readlane(vincl, 63);
-            if (n) emit_block<ELEM>(P, V, s_case, s_nv, s_w, n, node_off);
-          }
-        }
-        ++a;
-        continue;
-      }
-      const int R = (int)__builtin_amdgcn_readlane(incl, b - 1);
-      // lane i < R: record i of the run belongs to tile k (lane k's item)
-      if (lane >= a && lane < b) mark[incl - v] = (uint32_t)lane + 1u;
-      wave_sync();
-      uint32_t mk = 0;
-      if (lane < R) {
-        mk = mark[lane];
-        mark[lane] = 0;
-      }
-      const int k = (int)wave_incl_max32(mk) - 1;
-      const int kk = lane < R ? k : a;
-      const int first = (int)__shfl((int)(incl - v), kk);  // the tile's first record in the run
-      const int64_t rs = __shfl(rs_l, kk);
-      uint32_t rec[kRecWords];
-      if (lane < R) {
-        const uint32_t* src = P.rec + (rs + (lane - first)) * kRecWords;
-#pragma unroll
-        for (int q = 0; q < kRecWords; ++q) rec[q] = src[q];
-      } else {
-#pragma unroll
-        for (int q = 0; q < kRecWords; ++q) rec[q] = 0;
-      }
-      const uint32_t nv = lane < R ? s_nv[rec[0] & 0xffu] : 0u;
-      const uint32_t vincl = wave_incl_scan32(nv);
-      const uint32_t vex = vincl - nv;
-      const uint32_t vfirst = (uint32_t)__shfl((int)vex, first);
-      const int64_t off = __shfl(off_l, kk);
-      const uint32_t p0 = (uint32_t)__shfl((int)p0_l, kk);
-      const uint32_t z = (uint32_t)__shfl((int)z_l, kk);
-      if (lane < R) {
-#pragma unroll
-        for (int q = 0; q < kRecWords; ++q) s_w[lane * kRecWords + q] = rec[q];
-        const uint64_t o = (uint64_t)(off + (int64_t)(vex - vfirst));
-        uint32_t* mt = s_meta + 4 * lane;
-        mt[0] = (uint32_t)o;
-        mt[1] = (uint32_t)(o >> 32);
-        mt[2] = p0 + (rec[0] >> 8);
-        mt[3] = z;
-      }
-      emit_block<ELEM>(P, V, s_case, s_nv, s_w, R, node_off);
-      a = b;
+    // records of the first entry's tile before the unit: their vertices
+    const int64_t head = g0 - __shfl(pre, 0);
+    const int64_t rs0 = __shfl(rs, 0) - head;  // the tile's first record
+    wave_sync();
+    uint32_t mk = 0;
+    if (lane < R) {
+      mk = mark[lane];
+      mark[lane] = 0;
     }
+    const int kk = lane < R ? (int)wave_incl_max32(mk) - 1 : 0;
+    const int fr = __shfl(first, kk);
+    const int64_t rsk = __shfl(rs, kk);
+    uint32_t rec[kRecWords];
+    if (lane < R) {
+      const uint32_t* src = P.rec + (rsk + (lane - fr)) * kRecWords;
+#pragma unroll
+      for (int q = 0; q < kRecWords; ++q) rec[q] = src[q];
+    } else {
+#pragma unroll
+      for (int q = 0; q < kRecWords; ++q) rec[q] = 0;
+    }
+    // a tile has at most kTile records: head < 512, eight loads per lane
+    uint32_t hv = 0;
+    if (head > 0) {
+      uint32_t hw[kTile / 64];
+#pragma unroll
+      for (int q = 0; q < kTile / 64; ++q) {
+        const int64_t j = lane + 64 * q;
+        hw[q] = j < head ? P.rec[(rs0 + j) * kRecWords] : 0u;
+      }
+#pragma unroll
+      for (int q = 0; q < kTile / 64; ++q) hv += lane + 64 * q < head ? s_nv[hw[q] & 0xffu] : 0u;
+      hv = wave_sum32(hv);
+    }
+    const uint32_t nv = lane < R ? s_nv[rec[0] & 0xffu] : 0u;
+    const uint32_t vincl = wave_incl_scan32(nv);
+    const uint32_t vex = vincl - nv;
+    const uint32_t vfirst = (uint32_t)__shfl((int)vex, fr);
+    const int64_t offk = __shfl(off, kk) + (kk == 0 ? (int64_t)hv : 0);
+    const uint32_t p0k = (uint32_t)__shfl((int)p0, kk);
+    const uint32_t zk = (uint32_t)__shfl((int)z, kk);
+    if (lane < R) {
+#pragma unroll
+      for (int q = 0; q < kRecWords; ++q) s_w[lane * kRecWords + q] = rec[q];
+      const uint64_t o = (uint64_t)(offk + (int64_t)(vex - vfirst));
+      uint32_t* mt = s_meta + 4 * lane;
+      mt[0] = (uint32_t)o;
+      mt[1] = (uint32_t)(o >> 32);
+      mt[2] = p0k + (rec[0] >> 8);
+      mt[3] = zk;
+    }
+    emit_block<ELEM>(P, V, s_case, s_nv, s_w, R, node_off);
   }
 }
 
@@ -1658,8 +1767,18 @@ int64_t cubes_of(int X, int Y, int Z) {
 
 size_t fixed_ws_bytes(int64_t n_tiles) {
   const int64_t nscan = n_tiles > 0 ? (n_tiles + kScanItems - 1) / kScanItems : 1;
-  return kWsHeader + al256(16 * (size_t)nscan) + 3 * al256(4 * (size_t)n_tiles) +
-         2 * al256(8 * (size_t)n_tiles) + al256(8 * (size_t)kCPW * (size_t)n_tiles);
+  return kWsHeader + al256(24 * (size_t)nscan) + 5 * al256(4 * (size_t)n_tiles) +
+         3 * al256(8 * (size_t)n_tiles) + al256(8 * (size_t)kCPW * (size_t)n_tiles) +
+         al256(8 * (size_t)(n_tiles + 1));
+}
+
+// The record area: records of 9 words and the unit table (one word per 64
+// records, plus two).  rec_area_bytes(r) always holds r records.
+size_t rec_area_bytes(int64_t records) {
+  return (size_t)records * kRecWords * 4 + (size_t)(records + 15) / 16 + 512;
+}
+int64_t rec_cap_of(size_t rest) {
+  return rest > 512 ? (int64_t)((rest - 512) * 16 / (kRecWords * 4 * 16 + 1)) : 0;
 }
 
 int layout_ws(void* d_ws, size_t ws_bytes, int64_t n_tiles, ExtractParams& P) {
@@ -1671,7 +1790,7 @@ int layout_ws(void* d_ws, size_t ws_bytes, int64_t n_tiles, ExtractParams& P) {
   size_t o = kWsHeader;
   P.cap_desc = n_tiles > 0 ? (n_tiles + kScanItems - 1) / kScanItems : 1;
   P.desc = reinterpret_cast<uint64_t*>(b + o);
-  o += al256(16 * (size_t)P.cap_desc);
+  o += al256(24 * (size_t)P.cap_desc);  // three descriptor sets: tiles, list records, list entries
   P.tileT = reinterpret_cast<int32_t*>(b + o);
   o += al256(4 * (size_t)n_tiles);
   P.tile_nrec = reinterpret_cast<int32_t*>(b + o);
@@ -1684,8 +1803,20 @@ int layout_ws(void* d_ws, size_t ws_bytes, int64_t n_tiles, ExtractParams& P) {
   o += al256(8 * (size_t)kCPW * (size_t)n_tiles);
   P.nz_list = reinterpret_cast<uint32_t*>(b + o);
   o += al256(4 * (size_t)n_tiles);
+  P.dense_list = reinterpret_cast<uint32_t*>(b + o);
+  o += al256(4 * (size_t)n_tiles);
+  P.clist = reinterpret_cast<uint32_t*>(b + o);
+  o += al256(4 * (size_t)n_tiles);
+  P.lpre = reinterpret_cast<int64_t*>(b + o);
+  o += al256(8 * (size_t)(n_tiles + 1));
+  P.lrs = reinterpret_cast<int64_t*>(b + o);
+  o += al256(8 * (size_t)n_tiles);
+  P.list_scan = false;  // set by run_extract for single-volume emission
+  const int64_t rc = rec_cap_of(ws_bytes - o);
+  P.units = reinterpret_cast<uint32_t*>(b + o);
+  o += al256(4 * (size_t)(rc / 64 + 2));
   P.rec = reinterpret_cast<uint32_t*>(b + o);
-  P.rec_cap = (int64_t)((ws_bytes - o) / (kRecWords * 4));
+  P.rec_cap = rc;
   return MCX_OK;
 }
 
@@ -1711,6 +1842,7 @@ int run_extract(ExtractParams& P, int elem, bool batch, int maxX, hipStream_t s,
   // record scratch: per-column quotas; the wide count kernel keeps a quarter
   // of the scratch as a shared overflow pool
   const bool wide = P.wide && elem == MCX_ELEM_SHORT2 && !P.write_masks;
+  P.list_scan = !batch && emit;
   const int64_t pool = wide ? P.rec_cap / 4 : 0;
   P.quota = P.ncols > 0 ? (P.rec_cap - pool) / P.ncols : 0;
   if (P.quota > (int64_t)kTile * P.slab_max) P.quota = (int64_t)kTile * P.slab_max;
@@ -1735,7 +1867,7 @@ int run_extract(ExtractParams& P, int elem, bool batch, int maxX, hipStream_t s,
     }                                                                                            \
     if (!emit) break;                                                                            \
     if (B) hipLaunchKernelGGL((emit_kernel<E, B>), dim3(k3_blocks), dim3(kEmitThreads), 0, s, P); \
-    else hipLaunchKernelGGL(emit_packed_kernel<E>, dim3(k3_blocks), dim3(kEmitThreads), 0, s, P); \
+    else hipLaunchKernelGGL(emit_units_kernel<E>, dim3(k3_blocks), dim3(kEmitThreads), 0, s, P); \
     MCX_HIP(hipGetLastError());                                                                  \
   } while (0)
   if (elem == MCX_ELEM_SHORT2) {
@@ -1807,7 +1939,7 @@ using namespace mcx;
 
 extern "C" size_t mcx_extract_workspace_size_ex(int X, int Y, int Z, int64_t records) {
   if (records < 0) records = 0;
-  return fixed_ws_bytes(tiles_of(X, Y, Z)) + (size_t)records * kRecWords * 4;
+  return fixed_ws_bytes(tiles_of(X, Y, Z)) + rec_area_bytes(records);
 }
 
 // Default record scratch: one record per 16 cubes (6.25 % of the cubes of a
@@ -2078,7 +2210,7 @@ extern "C" int64_t mcx_batch_tiles(const mcx_batch* p) { return p ? p->n_tiles :
 extern "C" size_t mcx_batch_workspace_size(const mcx_batch* p, int64_t records) {
   if (!p) return 0;
   if (records < 0) records = std::max<int64_t>(65536, p->cubes / 16);
-  return fixed_ws_bytes(p->n_tiles) + (size_t)records * kRecWords * 4;
+  return fixed_ws_bytes(p->n_tiles) + rec_area_bytes(records);
 }
 
 extern "C" int mcx_extract_batched(const mcx_batch* plan, mcx_vertex_voxel* d_vv, float* d_pos,

@@ -43,6 +43,8 @@ struct WsHeader {
   uint32_t nz_count;    // non-empty tiles appended by the running count kernel
   uint32_t nz_total;    // the same, frozen by the scan for the emit kernel
   uint64_t ovf_next;    // records taken from the shared overflow pool
+  uint32_t dense_n;     // staged-path tiles listed by the scan for the emit kernel
+  uint32_t list_n;      // the list's tiles with records, compacted by the scan
 };
 static_assert(sizeof(WsHeader) <= 256, "header fits its slot");
 

@@ -141,6 +141,43 @@ def test_record_overflow_uses_staged_path(dev, records):
         assert r.ver_voxel.cpu().numpy().tobytes() == ref["ver_voxel"].tobytes()
 
 
+def layered_volume(shape, seed):
+    """Noise with surfaces parallel to the planes: layers of z where every
+    cube is active (tiles of hundreds of records) between noisy slabs."""
+    rng = np.random.default_rng(seed)
+    Z, Y, X = shape
+    t = np.where((np.arange(Z) % 5 < 2)[:, None, None], 400, -400) * np.ones((Z, Y, X), np.int64)
+    t = t + rng.integers(-30, 30, shape)
+    noisy = rng.random(shape) < 0.3
+    t[noisy] = rng.integers(-2000, 2000, int(noisy.sum()))
+    w = rng.integers(1, 100, shape)
+    w[rng.random(shape) < 0.02] = 0
+    return synth.pack_short2(t.astype(np.int16), w.astype(np.uint16))
+
+
+@pytest.mark.parametrize("records", [300, 3000, None])
+def test_large_tiles_and_mixed_staged_path(dev, records):
+    # 128-bit count kernel (X % 4 == 0): tiles of up to 511 records, so the
+    # 64-record emission units start deep inside tiles; with a small record
+    # scratch, staged-path tiles are interleaved with tiles that kept their
+    # records
+    lib = _capi.load()
+    for shape in [(16, 12, 256), (11, 40, 96)]:
+        print(f"[case] layered {shape} records={records}", flush=True)
+        vol = layered_volume(shape, 11)
+        Z, Y, X = shape
+        ws = None
+        if records is not None:
+            ws = mesh.Workspace(dev, lib.mcx_extract_workspace_size_ex(X, Y, Z, records),
+                                grow=False)
+        ref = O.extract(vol, 0.01, want=("vv", "v2vox", "ver_voxel"))
+        r = mesh.extract(to_dev(vol, dev), 0.01, want=("vv", "v2vox", "ver_voxel"), workspace=ws)
+        assert r.count == ref["n"]
+        assert r.vv.cpu().numpy().tobytes() == ref["vv"].view(np.int32).reshape(-1, 7).tobytes()
+        assert r.v2vox.cpu().numpy().tobytes() == ref["v2vox"].tobytes()
+        assert r.ver_voxel.cpu().numpy().tobytes() == ref["ver_voxel"].tobytes()
+
+
 def test_normals_bit_exact(dev):
     for vol, h in [synth.sphere_volume(64), (synth.noise_volume((12, 13, 14), seed=3), 0.02)]:
         print(f"[case] normals {vol.shape}", flush=True)
