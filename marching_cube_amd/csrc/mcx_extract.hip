@@ -87,6 +87,43 @@ __device__ __forceinline__ int corner_of(int dx, int dy, int dz) {
   return 4 * dz + (dy ? (dx ? 2 : 3) : dx);
 }
 
+// Per case and output slot q: the edge's corners and axis,
+// ca | cb << 3 | axis << 6.  Output slot q = 3*tri + r holds table corner
+// 2 - r of triangle tri (:324-332); slots past the case's vertex count are 0.
+struct SlotTable {
+  uint8_t v[256 * 16];
+};
+constexpr SlotTable make_slot_table() {
+  constexpr unsigned long long cs[256] = MCX_CASE_TABLE_INIT;
+  SlotTable t{};
+  for (int ci = 0; ci < 256; ++ci) {
+    for (int q = 0; q < 15; ++q) {
+      const int m = 3 * (q / 3) + 2 - (q % 3);
+      const int e = (int)((cs[ci] >> (4 * m)) & 0xfu);
+      if (e >= 12) continue;
+      const int ca = (int)((kEdgeA >> (3 * e)) & 7u), cb = (int)((kEdgeB >> (3 * e)) & 7u);
+      const int ax = (((ca + 1) >> 1) & 1) != (((cb + 1) >> 1) & 1) ? 0
+                     : ((ca >> 1) & 1) != ((cb >> 1) & 1)          ? 1
+                                                                    : 2;
+      t.v[ci * 16 + q] = (uint8_t)(ca | (cb << 3) | (ax << 6));
+    }
+  }
+  return t;
+}
+__constant__ SlotTable dSlot = make_slot_table();
+
+// num / den correctly rounded, for the interpolation weight of two short2
+// corners of opposite classification (num = 0 - fa, den = fb - fa + 1e-15f,
+// fa and fb = k / 32766 correctly rounded): a reciprocal refined once and one
+// quotient correction.  Equal to IEEE division on all 2^31 such pairs
+// (checked exhaustively on the device); fp32 volumes keep IEEE division.
+__device__ __forceinline__ float div_weight_short2(float num, float den) {
+  float r = __builtin_amdgcn_rcpf(den);
+  r = __fmaf_rn(__fmaf_rn(-den, r, 1.0f), r, r);
+  const float q = num * r;
+  return __fmaf_rn(__fmaf_rn(-den, q, num), r, q);
+}
+
 struct VolDesc {
   const uint32_t* vol;   // X*Y*Z raw voxels (short2 bits or fp32 bits), [z][y][x]
   int X, Y, Z;
@@ -601,12 +638,27 @@ void count4_kernel(ExtractParams P) {
     asm volatile("buffer_load_dword %0, %1, %2, 0 offen" : "=v"(L.xu) : "v"(xa), "s"(rsv) : "memory");
     asm volatile("buffer_load_dword %0, %1, %2, 0 offen" : "=v"(L.xw) : "v"(xb), "s"(rsv) : "memory");
   };
+  // All but the newest n vector memory operations done (they retire in
+  // issue order); ties L's registers.  n is wave-uniform and a lower bound of
+  // the operations issued after L's loads, rounded down to an immediate.
   auto wait_plane = [&](Plane4& L, int n) {
-    // all but the newest n vector memory operations done; ties L's registers
-    if (n == 6)
-      asm volatile("s_waitcnt vmcnt(6)" : "+v"(L.u[0]), "+v"(L.u[1]), "+v"(L.w[0]), "+v"(L.w[1]), "+v"(L.xu), "+v"(L.xw) :: "memory");
-    else
-      asm volatile("s_waitcnt vmcnt(0)" : "+v"(L.u[0]), "+v"(L.u[1]), "+v"(L.w[0]), "+v"(L.w[1]), "+v"(L.xu), "+v"(L.xw) :: "memory");
+#define MCX_WAIT_VM(K)                                                                        \
+  asm volatile("s_waitcnt vmcnt(" #K ")"                                                      \
+               : "+v"(L.u[0]), "+v"(L.u[1]), "+v"(L.w[0]), "+v"(L.w[1]), "+v"(L.xu), "+v"(L.xw) \
+               :                                                                              \
+               : "memory")
+    if (n >= 63) MCX_WAIT_VM(63);
+    else if (n >= 48) MCX_WAIT_VM(48);
+    else if (n >= 32) MCX_WAIT_VM(32);
+    else if (n >= 24) MCX_WAIT_VM(24);
+    else if (n >= 16) MCX_WAIT_VM(16);
+    else if (n >= 12) MCX_WAIT_VM(12);
+    else if (n >= 10) MCX_WAIT_VM(10);
+    else if (n >= 9) MCX_WAIT_VM(9);
+    else if (n >= 8) MCX_WAIT_VM(8);
+    else if (n >= 7) MCX_WAIT_VM(7);
+    else MCX_WAIT_VM(6);
+#undef MCX_WAIT_VM
   };
   // Four plane buffers whose roles rotate over an unrolled loop of four steps
   // (no register copies, which would wait for in-flight loads): the step for
@@ -620,13 +672,17 @@ void count4_kernel(ExtractParams P) {
 
   int32_t used = 0;  // records of the column's quota taken
   uint64_t nzmask = 0;  // planes zb + i of this column whose tile has vertices
+  // store instructions of the last step issued before its plane loads (a
+  // lower bound): the next step's wait lets them stay in flight
+  int pend = 0;
   int zb = z0;
   const uint32_t quota = (uint32_t)P.quota;
   uint32_t t = (uint32_t)(V.tile_begin + (int64_t)z0 * V.R + r);  // this step's tile
   const uint32_t tR = (uint32_t)V.R;
   // The plane two steps ahead is loaded at the end of a step, after the
-  // step's record stores: the data a step waits for is then always older
-  // than the newest loads (a fixed count), however many stores were issued.
+  // step's record stores: a step waits for the plane loaded at the end of the
+  // step before last, i.e. for all but the last step's stores (counted in
+  // pend) and the last step's six loads.
   auto step = [&](Plane4& A, Plane4& B, Plane4& L, int z) __attribute__((always_inline)) {
 
     // ---- pass 1: active masks m[c][k] (bit L: cube 256c + 4L + k).  Signs
@@ -639,7 +695,7 @@ void count4_kernel(ExtractParams P) {
     // row end are masked.  Only chunks with such cubes test the weights:
     // the minimum word over the eight corners below 0x10000 means some
     // weight is zero.
-    wait_plane(B, 6);
+    wait_plane(B, 6 + pend);
     plane_signs(B);
     uint64_t m[2][4];
     uint32_t nxo, nxn, nxm;  // position 0 of the next chunk's lane 0 (wave-uniform)
@@ -690,6 +746,7 @@ void count4_kernel(ExtractParams P) {
     if ((any_c0 | any_c1) == 0) {
       // no active cube: only the tile's vertex count (0) is read later
       if (lane == 0) P.tileT[t] = 0;
+      pend = 1;
       load_plane(z + 3, L);
       t += tR;
       if (z - zb == 63 || z + 1 == z1) {
@@ -736,6 +793,7 @@ void count4_kernel(ExtractParams P) {
     // signs), one bit per corner in corner order.
     uint32_t vsum = 0;
     int64_t cbase = 0;
+    int nst = 3;  // the tile's count, record count and first record
 #pragma unroll
     for (int c = 0; c < 2; ++c) {
       const uint64_t any = m[c][0] | m[c][1] | m[c][2] | m[c][3];
@@ -810,9 +868,11 @@ void count4_kernel(ExtractParams P) {
       const int nc = __popcll(m[c][0]) + __popcll(m[c][1]) + __popcll(m[c][2]) + __popcll(m[c][3]);
       wave_sync();
       if (!dense) {
-        // the chunk's records are consecutive in scratch
+        // the chunk's records are consecutive in scratch: one store
+        // instruction per 64 words
         uint32_t* dst = P.rec + (rbase + cbase) * kRecWords;
         for (int w = lane; w < nc * kRecWords; w += 64) dst[w] = my_rec[w];
+        nst += (nc * kRecWords + 63) >> 6;
         wave_sync();
       }
       cbase += nc;
@@ -825,6 +885,7 @@ void count4_kernel(ExtractParams P) {
       if (dense) atomicAdd(&P.hdr->dense_acc, 1u);
     }
     if (in_quota) used += nrec32;
+    pend = nst;
     load_plane(z + 3, L);
     t += tR;
     // append the column's non-empty tiles to the list, 64 planes at a time
@@ -1212,6 +1273,98 @@ __device__ __forceinline__ int emit_records(const ExtractParams& P, const VolDes
   return tot;
 }
 
+// One output vertex, slot q of the record at rc (LDS: word 0 = cube offset
+// << 8 | case, words 1..8 = the corner words in corner order) of cube
+// (x, y, z), in the same fp32 operations as make_vertex; returns the grid
+// edge key (lower corner voxel * 3 + axis) when want_key: the edge's corners
+// come from the slot table and their words are read directly; along the two
+// axes the edge does not run on, the interpolated coordinate is exactly the
+// corner's node coordinate (fma(t, +0, p) = p), so only the edge axis is
+// interpolated, and with node_off only that axis divides.
+template <int ELEM>
+__device__ __forceinline__ int64_t record_vertex(const VolDesc& V, const uint8_t* s_slot,
+                                                 const uint32_t* rc, int x, int y, int z, int q,
+                                                 uint32_t* o, bool want_key,
+                                                 const int8_t* node_off) {
+  const uint32_t sl = s_slot[(rc[0] & 0xffu) * 16 + q];
+  const int ca = (int)(sl & 7u), cb = (int)((sl >> 3) & 7u), ax = (int)(sl >> 6);
+  const uint32_t wa = rc[1 + ca], wb = rc[1 + cb];
+  const int ix = x + corner_dx(ca), iy = y + corner_dy(ca), iz = z + corner_dz(ca);
+  const int ia = ax == 0 ? ix : (ax == 1 ? iy : iz);          // ca along the edge
+  const int ib = ia + (ax == 0 ? corner_dx(cb) - corner_dx(ca)
+                               : (ax == 1 ? corner_dy(cb) - corner_dy(ca)
+                                          : corner_dz(cb) - corner_dz(ca)));
+  int64_t key = 0;
+  if (want_key) {
+    // the grid edge: its lower corner and its axis
+    const int lo = min(ia, ib);
+    const int mx = ax == 0 ? lo : ix, my = ax == 1 ? lo : iy, mz = ax == 2 ? lo : iz;
+    key = (((int64_t)mz * V.Y + my) * V.X + mx) * 3 + ax;
+  }
+  const float h = V.h;
+  const float fa = unpack_f<ELEM>(wa);
+  const float fb = unpack_f<ELEM>(wb);
+  const float num = 0.0f - fa, den = fb - fa + 1e-15f;
+  float tt;
+  if constexpr (ELEM == MCX_ELEM_SHORT2) tt = div_weight_short2(num, den);
+  else tt = num / den;
+  const float pax = ((float)ix + 0.5f) * h;
+  const float pay = ((float)iy + 0.5f) * h;
+  const float paz = ((float)iz + 0.5f) * h;
+  const float pae = ax == 0 ? pax : (ax == 1 ? pay : paz);
+  const float pbe = ((float)ib + 0.5f) * h;
+  const float pe = __fmaf_rn(tt, pbe - pae, pae);
+  const float px = ax == 0 ? pe : pax;
+  const float py = ax == 1 ? pe : pay;
+  const float pz = ax == 2 ? pe : paz;
+  // record fields (store_point :336-367)
+  int vx, vy, vz;
+  if (node_off) {
+    const int nx = node_off[ix], ny = node_off[iy], nz = node_off[iz];
+    float ce = pe / h;
+    ce = ce - 0.5f;
+    const int ve = (int)floorf(ce);
+    vx = ax == 0 ? ve : ix + nx;
+    vy = ax == 1 ? ve : iy + ny;
+    vz = ax == 2 ? ve : iz + nz;
+  } else {
+    float cx = px / h, cy = py / h, cz = pz / h;
+    cx = cx - 0.5f;
+    cy = cy - 0.5f;
+    cz = cz - 0.5f;
+    vx = (int)floorf(cx);
+    vy = (int)floorf(cy);
+    vz = (int)floorf(cz);
+  }
+  const bool inb = vx >= 0 && vy >= 0 && vz >= 0 && vx < V.X && vy < V.Y && vz < V.Z;
+  uint32_t tw = 0;
+  auto lin = [&]() { return ((int64_t)vz * V.Y + vy) * V.X + vx; };
+  if (inb) {
+    if constexpr (ELEM == MCX_ELEM_SHORT2) {
+      // the voxel is almost always one of the cube's corners; otherwise it
+      // is loaded and consumed in the branch (a VALU copy), so that no load
+      // is outstanding where the paths join: a wait there would also wait
+      // for every store issued before it (one counter for both)
+      const unsigned dx = (unsigned)(vx - x), dy = (unsigned)(vy - y), dz = (unsigned)(vz - z);
+      if ((dx | dy | dz) <= 1u) {
+        tw = rc[1 + corner_of((int)dx, (int)dy, (int)dz)];
+      } else {
+        const uint32_t t = V.vol[lin()];
+        asm volatile("v_mov_b32 %0, %1" : "=v"(tw) : "v"(t));
+      }
+    }
+  }
+  o[0] = __float_as_uint(px);
+  o[1] = __float_as_uint(py);
+  o[2] = __float_as_uint(pz);
+  o[3] = (uint32_t)vx;
+  o[4] = (uint32_t)vy;
+  o[5] = (uint32_t)vz;
+  o[6] = tw;
+  if (V.ver_voxel && inb) V.ver_voxel[lin()] = (int16_t)(tw & 0xffffu);
+  return key;
+}
+
 // Emit the vertices of a packed block: nb records (up to 64, possibly of
 // several tiles) in the wave's record area, record r with its output index
 // (of its first vertex) and cube position (p = in-plane cube index, z) in
@@ -1222,7 +1375,7 @@ __device__ __forceinline__ int emit_records(const ExtractParams& P, const VolDes
 // consecutive records).
 template <int ELEM>
 __device__ __forceinline__ void emit_block(const ExtractParams& P, const VolDesc& V,
-                                           const uint64_t* s_case, const uint8_t* s_nv,
+                                           const uint8_t* s_slot, const uint8_t* s_nv,
                                            uint32_t* s_w, int nb,
                                            const int8_t* node_off = nullptr) {
   const uint32_t* s_rec = s_w;
@@ -1244,10 +1397,6 @@ __device__ __forceinline__ void emit_block(const ExtractParams& P, const VolDesc
     if (j < tot) {
       const int r = s_map[j];
       const uint32_t* rc = s_rec + r * kRecWords;
-      const uint32_t rw = rc[0];
-      uint32_t c[8];
-#pragma unroll
-      for (int k = 0; k < 8; ++k) c[k] = rc[1 + k];
       const uint32_t* mt = s_meta + 4 * r;
       const int slot = j - (int)s_ex[r];
       const int64_t out = (int64_t)(((uint64_t)mt[1] << 32) | mt[0]) + slot;
@@ -1256,9 +1405,8 @@ __device__ __forceinline__ void emit_block(const ExtractParams& P, const VolDesc
         const uint32_t y = V.divX.div(p);
         const int x = (int)(p - y * (uint32_t)V.X);
         uint32_t o[7];
-        int64_t key;
-        make_vertex<ELEM>(V, s_case, c, x, (int)y, (int)mt[3], rw & 0xffu, slot, true, o,
-                          P.ekey ? &key : nullptr, node_off);
+        const int64_t key = record_vertex<ELEM>(V, s_slot, rc, x, (int)y, (int)mt[3], slot, o,
+                                                P.ekey != nullptr, node_off);
         if (vv) {
 #pragma unroll
           for (int f = 0; f < 7; ++f) vv[7 * out + f] = o[f];
@@ -1282,7 +1430,7 @@ __device__ __forceinline__ void emit_block(const ExtractParams& P, const VolDesc
 // cubes at a time (cube order) into the record form and emitted from voff[t].
 template <int ELEM>
 __device__ __forceinline__ void emit_staged_tile(const ExtractParams& P, const VolDesc& V,
-                                                 const uint64_t* s_case, const uint8_t* s_nv,
+                                                 const uint8_t* s_slot, const uint8_t* s_nv,
                                                  uint32_t* s_w, uint32_t t,
                                                  const int8_t* node_off) {
   const int lane = threadIdx.x & 63;
@@ -1327,7 +1475,7 @@ __device__ __forceinline__ void emit_staged_tile(const ExtractParams& P, const V
       mt[3] = (uint32_t)z;
     }
     out += (int64_t)__builtin_amdgcn_readlane(vincl, 63);
-    if (n) emit_block<ELEM>(P, V, s_case, s_nv, s_w, n, node_off);
+    if (n) emit_block<ELEM>(P, V, s_slot, s_nv, s_w, n, node_off);
   }
 }
 
@@ -1344,13 +1492,15 @@ __device__ __forceinline__ void emit_staged_tile(const ExtractParams& P, const V
 // from their case bytes.
 template <int ELEM>
 __global__ __launch_bounds__(kEmitThreads) void emit_units_kernel(ExtractParams P) {
-  __shared__ uint64_t s_case[256];
+  __shared__ __attribute__((aligned(16))) uint8_t s_slot[256 * 16];
   __shared__ uint8_t s_nv[256];
   __shared__ uint32_t s_wave[kEmitThreads / 64][kPackedWords];
   __shared__ uint32_t s_mark[kEmitThreads / 64][64];
   __shared__ int8_t s_node[kNodeTable];
   __shared__ int s_node_ok;
-  s_case[threadIdx.x] = dCase[threadIdx.x];
+  static_assert(kEmitThreads * 16 == sizeof(SlotTable), "16 table bytes per thread");
+  reinterpret_cast<u32x4*>(s_slot)[threadIdx.x] =
+      reinterpret_cast<const u32x4*>(dSlot.v)[threadIdx.x];
   s_nv[threadIdx.x] = dNumVerts[threadIdx.x];
   const int lane = threadIdx.x & 63;
   const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
@@ -1383,7 +1533,7 @@ __global__ __launch_bounds__(kEmitThreads) void emit_units_kernel(ExtractParams 
   const int64_t nwaves = (int64_t)gridDim.x * (kEmitThreads / 64);
   for (int64_t k = (int64_t)blockIdx.x * (kEmitThreads / 64) + wv; k < n_work; k += nwaves) {
     if (k >= n_units) {
-      emit_staged_tile<ELEM>(P, V, s_case, s_nv, s_w, P.dense_list[k - n_units], node_off);
+      emit_staged_tile<ELEM>(P, V, s_slot, s_nv, s_w, P.dense_list[k - n_units], node_off);
       continue;
     }
     const int64_t g0 = k << 6;
@@ -1458,7 +1608,7 @@ __global__ __launch_bounds__(kEmitThreads) void emit_units_kernel(ExtractParams 
       mt[2] = p0k + (rec[0] >> 8);
       mt[3] = zk;
     }
-    emit_block<ELEM>(P, V, s_case, s_nv, s_w, R, node_off);
+    emit_block<ELEM>(P, V, s_slot, s_nv, s_w, R, node_off);
   }
 }
 
