@@ -469,6 +469,7 @@ __global__ __launch_bounds__(256) void count_kernel(ExtractParams P) {
 // staged path.  No active-cube masks are written (the backward runs on fp32
 // volumes, which take the 32-bit kernel).
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+typedef uint32_t u32x4a4 __attribute__((ext_vector_type(4), aligned(4)));  // 4-byte aligned
 typedef short s16x2 __attribute__((ext_vector_type(2)));
 typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
 
@@ -542,7 +543,7 @@ template <bool BATCH>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4)))
 void count4_kernel(ExtractParams P) {
   __shared__ uint8_t s_nv[256];
-  __shared__ uint32_t s_rec[4][256 * kRecWords];  // per wave: one chunk's records
+  __shared__ __attribute__((aligned(16))) uint32_t s_rec[4][256 * kRecWords];  // per wave: one chunk's records
   s_nv[threadIdx.x] = dNumVerts[threadIdx.x];
   clear_scan_descriptors(P);
   __syncthreads();
@@ -638,27 +639,10 @@ void count4_kernel(ExtractParams P) {
     asm volatile("buffer_load_dword %0, %1, %2, 0 offen" : "=v"(L.xu) : "v"(xa), "s"(rsv) : "memory");
     asm volatile("buffer_load_dword %0, %1, %2, 0 offen" : "=v"(L.xw) : "v"(xb), "s"(rsv) : "memory");
   };
-  // All but the newest n vector memory operations done (they retire in
-  // issue order); ties L's registers.  n is wave-uniform and a lower bound of
-  // the operations issued after L's loads, rounded down to an immediate.
-  auto wait_plane = [&](Plane4& L, int n) {
-#define MCX_WAIT_VM(K)                                                                        \
-  asm volatile("s_waitcnt vmcnt(" #K ")"                                                      \
-               : "+v"(L.u[0]), "+v"(L.u[1]), "+v"(L.w[0]), "+v"(L.w[1]), "+v"(L.xu), "+v"(L.xw) \
-               :                                                                              \
-               : "memory")
-    if (n >= 63) MCX_WAIT_VM(63);
-    else if (n >= 48) MCX_WAIT_VM(48);
-    else if (n >= 32) MCX_WAIT_VM(32);
-    else if (n >= 24) MCX_WAIT_VM(24);
-    else if (n >= 16) MCX_WAIT_VM(16);
-    else if (n >= 12) MCX_WAIT_VM(12);
-    else if (n >= 10) MCX_WAIT_VM(10);
-    else if (n >= 9) MCX_WAIT_VM(9);
-    else if (n >= 8) MCX_WAIT_VM(8);
-    else if (n >= 7) MCX_WAIT_VM(7);
-    else MCX_WAIT_VM(6);
-#undef MCX_WAIT_VM
+  // all but the newest six vector memory operations done (they retire in
+  // issue order); ties L's registers
+  auto wait_plane6 = [&](Plane4& L) {
+    asm volatile("s_waitcnt vmcnt(6)" : "+v"(L.u[0]), "+v"(L.u[1]), "+v"(L.w[0]), "+v"(L.w[1]), "+v"(L.xu), "+v"(L.xw) :: "memory");
   };
   // Four plane buffers whose roles rotate over an unrolled loop of four steps
   // (no register copies, which would wait for in-flight loads): the step for
@@ -672,17 +656,13 @@ void count4_kernel(ExtractParams P) {
 
   int32_t used = 0;  // records of the column's quota taken
   uint64_t nzmask = 0;  // planes zb + i of this column whose tile has vertices
-  // store instructions of the last step issued before its plane loads (a
-  // lower bound): the next step's wait lets them stay in flight
-  int pend = 0;
   int zb = z0;
   const uint32_t quota = (uint32_t)P.quota;
   uint32_t t = (uint32_t)(V.tile_begin + (int64_t)z0 * V.R + r);  // this step's tile
   const uint32_t tR = (uint32_t)V.R;
   // The plane two steps ahead is loaded at the end of a step, after the
-  // step's record stores: a step waits for the plane loaded at the end of the
-  // step before last, i.e. for all but the last step's stores (counted in
-  // pend) and the last step's six loads.
+  // step's record stores: the data a step waits for is then always older
+  // than the newest loads (a fixed count), however many stores were issued.
   auto step = [&](Plane4& A, Plane4& B, Plane4& L, int z) __attribute__((always_inline)) {
 
     // ---- pass 1: active masks m[c][k] (bit L: cube 256c + 4L + k).  Signs
@@ -695,7 +675,7 @@ void count4_kernel(ExtractParams P) {
     // row end are masked.  Only chunks with such cubes test the weights:
     // the minimum word over the eight corners below 0x10000 means some
     // weight is zero.
-    wait_plane(B, 6 + pend);
+    wait_plane6(B);
     plane_signs(B);
     uint64_t m[2][4];
     uint32_t nxo, nxn, nxm;  // position 0 of the next chunk's lane 0 (wave-uniform)
@@ -746,7 +726,6 @@ void count4_kernel(ExtractParams P) {
     if ((any_c0 | any_c1) == 0) {
       // no active cube: only the tile's vertex count (0) is read later
       if (lane == 0) P.tileT[t] = 0;
-      pend = 1;
       load_plane(z + 3, L);
       t += tR;
       if (z - zb == 63 || z + 1 == z1) {
@@ -793,7 +772,6 @@ void count4_kernel(ExtractParams P) {
     // signs), one bit per corner in corner order.
     uint32_t vsum = 0;
     int64_t cbase = 0;
-    int nst = 3;  // the tile's count, record count and first record
 #pragma unroll
     for (int c = 0; c < 2; ++c) {
       const uint64_t any = m[c][0] | m[c][1] | m[c][2] | m[c][3];
@@ -868,11 +846,15 @@ void count4_kernel(ExtractParams P) {
       const int nc = __popcll(m[c][0]) + __popcll(m[c][1]) + __popcll(m[c][2]) + __popcll(m[c][3]);
       wave_sync();
       if (!dense) {
-        // the chunk's records are consecutive in scratch: one store
-        // instruction per 64 words
+        // the chunk's records are consecutive in scratch: 16 bytes a lane
+        // (the staging area is 16-byte aligned, the scratch 4-byte aligned),
+        // then the last words
         uint32_t* dst = P.rec + (rbase + cbase) * kRecWords;
-        for (int w = lane; w < nc * kRecWords; w += 64) dst[w] = my_rec[w];
-        nst += (nc * kRecWords + 63) >> 6;
+        const int nw = nc * kRecWords, n4 = nw >> 2;
+        u32x4a4* dst4 = reinterpret_cast<u32x4a4*>(dst);
+        const u32x4* src4 = reinterpret_cast<const u32x4*>(my_rec);
+        for (int w = lane; w < n4; w += 64) dst4[w] = src4[w];
+        if (lane < nw - 4 * n4) dst[4 * n4 + lane] = my_rec[4 * n4 + lane];
         wave_sync();
       }
       cbase += nc;
@@ -885,7 +867,6 @@ void count4_kernel(ExtractParams P) {
       if (dense) atomicAdd(&P.hdr->dense_acc, 1u);
     }
     if (in_quota) used += nrec32;
-    pend = nst;
     load_plane(z + 3, L);
     t += tR;
     // append the column's non-empty tiles to the list, 64 planes at a time
