@@ -72,6 +72,7 @@ def build_dropin():
 
 
 TEST_BIN = os.path.join(ROOT, "tests", "cpp", "test_dropin")
+EXACT_DIV_BIN = os.path.join(ROOT, "tests", "cpp", "exact_div_check")
 
 
 def build_tests():
@@ -83,6 +84,9 @@ def build_tests():
           "-D__HIP_PLATFORM_AMD__", "-o", TEST_BIN, src, "-L", PKG, "-lmarching_cube_amd",
           "-lmcx", "-Wl,-rpath,$ORIGIN/../../marching_cube_amd", "-L/opt/rocm/lib",
           "-lamdhip64", "-Wl,-rpath,/opt/rocm/lib"])
+    # device check of the exact fp32 forms (same flags as the library)
+    chk = os.path.join(ROOT, "tests", "cpp", "exact_div_check.hip")
+    _run([HIPCC, *HIP_FLAGS, "-I", INCLUDE, "-I", CSRC, "-o", EXACT_DIV_BIN, chk])
     return TEST_BIN
 
 

@@ -63,7 +63,7 @@ constexpr int kEmitThreads = 256;
 // batched plans (taken when a plan is created).
 struct Tuning {
   int columns_per_cu = 64;
-  int emit_groups_per_cu = 8;
+  int emit_groups_per_cu = 16;
   int batch_slab = 64;
 };
 Tuning g_tuning;
@@ -112,17 +112,6 @@ constexpr SlotTable make_slot_table() {
 }
 __constant__ SlotTable dSlot = make_slot_table();
 
-// num / den correctly rounded, for the interpolation weight of two short2
-// corners of opposite classification (num = 0 - fa, den = fb - fa + 1e-15f,
-// fa and fb = k / 32766 correctly rounded): a reciprocal refined once and one
-// quotient correction.  Equal to IEEE division on all 2^31 such pairs
-// (checked exhaustively on the device); fp32 volumes keep IEEE division.
-__device__ __forceinline__ float div_weight_short2(float num, float den) {
-  float r = __builtin_amdgcn_rcpf(den);
-  r = __fmaf_rn(__fmaf_rn(-den, r, 1.0f), r, r);
-  const float q = num * r;
-  return __fmaf_rn(__fmaf_rn(-den, q, num), r, q);
-}
 
 struct VolDesc {
   const uint32_t* vol;   // X*Y*Z raw voxels (short2 bits or fp32 bits), [z][y][x]
@@ -181,16 +170,12 @@ struct ExtractParams {
   int64_t rec_cap;
 };
 
-// tsdf / 32766 correctly rounded for every int16 tsdf: a multiply by the
-// rounded reciprocal and one fused correction step (equal to IEEE division
-// over [-32768, 32767], checked exhaustively).
+// The fp32 value of a voxel: tsdf / 32766 (tsdf_unit, correctly rounded) or
+// the fp32 bits.
 template <int ELEM>
 __device__ __forceinline__ float unpack_f(uint32_t v) {
   if constexpr (ELEM == MCX_ELEM_SHORT2) {
-    constexpr float r = 1.0f / 32766.0f;
-    const float x = (float)(int16_t)(v & 0xffffu);
-    const float q = x * r;
-    return __fmaf_rn(__fmaf_rn(-q, 32766.0f, x), r, q);
+    return tsdf_unit((int16_t)(v & 0xffffu));
   } else {
     return __uint_as_float(v);
   }

@@ -13,6 +13,30 @@ namespace mcx {
 
 constexpr int kWave = 64;
 
+// ------------------------------------------------------- exact fp32 forms
+// s / 32766 correctly rounded for every 16-bit s: a multiply by the rounded
+// reciprocal and one fused correction step (equal to IEEE division on all
+// 65536 values; tests/cpp/exact_div_check.hip checks it on the device).
+__device__ __forceinline__ float tsdf_unit(int s) {
+  constexpr float r = 1.0f / 32766.0f;
+  const float x = (float)s;
+  const float q = x * r;
+  return __fmaf_rn(__fmaf_rn(-q, 32766.0f, x), r, q);
+}
+
+// num / den correctly rounded, for the marching-cubes interpolation weight
+// of two short2 corners of opposite classification (num = 0 - fa, den = fb -
+// fa + 1e-15f, fa = tsdf_unit(ka), fb = tsdf_unit(kb), ka < 0 <= kb or
+// kb < 0 <= ka): a reciprocal refined once and one quotient correction.
+// Equal to IEEE division on all 2^31 such pairs (tests/cpp/exact_div_check.hip
+// checks every pair on the device); fp32 volumes keep IEEE division.
+__device__ __forceinline__ float div_weight_short2(float num, float den) {
+  float r = __builtin_amdgcn_rcpf(den);
+  r = __fmaf_rn(__fmaf_rn(-den, r, 1.0f), r, r);
+  const float q = num * r;
+  return __fmaf_rn(__fmaf_rn(-den, q, num), r, q);
+}
+
 // ---------------------------------------------------------------- host side
 // Map a HIP error to a status code; keep the first HIP error for strerror.
 int hip_status(hipError_t e);
