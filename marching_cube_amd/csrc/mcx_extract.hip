@@ -1801,36 +1801,84 @@ __global__ __launch_bounds__(kEmitThreads) void contrib_kernel(ExtractParams P, 
   }
 }
 
-template <bool BATCH>
+// Backward BW2: dL/dv of every voxel = the sum, over the up to 8 cubes it
+// is corner k of (k = 0..7 in corner order), of the cube's contribution k.
+// A workgroup takes kGatherSpan consecutive voxels of one plane z.  The
+// cubes they are corners of lie in planes z - 1 and z and, per plane, in two
+// in-plane windows (rows y and y - 1: cube ids [q0 - 1, q0 + span) and the
+// same minus X), each within 4 consecutive tiles: the workgroup stages those
+// tiles' active-cube mask words, the popcount prefix of each word and the
+// tiles' first records in LDS, so a voxel's lookups read LDS and only active
+// cubes read their contribution (record index = tile's first record + rank
+// of the cube's bit).
+constexpr int kGatherSpan = 1024;  // voxels per workgroup (4 per thread)
 __global__ __launch_bounds__(256) void gather_kernel(ExtractParams P, const float* contrib,
-                                                     float* gvol, int vi) {
-  const VolDesc& V = vol_at<BATCH>(P, vi);
+                                                     float* gvol) {
+  __shared__ uint64_t s_mask[2][2][4][kCPW];  // [dz][dy][tile slot][word]
+  __shared__ uint32_t s_pre[2][2][4][kCPW];   // set bits of the tile's earlier words
+  __shared__ int64_t s_rec[2][2][4];          // first record of the tile
+  const VolDesc& V = P.vol0;
   const int X = V.X, Y = V.Y, Z = V.Z;
-  const int64_t XY = (int64_t)X * Y;
-  const int64_t N = XY * Z;
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < N;
-       i += (int64_t)gridDim.x * blockDim.x) {
-    const int z = (int)(i / XY);
-    const uint32_t rem = (uint32_t)(i - (int64_t)z * XY);
-    const uint32_t y = V.divX.div(rem);
-    const int x = (int)(rem - y * (uint32_t)X);
+  const uint32_t XY = (uint32_t)X * (uint32_t)Y;  // < 2^30 (plane_ok)
+  const uint32_t per_plane = (XY + kGatherSpan - 1) / kGatherSpan;
+  const int z = (int)(blockIdx.x / per_plane);
+  const uint32_t q0 = (blockIdx.x - (uint32_t)z * per_plane) * kGatherSpan;
+  const int tid = threadIdx.x;
+  // first tile of each window (cube ids below 0 are never looked up)
+  int base[2];
+#pragma unroll
+  for (int dy = 0; dy < 2; ++dy) {
+    const int64_t lo = (int64_t)q0 - 1 - (dy ? X : 0);
+    base[dy] = (int)(max(lo, (int64_t)0) >> 9);
+  }
+  if (tid < 128) {
+    const int dz = tid >> 6, dy = (tid >> 5) & 1, slot = (tid >> 3) & 3, word = tid & 7;
+    const int cz = z - dz, r = base[dy] + slot;
+    uint64_t m = 0;
+    if (cz >= 0 && cz <= Z - 2 && r < V.R)
+      m = P.masks[(V.tile_begin + (int64_t)cz * V.R + r) * kCPW + word];
+    s_mask[dz][dy][slot][word] = m;
+  } else if (tid < 128 + 16) {
+    const int e = tid - 128, dz = e >> 3, dy = (e >> 2) & 1, slot = e & 3;
+    const int cz = z - dz, r = base[dy] + slot;
+    int64_t rs = 0;
+    if (cz >= 0 && cz <= Z - 2 && r < V.R) rs = P.tile_rec[V.tile_begin + (int64_t)cz * V.R + r];
+    s_rec[dz][dy][slot] = rs;
+  }
+  __syncthreads();
+  if (tid < 16) {
+    const int dz = tid >> 3, dy = (tid >> 2) & 1, slot = tid & 3;
+    uint32_t pre = 0;
+#pragma unroll
+    for (int w = 0; w < kCPW; ++w) {
+      s_pre[dz][dy][slot][w] = pre;
+      pre += (uint32_t)__popcll(s_mask[dz][dy][slot][w]);
+    }
+  }
+  __syncthreads();
+  float* const gz = gvol + (int64_t)z * XY;
+#pragma unroll
+  for (int j = 0; j < kGatherSpan / 256; ++j) {
+    const uint32_t q = q0 + (uint32_t)(j * 256 + tid);
+    if (q >= XY) break;
+    const uint32_t y = V.divX.div(q);
+    const int x = (int)(q - y * (uint32_t)X);
     float g = 0.0f;
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
-      // cube whose corner k is this voxel
-      const int cx = x - corner_dx(k), cy = (int)y - corner_dy(k), cz = z - corner_dz(k);
+      // the cube whose corner k is this voxel
+      const int dx = corner_dx(k), dy = corner_dy(k), dz = corner_dz(k);
+      const int cx = x - dx, cy = (int)y - dy, cz = z - dz;
       if (cx < 0 || cy < 0 || cz < 0 || cx > X - 2 || cy > Y - 2 || cz > Z - 2) continue;
-      const int64_t p = (int64_t)cy * X + cx;
-      const int64_t t = V.tile_begin + (int64_t)cz * V.R + (p / kTile);
-      const int bit = (int)(p % kTile);
-      const uint64_t* mk = P.masks + t * kCPW;
-      const uint64_t w = mk[bit >> 6];
-      if (!((w >> (bit & 63)) & 1ull)) continue;
-      int rank = __popcll(w & ((1ull << (bit & 63)) - 1ull));
-      for (int j = 0; j < (bit >> 6); ++j) rank += __popcll(mk[j]);
-      g += contrib[(P.tile_rec[t] + rank) * 8 + k];
+      const uint32_t p = (uint32_t)cy * (uint32_t)X + (uint32_t)cx;
+      const int slot = (int)(p >> 9) - base[dy];
+      const int word = (int)((p >> 6) & 7u), bit = (int)(p & 63u);
+      const uint64_t m = s_mask[dz][dy][slot][word];
+      if (!((m >> bit) & 1ull)) continue;
+      const uint32_t rank = s_pre[dz][dy][slot][word] + (uint32_t)__popcll(m & ((1ull << bit) - 1ull));
+      g += contrib[(s_rec[dz][dy][slot] + rank) * 8 + k];
     }
-    gvol[i] = g;
+    gz[q] = g;
   }
 }
 
@@ -2218,10 +2266,9 @@ int backward_impl(const float* d_vol, int X, int Y, int Z, float h, const float*
   hipLaunchKernelGGL(contrib_kernel<false>, dim3(g1), dim3(kEmitThreads), 0, s, P, d_grad_pos,
                      contrib);
   MCX_HIP(hipGetLastError());
-  int64_t g2 = (N + 255) / 256;
-  if (g2 > 1 << 20) g2 = 1 << 20;
-  hipLaunchKernelGGL(gather_kernel<false>, dim3((unsigned)g2), dim3(256), 0, s, P, contrib,
-                     d_grad_vol, 0);
+  const int64_t g2 = ((int64_t)X * Y + kGatherSpan - 1) / kGatherSpan * Z;
+  if (g2 >= ((int64_t)1 << 31)) return MCX_ERR_UNSUPPORTED;
+  hipLaunchKernelGGL(gather_kernel, dim3((unsigned)g2), dim3(256), 0, s, P, contrib, d_grad_vol);
   MCX_HIP(hipGetLastError());
   return MCX_OK;
 }
