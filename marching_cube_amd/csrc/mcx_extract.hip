@@ -1175,7 +1175,7 @@ constexpr int kWaveMeta = 64 * 4;
 constexpr int kPackedMap = kWaveRec, kPackedEx = kPackedMap + kWaveMap;
 constexpr int kPackedMeta = kPackedEx + kWaveEx;
 constexpr int kPackedWords = kPackedMeta + kWaveMeta;
-constexpr int kNodeTable = 4096;  // node offsets for grids up to 4096 per axis
+constexpr int kNodeTable = 2048;  // node offsets for grids up to 2048 per axis (else divisions)
 
 // Emit the vertices of the (up to 64) records in the wave's LDS area s_rec
 // (record r at s_rec[9r]; n of them) to output indices out, out + 1, ...:
@@ -1457,7 +1457,9 @@ __device__ __forceinline__ void emit_staged_tile(const ExtractParams& P, const V
 // unit's vertex counts, those before the unit (first entry only) are summed
 // from their case bytes.
 template <int ELEM>
-__global__ __launch_bounds__(kEmitThreads) void emit_units_kernel(ExtractParams P) {
+// Six waves per SIMD (at most 80 VGPRs, 25.6 KB of LDS per workgroup).
+__global__ __launch_bounds__(kEmitThreads) __attribute__((amdgpu_waves_per_eu(6)))
+void emit_units_kernel(ExtractParams P) {
   __shared__ __attribute__((aligned(16))) uint8_t s_slot[256 * 16];
   __shared__ uint8_t s_nv[256];
   __shared__ uint32_t s_wave[kEmitThreads / 64][kPackedWords];

@@ -22,7 +22,7 @@ import os
 import sys
 
 EXTRACT_KERNELS = ("count4_kernel", "count_kernel", "scan_kernel", "emit_kernel",
-                   "emit_packed_kernel", "emit_units_kernel")
+                   "emit_units_kernel")
 COPY_BYTES = 512 << 20
 
 

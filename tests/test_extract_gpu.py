@@ -178,6 +178,14 @@ def test_large_tiles_and_mixed_staged_path(dev, records):
         assert r.ver_voxel.cpu().numpy().tobytes() == ref["ver_voxel"].tobytes()
 
 
+@pytest.mark.parametrize("shape", [(4, 6, 2600), (2050, 3, 5)])
+def test_long_axis_without_node_table(dev, shape):
+    # an axis above 2048 voxels: the emit kernel divides on every axis
+    # instead of reading its per-workgroup node-offset table
+    vol = synth.noise_volume(shape, seed=13, p_zero_w=0.05)
+    check_against_oracle(vol, 0.0037, dev)
+
+
 def test_normals_bit_exact(dev):
     for vol, h in [synth.sphere_volume(64), (synth.noise_volume((12, 13, 14), seed=3), 0.02)]:
         print(f"[case] normals {vol.shape}", flush=True)
