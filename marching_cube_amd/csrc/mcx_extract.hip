@@ -1170,7 +1170,8 @@ constexpr int kWaveMap = 64 * 15 / 4;     // vertex -> record (u8), at most 15 v
 constexpr int kWaveEx = 64;               // first vertex of each record
 constexpr int kWaveWords = kWaveRec + kWaveOut + kWaveMap + kWaveEx;
 // Packed emission (single volumes): records, vertex -> record map, first
-// vertex of each record, and per record its output index (2 words), p and z.
+// vertex of each record, and per record its output index and cube (x, y, z)
+// in 4 words (meta_hi).
 constexpr int kWaveMeta = 64 * 4;
 constexpr int kPackedMap = kWaveRec, kPackedEx = kPackedMap + kWaveMap;
 constexpr int kPackedMeta = kPackedEx + kWaveEx;
@@ -1237,6 +1238,13 @@ __device__ __forceinline__ int emit_records(const ExtractParams& P, const VolDes
     wave_sync();
   }
   return tot;
+}
+
+// Word 1 of a record's metadata in the emit kernels: bits 32..39 of its
+// output index (saturated: an index of 2^40 or more lies past any capacity)
+// and the cube's z (< 2^24).  Words 0, 2, 3: the index's low bits, x, y.
+__device__ __forceinline__ uint32_t meta_hi(uint64_t out, uint32_t z) {
+  return (uint32_t)min(out >> 32, (uint64_t)0xff) | (z << 8);
 }
 
 // One output vertex, slot q of the record at rc (LDS: word 0 = cube offset
@@ -1333,8 +1341,8 @@ __device__ __forceinline__ int64_t record_vertex(const VolDesc& V, const uint8_t
 
 // Emit the vertices of a packed block: nb records (up to 64, possibly of
 // several tiles) in the wave's record area, record r with its output index
-// (of its first vertex) and cube position (p = in-plane cube index, z) in
-// s_meta[4r..4r+3].  The records' vertex counts are scanned across the wave,
+// (of its first vertex) and cube (x, y, z) in s_meta[4r..4r+3] (words: index
+// bits 0..31, meta_hi, x, y).  The records' vertex counts are scanned across the wave,
 // every record writes its index into the vertex -> record map, and each lane
 // produces one vertex of a round of 64 and stores its record fields at its
 // output index (the vertex's own slot; consecutive lanes mostly write
@@ -1365,14 +1373,12 @@ __device__ __forceinline__ void emit_block(const ExtractParams& P, const VolDesc
       const uint32_t* rc = s_rec + r * kRecWords;
       const uint32_t* mt = s_meta + 4 * r;
       const int slot = j - (int)s_ex[r];
-      const int64_t out = (int64_t)(((uint64_t)mt[1] << 32) | mt[0]) + slot;
+      const int64_t out = (int64_t)(((uint64_t)(mt[1] & 0xffu) << 32) | mt[0]) + slot;
       if (out < P.cap) {
-        const uint32_t p = mt[2];
-        const uint32_t y = V.divX.div(p);
-        const int x = (int)(p - y * (uint32_t)V.X);
         uint32_t o[7];
-        const int64_t key = record_vertex<ELEM>(V, s_slot, rc, x, (int)y, (int)mt[3], slot, o,
-                                                P.ekey != nullptr, node_off);
+        const int64_t key = record_vertex<ELEM>(V, s_slot, rc, (int)mt[2], (int)mt[3],
+                                                (int)(mt[1] >> 8), slot, o, P.ekey != nullptr,
+                                                node_off);
         if (vv) {
 #pragma unroll
           for (int f = 0; f < 7; ++f) vv[7 * out + f] = o[f];
@@ -1434,11 +1440,12 @@ __device__ __forceinline__ void emit_staged_tile(const ExtractParams& P, const V
 #pragma unroll
       for (int q = 0; q < 8; ++q) rp[1 + q] = cw[q];
       const uint64_t ob = (uint64_t)(out + (int64_t)(vincl - nvc));
+      const uint32_t y = V.divX.div((uint32_t)p);
       uint32_t* mt = s_meta + 4 * rk;
       mt[0] = (uint32_t)ob;
-      mt[1] = (uint32_t)(ob >> 32);
-      mt[2] = (uint32_t)p;
-      mt[3] = (uint32_t)z;
+      mt[1] = meta_hi(ob, (uint32_t)z);
+      mt[2] = (uint32_t)p - y * (uint32_t)X;
+      mt[3] = y;
     }
     out += (int64_t)__builtin_amdgcn_readlane(vincl, 63);
     if (n) emit_block<ELEM>(P, V, s_slot, s_nv, s_w, n, node_off);
@@ -1570,11 +1577,13 @@ void emit_units_kernel(ExtractParams P) {
 #pragma unroll
       for (int q = 0; q < kRecWords; ++q) s_w[lane * kRecWords + q] = rec[q];
       const uint64_t o = (uint64_t)(offk + (int64_t)(vex - vfirst));
+      const uint32_t p = p0k + (rec[0] >> 8);
+      const uint32_t y = V.divX.div(p);
       uint32_t* mt = s_meta + 4 * lane;
       mt[0] = (uint32_t)o;
-      mt[1] = (uint32_t)(o >> 32);
-      mt[2] = p0k + (rec[0] >> 8);
-      mt[3] = zk;
+      mt[1] = meta_hi(o, zk);
+      mt[2] = p - y * (uint32_t)V.X;
+      mt[3] = y;
     }
     emit_block<ELEM>(P, V, s_slot, s_nv, s_w, R, node_off);
   }
@@ -2071,6 +2080,7 @@ VolDesc make_vol(const void* d_vol, int X, int Y, int Z, float h, int16_t* ver_v
 // record slots.
 int prepare_single(ExtractParams& P, const void* d_vol, int X, int Y, int Z, float h,
                    int16_t* ver_voxel, void* d_ws, size_t ws_bytes) {
+  if (Z > (1 << 24)) return MCX_ERR_UNSUPPORTED;  // z is packed in 24 bits by the emit kernel
   const int64_t n_tiles = tiles_of(X, Y, Z);
   const int rc = layout_ws(d_ws, ws_bytes, n_tiles, P);
   if (rc) return rc;
