@@ -438,18 +438,18 @@ __global__ __launch_bounds__(256) void count_kernel(ExtractParams P) {
 }
 
 // ============================================================ K1, wide form
-// SHORT2 volumes with X % 4 == 0 (16-byte aligned planes and rows).  128-bit
-// loads: in chunk c (256 cubes) of a tile, lane L holds the voxels 4L..4L+3
-// of each of the four corner rows (rows y, y+1 of planes z, z+1).  Per voxel
-// position the four rows are reduced in the vector ALU with packed 16-bit
-// min/max (low half: min and max tsdf; high half: min weight), and three
-// ballots per position give "some row negative", "some row non-negative" and
-// "some weight zero" over the 64 lanes.  A cube (positions k, k + 1) is active
-// when its two positions together have both signs and no zero weight; only
-// position 4 (the next lane's position 0) needs a one-bit mask shift.  Active
-// cube 4L + k of the chunk is in cube order, so a record's rank within the
-// tile is a sum of masked popcounts.  Records go to the column's quota, or,
-// when the tile does not fit there, to a shared overflow pool (one atomic per
+// SHORT2 volumes of any width.  128-bit loads: in chunk c (256 cubes) of a
+// tile, lane L holds the voxels 4L..4L+3 of each of the four corner rows
+// (rows y, y+1 of planes z, z+1); tiles start at multiples of 512 voxels,
+// rows and planes of a width X % 4 != 0 at 4-byte alignment, which the
+// vector memory path accepts.  The rows' tsdf signs become bytes (0xff:
+// negative); OR / AND over the four rows and over positions k and k + 1 (the
+// next lane's position 0 by one DPP lane shift) mark the cubes with both
+// signs, four per lane, masked by the cubes that exist (a row may end inside
+// a lane); only chunks with such cubes test the weights.  Active cube
+// 4L + k of the chunk is in cube order, so a record's rank within the tile
+// is a sum of masked popcounts.  Records go to the column's quota, or, when
+// the tile does not fit there, to a shared overflow pool (one atomic per
 // overflowing tile); only when that is exhausted does the tile take the
 // staged path.  No active-cube masks are written (the backward runs on fp32
 // volumes, which take the 32-bit kernel).
@@ -558,29 +558,25 @@ void count4_kernel(ExtractParams P) {
   const int64_t p0 = (int64_t)r * kTile;
   const int64_t rec_base = col * P.quota;
 
-  // lanes whose 4 voxels lie on a row without cubes (the last row, or past the
-  // plane), and lanes at the end of a row (their position 4 starts the next row)
-  // as word masks: lane_ok = 0 on lanes without cubes, lane_mid = 0 on lanes
-  // at a row end (where position 4 is not a neighbour)
-  uint32_t lane_ok[2], lane_mid[2];
+  // byte k of cube_mask[c] = 0xff when cube k of the lane exists: in-plane
+  // index p + k inside the plane, not in the last row and not the last of
+  // its row (its x + 1 neighbour, position k + 1, is then in the same row;
+  // a row may end inside a lane when X % 4 != 0)
+  uint32_t cube_mask[2];
 #pragma unroll
   for (int c = 0; c < 2; ++c) {
     const int64_t p = p0 + 256 * c + 4 * lane;
-    bool ok = false, end = false;
-    if (p < XY) {
-      const uint32_t y = V.divX.div((uint32_t)p);
-      const uint32_t x = (uint32_t)p - y * (uint32_t)X;
-      ok = y + 1 < (uint32_t)V.Y;
-      end = x + 4 == (uint32_t)X;
-    }
-    lane_ok[c] = ok ? 0xffffffffu : 0u;
-    lane_mid[c] = end ? 0u : 0xffffffffu;
-  }
-  // byte k of cube_mask[c]: cube k of the lane exists (lane_ok; cube 3 also
-  // needs the next position in the row)
-  uint32_t cube_mask[2];
+    uint32_t mk = 0;
 #pragma unroll
-  for (int c = 0; c < 2; ++c) cube_mask[c] = lane_ok[c] & (lane_mid[c] | 0x00ffffffu);
+    for (int k = 0; k < 4; ++k) {
+      if (p + k < XY) {
+        const uint32_t y = V.divX.div((uint32_t)(p + k));
+        const uint32_t x = (uint32_t)(p + k) - y * (uint32_t)X;
+        if (y + 1 < (uint32_t)V.Y && x + 1 < (uint32_t)X) mk |= 0xffu << (8 * k);
+      }
+    }
+    cube_mask[c] = mk;
+  }
 
   // One buffer resource over the planes the column reads (z0 .. z1); the
   // plane offset is part of the VGPR offset, which the hardware range-checks
@@ -2097,7 +2093,7 @@ int prepare_single(ExtractParams& P, const void* d_vol, int X, int Y, int Z, flo
   S = (Z - 1 + P.slab - 1) / P.slab;
   P.ncols = R * S;
   P.slab_max = P.slab;
-  P.wide = X % 4 == 0 && (reinterpret_cast<uintptr_t>(d_vol) & 15u) == 0 &&
+  P.wide = (reinterpret_cast<uintptr_t>(d_vol) & 3u) == 0 &&
            span_ok((int64_t)X * Y, P.slab);
   return MCX_OK;
 }
@@ -2344,7 +2340,7 @@ extern "C" int mcx_batch_create(mcx_batch** out, int nvol, const int* dims_xyz, 
     tb += nt;
     cubes += cubes_of(X, Y, Z);
     maxX = std::max(maxX, X);
-    wide = wide && X % 4 == 0 && (reinterpret_cast<uintptr_t>(d_vols[b]) & 15u) == 0 &&
+    wide = wide && (reinterpret_cast<uintptr_t>(d_vols[b]) & 3u) == 0 &&
            span_ok((int64_t)X * Y, g_tuning.batch_slab);
   }
   mcx_batch* p = new mcx_batch();

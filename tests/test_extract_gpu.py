@@ -157,7 +157,7 @@ def layered_volume(shape, seed):
 
 @pytest.mark.parametrize("records", [300, 3000, None])
 def test_large_tiles_and_mixed_staged_path(dev, records):
-    # 128-bit count kernel (X % 4 == 0): tiles of up to 511 records, so the
+    # 128-bit count kernel: tiles of up to 511 records, so the
     # 64-record emission units start deep inside tiles; with a small record
     # scratch, staged-path tiles are interleaved with tiles that kept their
     # records
