@@ -165,6 +165,8 @@ struct ExtractParams {
   int64_t* lrs;          // tile_rec of clist entry j
   uint32_t* units;       // clist entry holding record 64 k, for every unit k
   uint32_t* dense_list;  // the list's tiles that took the staged path
+  uint32_t* nz_vol;      // batched launches: volume of each list entry (nz_list order)
+  uint32_t* clist_vol;   // batched launches: volume of each clist entry
   bool write_masks;
   uint32_t* rec;
   int64_t rec_cap;
@@ -421,6 +423,7 @@ __global__ __launch_bounds__(256) void count_kernel(ExtractParams P) {
         if ((nzmask >> lane) & 1ull) {
           const uint32_t i = (uint32_t)__popcll(nzmask & ((1ull << lane) - 1ull));
           P.nz_list[base + i] = (uint32_t)(V.tile_begin + (int64_t)(zb + lane) * V.R + r);
+          if constexpr (BATCH) P.nz_vol[base + i] = (uint32_t)vi;
         }
       }
       nzmask = 0;
@@ -718,6 +721,7 @@ void count4_kernel(ExtractParams P) {
           if ((nzmask >> lane) & 1ull) {
             const uint32_t i = (uint32_t)__popcll(nzmask & ((1ull << lane) - 1ull));
             P.nz_list[base + i] = (uint32_t)(V.tile_begin + (int64_t)(zb + lane) * V.R + r);
+            if constexpr (BATCH) P.nz_vol[base + i] = (uint32_t)vi;
           }
         }
         nzmask = 0;
@@ -861,6 +865,7 @@ void count4_kernel(ExtractParams P) {
         if ((nzmask >> lane) & 1ull) {
           const uint32_t i = (uint32_t)__popcll(nzmask & ((1ull << lane) - 1ull));
           P.nz_list[base + i] = (uint32_t)(V.tile_begin + (int64_t)(zb + lane) * V.R + r);
+          if constexpr (BATCH) P.nz_vol[base + i] = (uint32_t)vi;
         }
       }
       nzmask = 0;
@@ -931,9 +936,10 @@ __global__ __launch_bounds__(256) void scan_kernel(ExtractParams P) {
   const int c = (int)((L + 255) / 256);                                     // <= kScanPer
   const int64_t lb = min(blk * L, (int64_t)nz), le = min(lb + L, (int64_t)nz);
   const int64_t j0 = lb + (int64_t)tid * c;
-  uint32_t tl[kScanPer];
+  uint32_t tl[kScanPer], vl[kScanPer];
   int32_t n[kScanPer];
   int64_t rsl[kScanPer];
+  const bool batch = P.vols != nullptr;  // entries carry their volume
   uint32_t keep = 0;  // bit k: entry j0 + k has records
   int64_t sn = 0, se = 0;
   if (lst) {
@@ -941,6 +947,8 @@ __global__ __launch_bounds__(256) void scan_kernel(ExtractParams P) {
     // entries outside the range read a valid one and are dropped
 #pragma unroll
     for (int k = 0; k < kScanPer; ++k) tl[k] = P.nz_list[min(j0 + k, (int64_t)nz - 1)];
+#pragma unroll
+    for (int k = 0; k < kScanPer; ++k) vl[k] = batch ? P.nz_vol[min(j0 + k, (int64_t)nz - 1)] : 0u;
 #pragma unroll
     for (int k = 0; k < kScanPer; ++k) {
       n[k] = P.tile_nrec[tl[k]];
@@ -1023,6 +1031,7 @@ __global__ __launch_bounds__(256) void scan_kernel(ExtractParams P) {
         P.clist[e] = tl[k];
         P.lpre[e] = r;
         P.lrs[e] = rsl[k];
+        if (batch) P.clist_vol[e] = vl[k];
         for (int64_t u = (r + 63) >> 6; (u << 6) < r + n[k]; ++u) P.units[u] = (uint32_t)e;
         r += n[k];
         ++e;
@@ -1079,162 +1088,18 @@ struct TileInfo {      // backward BW1 per-tile info in LDS (10 words)
 };
 static_assert(sizeof(TileInfo) == 40, "tile info is 10 words");
 
-// One output vertex: slot q of cube (x, y, z) with case ci and corner words
-// cw[8] in the reference corner order.  The 7 words of its vertex_voxel
-// record go to the wave's staging area o[0..6]; ver_voxel is written here.
-// node_off (optional): for every grid index i, floor((i + 0.5) h / h - 0.5) - i
-// in the same fp32 operations; along the two axes the edge does not run on
-// the vertex sits exactly on its corners' node coordinate, so only the edge
-// axis needs the division.
-template <int ELEM>
-__device__ __forceinline__ void make_vertex(const VolDesc& V, const uint64_t* s_case,
-                                            const uint32_t cw[8], int x, int y, int z,
-                                            uint32_t ci, int q, bool write_vv, uint32_t* o,
-                                            int64_t* key = nullptr,
-                                            const int8_t* node_off = nullptr) {
-  // output slot q = 3*tri + r holds table corner 2 - r of triangle tri (:324-332)
-  const int m = 3 * (q / 3) + 2 - (q % 3);
-  const int e = (int)((s_case[ci] >> (4 * m)) & 0xfu);
-  const int ca = (int)((kEdgeA >> (3 * e)) & 7u);
-  const int cb = (int)((kEdgeB >> (3 * e)) & 7u);
-  if (key) {
-    // the grid edge: its lower corner and its axis (the two corners differ
-    // along exactly one axis); shared by the up to 4 cubes around the edge
-    const int ax = corner_dx(ca) != corner_dx(cb) ? 0 : (corner_dy(ca) != corner_dy(cb) ? 1 : 2);
-    const int mx = x + min(corner_dx(ca), corner_dx(cb));
-    const int my = y + min(corner_dy(ca), corner_dy(cb));
-    const int mz = z + min(corner_dz(ca), corner_dz(cb));
-    *key = (((int64_t)mz * V.Y + my) * V.X + mx) * 3 + ax;
-  }
-  const float h = V.h;
-  const float fa = unpack_f<ELEM>(pick8(cw, ca));
-  const float fb = unpack_f<ELEM>(pick8(cw, cb));
-  const float pax = ((float)(x + corner_dx(ca)) + 0.5f) * h;
-  const float pay = ((float)(y + corner_dy(ca)) + 0.5f) * h;
-  const float paz = ((float)(z + corner_dz(ca)) + 0.5f) * h;
-  const float pbx = ((float)(x + corner_dx(cb)) + 0.5f) * h;
-  const float pby = ((float)(y + corner_dy(cb)) + 0.5f) * h;
-  const float pbz = ((float)(z + corner_dz(cb)) + 0.5f) * h;
-  const float tt = (0.0f - fa) / (fb - fa + 1e-15f);
-  const float px = __fmaf_rn(tt, pbx - pax, pax);
-  const float py = __fmaf_rn(tt, pby - pay, pay);
-  const float pz = __fmaf_rn(tt, pbz - paz, paz);
-  // record fields (store_point :336-367)
-  int vx, vy, vz;
-  if (node_off) {
-    const int ax = corner_dx(ca) != corner_dx(cb) ? 0 : (corner_dy(ca) != corner_dy(cb) ? 1 : 2);
-    float ce = (ax == 0 ? px : (ax == 1 ? py : pz)) / h;
-    ce = ce - 0.5f;
-    const int ve = (int)floorf(ce);
-    const int ix = x + corner_dx(ca), iy = y + corner_dy(ca), iz = z + corner_dz(ca);
-    vx = ax == 0 ? ve : ix + node_off[ix];
-    vy = ax == 1 ? ve : iy + node_off[iy];
-    vz = ax == 2 ? ve : iz + node_off[iz];
-  } else {
-    float cx = px / h, cy = py / h, cz = pz / h;
-    cx = cx - 0.5f;
-    cy = cy - 0.5f;
-    cz = cz - 0.5f;
-    vx = (int)floorf(cx);
-    vy = (int)floorf(cy);
-    vz = (int)floorf(cz);
-  }
-  const bool inb = vx >= 0 && vy >= 0 && vz >= 0 && vx < V.X && vy < V.Y && vz < V.Z;
-  uint32_t tw = 0;
-  auto lin = [&]() { return ((int64_t)vz * V.Y + vy) * V.X + vx; };
-  if (inb) {
-    if constexpr (ELEM == MCX_ELEM_SHORT2) {
-      // the voxel is almost always one of the cube's corners
-      const unsigned dx = (unsigned)(vx - x), dy = (unsigned)(vy - y), dz = (unsigned)(vz - z);
-      tw = (dx | dy | dz) <= 1u ? pick8(cw, corner_of((int)dx, (int)dy, (int)dz)) : V.vol[lin()];
-    }
-  }
-  o[0] = __float_as_uint(px);
-  o[1] = __float_as_uint(py);
-  o[2] = __float_as_uint(pz);
-  o[3] = (uint32_t)vx;
-  o[4] = (uint32_t)vy;
-  o[5] = (uint32_t)vz;
-  o[6] = tw;
-  if (write_vv && V.ver_voxel && inb) V.ver_voxel[lin()] = (int16_t)(tw & 0xffffu);
-}
-
-// Per-wave LDS areas of the emit kernel (32-bit words).
-constexpr int kWaveRec = 64 * kRecWords;  // 64 records (cube offset << 8 | case, 8 corners)
-constexpr int kWaveOut = 64 * 7;          // 64 staged vertex_voxel records
-constexpr int kWaveMap = 64 * 15 / 4;     // vertex -> record (u8), at most 15 vertices a record
-constexpr int kWaveEx = 64;               // first vertex of each record
-constexpr int kWaveWords = kWaveRec + kWaveOut + kWaveMap + kWaveEx;
-// Packed emission (single volumes): records, vertex -> record map, first
-// vertex of each record, and per record its output index and cube (x, y, z)
-// in 4 words (meta_hi).
+// Per-wave LDS area of the emit kernel (32-bit words): 64 records (cube
+// offset << 8 | case, 8 corners), the vertex -> record map (u8, at most 15
+// vertices a record), the first vertex of each record, and per record its
+// output index and cube (x, y, z) in 4 words (meta_hi).
+constexpr int kWaveRec = 64 * kRecWords;
+constexpr int kWaveMap = 64 * 15 / 4;
+constexpr int kWaveEx = 64;
 constexpr int kWaveMeta = 64 * 4;
 constexpr int kPackedMap = kWaveRec, kPackedEx = kPackedMap + kWaveMap;
 constexpr int kPackedMeta = kPackedEx + kWaveEx;
 constexpr int kPackedWords = kPackedMeta + kWaveMeta;
 constexpr int kNodeTable = 2048;  // node offsets for grids up to 2048 per axis (else divisions)
-
-// Emit the vertices of the (up to 64) records in the wave's LDS area s_rec
-// (record r at s_rec[9r]; n of them) to output indices out, out + 1, ...:
-// the records' vertex counts are scanned across the wave, every record writes
-// its index into the vertex -> record map, and each lane then produces one
-// vertex of a round of 64, staged in s_out and written with consecutive
-// lanes on consecutive words.  p0 and z place the tile.  Returns the number
-// of vertices (wave-uniform).
-template <int ELEM>
-__device__ __forceinline__ int emit_records(const ExtractParams& P, const VolDesc& V,
-                                            const uint64_t* s_case, const uint8_t* s_nv,
-                                            uint32_t* s_w, int n, int z, int64_t p0,
-                                            int64_t out) {
-  uint32_t* s_rec = s_w;
-  uint32_t* s_out = s_w + kWaveRec;
-  uint8_t* s_map = reinterpret_cast<uint8_t*>(s_w + kWaveRec + kWaveOut);
-  uint32_t* s_ex = s_w + kWaveRec + kWaveOut + kWaveMap;
-  const int lane = threadIdx.x & 63;
-  const int nv = lane < n ? s_nv[s_rec[lane * kRecWords] & 0xffu] : 0;
-  const uint32_t incl = wave_incl_scan32((uint32_t)nv);
-  const int ex = (int)incl - nv;
-  const int tot = __shfl((int)incl, 63);
-  s_ex[lane] = (uint32_t)ex;
-  for (int q = 0; q < nv; ++q) s_map[ex + q] = (uint8_t)lane;
-  wave_sync();
-  for (int vb = 0; vb < tot; vb += 64) {
-    const int64_t vbase = out + vb;
-    const int nr = min(64, tot - vb);
-    const int64_t room = P.cap - vbase;
-    const int nw = room <= 0 ? 0 : (room < nr ? (int)room : nr);  // vertices below the capacity
-    if (lane < nw) {
-      const int j = vb + lane;
-      const int r = s_map[j];
-      const uint32_t* rc = s_rec + r * kRecWords;
-      const uint32_t rw = rc[0];
-      uint32_t c[8];
-#pragma unroll
-      for (int k = 0; k < 8; ++k) c[k] = rc[1 + k];
-      const int64_t p = p0 + (rw >> 8);
-      const uint32_t y = V.divX.div((uint32_t)p);
-      const int x = (int)((uint32_t)p - y * (uint32_t)V.X);
-      int64_t key;
-      make_vertex<ELEM>(V, s_case, c, x, (int)y, z, rw & 0xffu, j - (int)s_ex[r], true,
-                        s_out + 7 * lane, P.ekey ? &key : nullptr);
-      if (P.ekey) P.ekey[vbase + lane] = key;
-    }
-    wave_sync();
-    if (P.vv) {
-      uint32_t* dst = reinterpret_cast<uint32_t*>(P.vv) + 7 * vbase;
-      for (int w = lane; w < 7 * nw; w += 64) dst[w] = s_out[w];
-    }
-    if (P.pos || P.v2vox) {
-      for (int w = lane; w < 3 * nw; w += 64) {
-        const int v = w / 3, f = w - 3 * v;
-        if (P.pos) P.pos[3 * vbase + w] = __uint_as_float(s_out[7 * v + f]);
-        if (P.v2vox) P.v2vox[3 * vbase + w] = (int32_t)s_out[7 * v + 3 + f];
-      }
-    }
-    wave_sync();
-  }
-  return tot;
-}
 
 // Word 1 of a record's metadata in the emit kernels: bits 32..39 of its
 // output index (saturated: an index of 2^40 or more lies past any capacity)
@@ -1245,7 +1110,9 @@ __device__ __forceinline__ uint32_t meta_hi(uint64_t out, uint32_t z) {
 
 // One output vertex, slot q of the record at rc (LDS: word 0 = cube offset
 // << 8 | case, words 1..8 = the corner words in corner order) of cube
-// (x, y, z), in the same fp32 operations as make_vertex; returns the grid
+// (x, y, z), in the fp32 operations of cu_marching_cubes_v1.cu:243-264, :336-367 (
+// node coordinate (i + 0.5) h, t = (0 - fa) / (fb - fa + 1e-15f), fma
+// interpolation, voxel floor(p / h - 0.5)); returns the grid
 // edge key (lower corner voxel * 3 + axis) when want_key: the edge's corners
 // come from the slot table and their words are read directly; along the two
 // axes the edge does not run on, the interpolated coordinate is exactly the
@@ -1343,11 +1210,11 @@ __device__ __forceinline__ int64_t record_vertex(const VolDesc& V, const uint8_t
 // produces one vertex of a round of 64 and stores its record fields at its
 // output index (the vertex's own slot; consecutive lanes mostly write
 // consecutive records).
-template <int ELEM>
+template <int ELEM, bool BATCH>
 __device__ __forceinline__ void emit_block(const ExtractParams& P, const VolDesc& V,
                                            const uint8_t* s_slot, const uint8_t* s_nv,
-                                           uint32_t* s_w, int nb,
-                                           const int8_t* node_off = nullptr) {
+                                           uint32_t* s_w, int nb, const int8_t* node_off,
+                                           const uint32_t* s_vi) {
   const uint32_t* s_rec = s_w;
   uint8_t* s_map = reinterpret_cast<uint8_t*>(s_w + kPackedMap);
   uint32_t* s_ex = s_w + kPackedEx;
@@ -1372,7 +1239,8 @@ __device__ __forceinline__ void emit_block(const ExtractParams& P, const VolDesc
       const int64_t out = (int64_t)(((uint64_t)(mt[1] & 0xffu) << 32) | mt[0]) + slot;
       if (out < P.cap) {
         uint32_t o[7];
-        const int64_t key = record_vertex<ELEM>(V, s_slot, rc, (int)mt[2], (int)mt[3],
+        const VolDesc& Vr = BATCH ? P.vols[s_vi[r]] : V;  // the record's volume
+        const int64_t key = record_vertex<ELEM>(Vr, s_slot, rc, (int)mt[2], (int)mt[3],
                                                 (int)(mt[1] >> 8), slot, o, P.ekey != nullptr,
                                                 node_off);
         if (vv) {
@@ -1396,12 +1264,15 @@ __device__ __forceinline__ void emit_block(const ExtractParams& P, const VolDesc
 
 // A tile whose records did not fit the scratch, re-read from the volume 64
 // cubes at a time (cube order) into the record form and emitted from voff[t].
-template <int ELEM>
-__device__ __forceinline__ void emit_staged_tile(const ExtractParams& P, const VolDesc& V,
-                                                 const uint8_t* s_slot, const uint8_t* s_nv,
-                                                 uint32_t* s_w, uint32_t t,
-                                                 const int8_t* node_off) {
+template <int ELEM, bool BATCH>
+__device__ __forceinline__ void emit_staged_tile(const ExtractParams& P, const uint8_t* s_slot,
+                                                 const uint8_t* s_nv, uint32_t* s_w,
+                                                 uint32_t t, const int8_t* node_off,
+                                                 uint32_t* s_vi) {
   const int lane = threadIdx.x & 63;
+  const int vi = BATCH ? volume_of_tile(P, t) : 0;
+  const VolDesc& V = vol_at<BATCH>(P, vi);
+  if constexpr (BATCH) s_vi[lane] = (uint32_t)vi;
   uint32_t* const s_meta = s_w + kPackedMeta;
   const uint32_t local = (uint32_t)(t - V.tile_begin);
   const int z = (int)V.divR.div(local);
@@ -1444,7 +1315,7 @@ __device__ __forceinline__ void emit_staged_tile(const ExtractParams& P, const V
       mt[3] = y;
     }
     out += (int64_t)__builtin_amdgcn_readlane(vincl, 63);
-    if (n) emit_block<ELEM>(P, V, s_slot, s_nv, s_w, n, node_off);
+    if (n) emit_block<ELEM, BATCH>(P, V, s_slot, s_nv, s_w, n, node_off, s_vi);
   }
 }
 
@@ -1459,15 +1330,19 @@ __device__ __forceinline__ void emit_staged_tile(const ExtractParams& P, const V
 // the tile's records before it: those in the unit come from a scan of the
 // unit's vertex counts, those before the unit (first entry only) are summed
 // from their case bytes.
-template <int ELEM>
-// Six waves per SIMD (at most 80 VGPRs, 25.6 KB of LDS per workgroup).
+// Six waves per SIMD (at most 80 VGPRs, 25.6 KB of LDS per workgroup, 26.6 KB
+// batched).  Batched launches: every clist entry carries its volume, each
+// record's volume index is kept beside its metadata, and there is no
+// node-offset table (the grid spacing differs between volumes).
+template <int ELEM, bool BATCH>
 __global__ __launch_bounds__(kEmitThreads) __attribute__((amdgpu_waves_per_eu(6)))
 void emit_units_kernel(ExtractParams P) {
   __shared__ __attribute__((aligned(16))) uint8_t s_slot[256 * 16];
   __shared__ uint8_t s_nv[256];
   __shared__ uint32_t s_wave[kEmitThreads / 64][kPackedWords];
   __shared__ uint32_t s_mark[kEmitThreads / 64][64];
-  __shared__ int8_t s_node[kNodeTable];
+  __shared__ int8_t s_node[BATCH ? 1 : kNodeTable];
+  __shared__ uint32_t s_vol[BATCH ? kEmitThreads / 64 : 1][64];  // volume of each record
   __shared__ int s_node_ok;
   static_assert(kEmitThreads * 16 == sizeof(SlotTable), "16 table bytes per thread");
   reinterpret_cast<u32x4*>(s_slot)[threadIdx.x] =
@@ -1476,11 +1351,11 @@ void emit_units_kernel(ExtractParams P) {
   const int lane = threadIdx.x & 63;
   const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
   s_mark[wv][lane] = 0;
-  const VolDesc& V = P.vol0;
-  const int mdim = max(V.X, max(V.Y, V.Z));
+  const VolDesc& V = P.vol0;  // single volumes
+  const int mdim = BATCH ? kNodeTable + 1 : max(V.X, max(V.Y, V.Z));
   if (threadIdx.x == 0) s_node_ok = mdim <= kNodeTable;
   __syncthreads();
-  if (mdim <= kNodeTable) {
+  if (!BATCH && mdim <= kNodeTable) {
     for (int i = threadIdx.x; i < mdim; i += kEmitThreads) {
       const float nd = ((float)i + 0.5f) * V.h;
       float c = nd / V.h;
@@ -1497,6 +1372,7 @@ void emit_units_kernel(ExtractParams P) {
   uint32_t* const s_w = s_wave[wv];
   uint32_t* const mark = s_mark[wv];
   uint32_t* const s_meta = s_w + kPackedMeta;
+  uint32_t* const s_vi = s_vol[BATCH ? wv : 0];
   const uint32_t nl = P.hdr->nz_total ? P.hdr->list_n : 0u;  // clist entries
   const int64_t total = nl ? P.lpre[nl] : 0;  // records of the list
   const int64_t n_units = (total + 63) >> 6;
@@ -1504,7 +1380,8 @@ void emit_units_kernel(ExtractParams P) {
   const int64_t nwaves = (int64_t)gridDim.x * (kEmitThreads / 64);
   for (int64_t k = (int64_t)blockIdx.x * (kEmitThreads / 64) + wv; k < n_work; k += nwaves) {
     if (k >= n_units) {
-      emit_staged_tile<ELEM>(P, V, s_slot, s_nv, s_w, P.dense_list[k - n_units], node_off);
+      emit_staged_tile<ELEM, BATCH>(P, s_slot, s_nv, s_w, P.dense_list[k - n_units], node_off,
+                                    s_vi);
       continue;
     }
     const int64_t g0 = k << 6;
@@ -1516,6 +1393,7 @@ void emit_units_kernel(ExtractParams P) {
     const int64_t nrec = i < nl ? P.lpre[ic + 1] - pre : 0;
     const uint32_t t = i < nl ? P.clist[i] : 0u;
     const int64_t trec = i < nl ? P.lrs[i] : 0;  // the tile's first record
+    const uint32_t vi = BATCH && i < nl ? P.clist_vol[i] : 0u;
     const bool inu = nrec > 0 && pre < g0 + R;  // the entry has records in the unit
     const int first = (int)max(pre - g0, (int64_t)0);  // its first record in the unit
     int64_t rs = 0, off = 0;
@@ -1523,9 +1401,10 @@ void emit_units_kernel(ExtractParams P) {
     if (inu) {
       rs = trec + (g0 + first - pre);  // the record of unit position `first`
       off = P.voff[t];
-      const uint32_t local = (uint32_t)(t - V.tile_begin);
-      z = V.divR.div(local);
-      p0 = (local - z * (uint32_t)V.R) * (uint32_t)kTile;
+      const VolDesc& Vl = vol_at<BATCH>(P, (int)vi);
+      const uint32_t local = (uint32_t)(t - Vl.tile_begin);
+      z = Vl.divR.div(local);
+      p0 = (local - z * (uint32_t)Vl.R) * (uint32_t)kTile;
       mark[first] = (uint32_t)lane + 1u;
     }
     // records of the first entry's tile before the unit: their vertices
@@ -1569,133 +1448,25 @@ void emit_units_kernel(ExtractParams P) {
     const int64_t offk = __shfl(off, kk) + (kk == 0 ? (int64_t)hv : 0);
     const uint32_t p0k = (uint32_t)__shfl((int)p0, kk);
     const uint32_t zk = (uint32_t)__shfl((int)z, kk);
+    const uint32_t vik = BATCH ? (uint32_t)__shfl((int)vi, kk) : 0u;
     if (lane < R) {
 #pragma unroll
       for (int q = 0; q < kRecWords; ++q) s_w[lane * kRecWords + q] = rec[q];
       const uint64_t o = (uint64_t)(offk + (int64_t)(vex - vfirst));
       const uint32_t p = p0k + (rec[0] >> 8);
-      const uint32_t y = V.divX.div(p);
+      const VolDesc& Vk = vol_at<BATCH>(P, (int)vik);
+      const uint32_t y = Vk.divX.div(p);
       uint32_t* mt = s_meta + 4 * lane;
       mt[0] = (uint32_t)o;
       mt[1] = meta_hi(o, zk);
-      mt[2] = p - y * (uint32_t)V.X;
+      mt[2] = p - y * (uint32_t)Vk.X;
       mt[3] = y;
+      if constexpr (BATCH) s_vi[lane] = vik;
     }
-    emit_block<ELEM>(P, V, s_slot, s_nv, s_w, R, node_off);
+    emit_block<ELEM, BATCH>(P, V, s_slot, s_nv, s_w, R, node_off, s_vi);
   }
 }
 
-// Wave-level emission: wave w of the grid takes list entries w, w + nwaves,
-// ... (no atomics: one counter shared by every wave serialises).  A tile's
-// vertices are [voff[t], voff[t] + T), in cube order, so the output does not
-// depend on which wave takes which tile.  The metadata of up to 64 of the
-// wave's tiles is loaded at once (one lane per tile) and broadcast; each
-// tile's first 64 records are fetched (coalesced, into registers) while the
-// previous tile is emitted.  A tile whose records did not fit the scratch is
-// re-read from the volume, 64 cubes at a time, into the same record form.
-template <int ELEM, bool BATCH>
-__global__ __launch_bounds__(kEmitThreads) void emit_kernel(ExtractParams P) {
-  __shared__ uint64_t s_case[256];
-  __shared__ uint8_t s_nv[256];
-  __shared__ uint32_t s_wave[kEmitThreads / 64][kWaveWords];
-  s_case[threadIdx.x] = dCase[threadIdx.x];
-  s_nv[threadIdx.x] = dNumVerts[threadIdx.x];
-  __syncthreads();  // the only workgroup barrier: waves run independently below
-  const int lane = threadIdx.x & 63;
-  const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
-  uint32_t* const s_w = s_wave[wv];
-  const uint32_t nz = P.hdr->nz_total;
-  const uint32_t nwaves = gridDim.x * (kEmitThreads / 64);
-  const uint32_t wave0 = blockIdx.x * (kEmitThreads / 64) + (uint32_t)wv;
-  // this wave's items: wave0 + i * nwaves, in batches of 64
-  for (uint32_t ib = 0; wave0 + ib * nwaves < nz; ib += 64) {
-    const uint32_t my_item = wave0 + (ib + (uint32_t)lane) * nwaves;
-    const bool have = my_item < nz;
-    const uint32_t nb = (uint32_t)__popcll(__ballot(have));
-    int64_t t_l = 0, off_l = 0, rs_l = 0;
-    int nrec_l = 0;
-    if (have) {
-      t_l = P.nz_list[my_item];
-      nrec_l = P.tile_nrec[t_l];
-      off_l = P.voff[t_l];
-      rs_l = P.tile_rec[t_l];
-    }
-    // first record block of tile j, 9 words per lane (coalesced)
-    auto fetch = [&](uint32_t j, uint32_t* pf) {
-      const int nr = __shfl(nrec_l, (int)j);
-      const int64_t rs = __shfl(rs_l, (int)j);
-      const int nw = min(nr, 64) * kRecWords;
-      const uint32_t* src = P.rec + rs * kRecWords;
-#pragma unroll
-      for (int q = 0; q < kRecWords; ++q) {
-        const int w = lane + 64 * q;
-        pf[q] = w < nw ? src[w] : 0u;
-      }
-    };
-    uint32_t pf[kRecWords];
-    fetch(0, pf);
-    for (uint32_t j = 0; j < nb; ++j) {
-      const int64_t t = __shfl(t_l, (int)j);
-      const int nrec = __shfl(nrec_l, (int)j);
-      int64_t out = __shfl(off_l, (int)j);
-      const int64_t rs = __shfl(rs_l, (int)j);
-      uint32_t cur[kRecWords];
-#pragma unroll
-      for (int q = 0; q < kRecWords; ++q) cur[q] = pf[q];
-      if (j + 1 < nb) fetch(j + 1, pf);  // in flight while tile j is emitted
-      const int vi = BATCH ? volume_of_tile(P, t) : 0;
-      const VolDesc& V = vol_at<BATCH>(P, vi);
-      const int64_t local = t - V.tile_begin;
-      const int z = (int)V.divR.div((uint32_t)local);
-      const int64_t p0 = (local - (int64_t)z * V.R) * kTile;
-      if (nrec >= 0) {
-        for (int rb = 0; rb < nrec; rb += 64) {
-          const int n = min(64, nrec - rb);
-          if (rb == 0) {
-#pragma unroll
-            for (int q = 0; q < kRecWords; ++q) s_w[lane + 64 * q] = cur[q];
-          } else {
-            const uint32_t* src = P.rec + (rs + rb) * kRecWords;
-            for (int w = lane; w < n * kRecWords; w += 64) s_w[w] = src[w];
-          }
-          wave_sync();
-          out += emit_records<ELEM>(P, V, s_case, s_nv, s_w, n, z, p0, out);
-        }
-      } else {
-        const int X = V.X;
-        const int64_t XY = (int64_t)X * V.Y;
-        const uint32_t* vol = V.vol + (int64_t)z * XY;
-        for (int cb = 0; cb < kTile; cb += 64) {
-          const int o = cb + lane;
-          const int64_t p = p0 + o;
-          uint32_t cw[8] = {}, ci = 0;
-          if (p < XY) {
-            const uint32_t y = V.divX.div((uint32_t)p);
-            const uint32_t x = (uint32_t)p - y * (uint32_t)X;
-            if (x + 1 < (uint32_t)X && y + 1 < (uint32_t)V.Y) {
-              const uint32_t* a = vol + p;
-              const uint32_t* b = a + XY;
-              cw[0] = a[0]; cw[1] = a[1]; cw[2] = a[X + 1]; cw[3] = a[X];
-              cw[4] = b[0]; cw[5] = b[1]; cw[6] = b[X + 1]; cw[7] = b[X];
-              ci = cube_index<ELEM>(cw);
-            }
-          }
-          // compact the cubes with vertices into the record area, in cube order
-          const uint64_t act = __ballot(s_nv[ci] != 0);
-          const int n = __popcll(act);
-          if ((act >> lane) & 1ull) {
-            uint32_t* rp = s_w + __popcll(act & ((1ull << lane) - 1ull)) * kRecWords;
-            rp[0] = ((uint32_t)o << 8) | ci;
-#pragma unroll
-            for (int q = 0; q < 8; ++q) rp[1 + q] = cw[q];
-          }
-          wave_sync();
-          if (n) out += emit_records<ELEM>(P, V, s_case, s_nv, s_w, n, z, p0, out);
-        }
-      }
-    }
-  }
-}
 
 // ============================================================ backward
 // Differentiable form (no counterpart in the reference).  For a vertex on
@@ -1938,7 +1709,7 @@ int64_t cubes_of(int X, int Y, int Z) {
 
 size_t fixed_ws_bytes(int64_t n_tiles) {
   const int64_t nscan = n_tiles > 0 ? (n_tiles + kScanItems - 1) / kScanItems : 1;
-  return kWsHeader + al256(24 * (size_t)nscan) + 5 * al256(4 * (size_t)n_tiles) +
+  return kWsHeader + al256(24 * (size_t)nscan) + 7 * al256(4 * (size_t)n_tiles) +
          3 * al256(8 * (size_t)n_tiles) + al256(8 * (size_t)kCPW * (size_t)n_tiles) +
          al256(8 * (size_t)(n_tiles + 1));
 }
@@ -1978,6 +1749,10 @@ int layout_ws(void* d_ws, size_t ws_bytes, int64_t n_tiles, ExtractParams& P) {
   o += al256(4 * (size_t)n_tiles);
   P.clist = reinterpret_cast<uint32_t*>(b + o);
   o += al256(4 * (size_t)n_tiles);
+  P.nz_vol = reinterpret_cast<uint32_t*>(b + o);
+  o += al256(4 * (size_t)n_tiles);
+  P.clist_vol = reinterpret_cast<uint32_t*>(b + o);
+  o += al256(4 * (size_t)n_tiles);
   P.lpre = reinterpret_cast<int64_t*>(b + o);
   o += al256(8 * (size_t)(n_tiles + 1));
   P.lrs = reinterpret_cast<int64_t*>(b + o);
@@ -2013,7 +1788,7 @@ int run_extract(ExtractParams& P, int elem, bool batch, int maxX, hipStream_t s,
   // record scratch: per-column quotas; the wide count kernel keeps a quarter
   // of the scratch as a shared overflow pool
   const bool wide = P.wide && elem == MCX_ELEM_SHORT2 && !P.write_masks;
-  P.list_scan = !batch && emit;
+  P.list_scan = emit;
   const int64_t pool = wide ? P.rec_cap / 4 : 0;
   P.quota = P.ncols > 0 ? (P.rec_cap - pool) / P.ncols : 0;
   if (P.quota > (int64_t)kTile * P.slab_max) P.quota = (int64_t)kTile * P.slab_max;
@@ -2037,8 +1812,7 @@ int run_extract(ExtractParams& P, int elem, bool batch, int maxX, hipStream_t s,
       MCX_HIP(hipGetLastError());                                                                \
     }                                                                                            \
     if (!emit) break;                                                                            \
-    if (B) hipLaunchKernelGGL((emit_kernel<E, B>), dim3(k3_blocks), dim3(kEmitThreads), 0, s, P); \
-    else hipLaunchKernelGGL(emit_units_kernel<E>, dim3(k3_blocks), dim3(kEmitThreads), 0, s, P); \
+    hipLaunchKernelGGL((emit_units_kernel<E, B>), dim3(k3_blocks), dim3(kEmitThreads), 0, s, P); \
     MCX_HIP(hipGetLastError());                                                                  \
   } while (0)
   if (elem == MCX_ELEM_SHORT2) {
