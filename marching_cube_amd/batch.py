@@ -35,7 +35,7 @@ class BatchPlan:
 
     def __del__(self):
         h = getattr(self, "handle", None)
-        if h is not None and h.value:
+        if h is not None and h.value and _capi is not None:  # modules may be gone at exit
             _capi.load().mcx_batch_destroy(h)
             self.handle = None
 

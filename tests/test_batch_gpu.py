@@ -57,3 +57,28 @@ def test_batch_repeat_and_f32(dev):
     for b, (f, h) in enumerate(zip(fs, hs)):
         ref = O.extract(f, h, want=("pos",))
         assert pos[vb[b]:vb[b + 1]].tobytes() == ref["pos"].tobytes()
+
+
+@pytest.mark.parametrize("groups", [1, 8, 32])
+def test_batch_and_single_launch_shapes(dev, groups):
+    # waves taking many 64-record units (few emit workgroups) and few: the
+    # batched segments and one-call-per-volume outputs equal the oracle
+    from marching_cube_amd import autotune, mesh
+    print(f"[case] batch of 150 shape volumes, {groups} emit groups per CU", flush=True)
+    vols, hs = synth.batch_shapes(150, lo=12, hi=80, seed=23)
+    tens = [torch.from_numpy(v.view(np.int32)).to(dev) for v in vols]
+    saved = autotune.get()
+    try:
+        autotune.set(emit_groups_per_cu=groups)
+        plan = batch.BatchPlan(tens, hs)
+        n, vbase, out = plan.extract(want=("vv",))
+        vbase = vbase.cpu().numpy()
+        vv = out["vv"].cpu().numpy()
+        ws = mesh.Workspace(dev)
+        for b, (v, h) in enumerate(zip(vols, hs)):
+            ref = O.extract(v, float(h), want=("vv",))["vv"].view(np.int32).reshape(-1, 7)
+            assert vv[vbase[b]:vbase[b + 1]].tobytes() == ref.tobytes(), ("batched", b)
+            r = mesh.extract(tens[b], float(h), want=("vv",), workspace=ws)
+            assert r.vv.cpu().numpy().tobytes() == ref.tobytes(), ("single", b)
+    finally:
+        autotune.set(saved["columns_per_cu"], saved["emit_groups_per_cu"], saved["batch_slab"])
