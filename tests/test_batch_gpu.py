@@ -82,3 +82,34 @@ def test_batch_and_single_launch_shapes(dev, groups):
             assert r.vv.cpu().numpy().tobytes() == ref.tobytes(), ("single", b)
     finally:
         autotune.set(saved["columns_per_cu"], saved["emit_groups_per_cu"], saved["batch_slab"])
+
+
+@pytest.mark.parametrize("records", [0, 2000])
+def test_batch_staged_path(dev, records):
+    # a record scratch too small for the batch: tiles take the staged path
+    # (re-read from their volume), all or some of them
+    import ctypes
+    from marching_cube_amd import _capi
+    from marching_cube_amd.mesh import VV_WORDS
+    print(f"[case] batch staged path, records={records}", flush=True)
+    vols, hs = synth.batch_shapes(40, lo=10, hi=60, seed=31)
+    tens = [torch.from_numpy(v.view(np.int32)).to(dev) for v in vols]
+    plan = batch.BatchPlan(tens, hs)
+    lib = _capi.load()
+    ws = torch.empty(max(4096, lib.mcx_batch_workspace_size(plan.handle, records)),
+                     dtype=torch.uint8, device=dev)
+    stream = torch.cuda.current_stream(dev).cuda_stream
+    assert lib.mcx_workspace_init(ws.data_ptr(), ws.numel(), stream) == _capi.OK
+    refs = [O.extract(v, float(h), want=("vv",))["vv"].view(np.int32).reshape(-1, 7)
+            for v, h in zip(vols, hs)]
+    total = sum(len(r) for r in refs)
+    vv = torch.empty((total, VV_WORDS), dtype=torch.int32, device=dev)
+    vbase = torch.empty(len(vols) + 1, dtype=torch.int64, device=dev)
+    hc = ctypes.c_int64(0)
+    st = lib.mcx_extract_batched(plan.handle, vv.data_ptr(), None, None, total, vbase.data_ptr(),
+                                 None, ctypes.byref(hc), ws.data_ptr(), ws.numel(), stream)
+    assert st == _capi.OK and hc.value == total
+    vb = vbase.cpu().numpy()
+    out = vv.cpu().numpy()
+    for b, ref in enumerate(refs):
+        assert out[vb[b]:vb[b + 1]].tobytes() == ref.tobytes(), b
