@@ -79,7 +79,9 @@ int mcx_workspace_init(void* d_ws, size_t ws_bytes, void* stream);
  * reference's reversed-winding slot order); entries with index >= cap are
  * not written.  d_count (device int64, optional) receives the total.
  * h_count (host, optional): the call synchronises the stream, stores the
- * total and returns MCX_ERR_CAPACITY if it exceeds cap. */
+ * total and returns MCX_ERR_CAPACITY if it exceeds cap.  Sizes: X*Y + X +
+ * 1024 < 2^30 and Z <= 2^24, else MCX_ERR_UNSUPPORTED (also for batched
+ * plans, per volume). */
 int mcx_extract(const void* d_vol, int elem, int X, int Y, int Z, float h,
                 mcx_vertex_voxel* d_vv, float* d_pos, int32_t* d_v2vox,
                 int16_t* d_ver_voxel, int64_t cap, int64_t* d_count,

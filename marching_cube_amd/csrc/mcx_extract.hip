@@ -1691,6 +1691,8 @@ size_t al256(size_t b) { return (b + 255) & ~(size_t)255; }
 // Byte offsets of the count kernel's plane loads (in-plane word + X + 1024)
 // stay below 2^32.
 bool plane_ok(int X, int Y) { return (int64_t)X * Y + X + 1024 < ((int64_t)1 << 30); }
+// ... and the emit kernels pack a cube's z in 24 bits.
+bool size_ok(int X, int Y, int Z) { return plane_ok(X, Y) && Z <= (1 << 24); }
 
 // The 128-bit count kernel addresses a column's planes (slab + 1, plus the
 // prefetched ones past it) through one buffer resource with 32-bit offsets.
@@ -1850,7 +1852,7 @@ VolDesc make_vol(const void* d_vol, int X, int Y, int Z, float h, int16_t* ver_v
 // record slots.
 int prepare_single(ExtractParams& P, const void* d_vol, int X, int Y, int Z, float h,
                    int16_t* ver_voxel, void* d_ws, size_t ws_bytes) {
-  if (Z > (1 << 24)) return MCX_ERR_UNSUPPORTED;  // z is packed in 24 bits by the emit kernel
+  if (!size_ok(X, Y, Z)) return MCX_ERR_UNSUPPORTED;
   const int64_t n_tiles = tiles_of(X, Y, Z);
   const int rc = layout_ws(d_ws, ws_bytes, n_tiles, P);
   if (rc) return rc;
@@ -1912,7 +1914,7 @@ extern "C" int mcx_extract_ex(const void* d_vol, int elem, int X, int Y, int Z, 
   if (!d_vol || X < 1 || Y < 1 || Z < 1 || cap < 0) return MCX_ERR_ARG;
   if (elem != MCX_ELEM_SHORT2 && elem != MCX_ELEM_F32) return MCX_ERR_ARG;
   if (elem == MCX_ELEM_F32 && (d_vv || d_ver_voxel)) return MCX_ERR_ARG;
-  if (!plane_ok(X, Y)) return MCX_ERR_UNSUPPORTED;
+  if (!size_ok(X, Y, Z)) return MCX_ERR_UNSUPPORTED;
   hipStream_t s = as_stream(stream);
   int rc = MCX_OK;
   const int64_t N = (int64_t)X * Y * Z;
@@ -1987,7 +1989,7 @@ extern "C" int mcx_extract_diff(const float* d_vol, int X, int Y, int Z, float h
                                 int64_t cap, int64_t* d_count, int64_t* h_count, void* d_ws,
                                 size_t ws_bytes, void* stream) {
   if (!d_vol || X < 1 || Y < 1 || Z < 1 || cap < 0) return MCX_ERR_ARG;
-  if (!plane_ok(X, Y)) return MCX_ERR_UNSUPPORTED;
+  if (!size_ok(X, Y, Z)) return MCX_ERR_UNSUPPORTED;
   hipStream_t s = as_stream(stream);
   const int64_t n_tiles = tiles_of(X, Y, Z);
   if (n_tiles == 0) {
@@ -2024,7 +2026,7 @@ int backward_impl(const float* d_vol, int X, int Y, int Z, float h, const float*
                   float* d_grad_vol, void* d_ws, size_t ws_bytes, void* d_scratch,
                   size_t scratch_bytes, void* stream, bool recompute) {
   if (!d_vol || !d_grad_vol || X < 1 || Y < 1 || Z < 1) return MCX_ERR_ARG;
-  if (!plane_ok(X, Y)) return MCX_ERR_UNSUPPORTED;
+  if (!size_ok(X, Y, Z)) return MCX_ERR_UNSUPPORTED;
   hipStream_t s = as_stream(stream);
   const int64_t N = (int64_t)X * Y * Z;
   const int64_t n_tiles = tiles_of(X, Y, Z);
@@ -2101,7 +2103,7 @@ extern "C" int mcx_batch_create(mcx_batch** out, int nvol, const int* dims_xyz, 
   for (int b = 0; b < nvol; ++b) {
     const int X = dims_xyz[3 * b], Y = dims_xyz[3 * b + 1], Z = dims_xyz[3 * b + 2];
     if (X < 1 || Y < 1 || Z < 1 || !d_vols[b]) return MCX_ERR_ARG;
-    if (!plane_ok(X, Y)) return MCX_ERR_UNSUPPORTED;
+    if (!size_ok(X, Y, Z)) return MCX_ERR_UNSUPPORTED;
     vols[b] = make_vol(d_vols[b], X, Y, Z, h[b], nullptr, tb);
     const int64_t nt = tiles_of(X, Y, Z);
     if (nt > 0) {

@@ -48,6 +48,11 @@ def test_argument_errors_without_device():
                            None, None, 0, None) == _capi.ERR_ARG
     assert lib.mcx_extract(fake, 1, 4, 4, 4, 1.0, fake, None, None, None, 10, None,
                            None, None, 0, None) == _capi.ERR_ARG
+    # size limits (checked before any device work): X*Y + X + 1024 < 2^30, Z <= 2^24
+    assert lib.mcx_extract(fake, 0, 4, 4, (1 << 24) + 1, 1.0, None, None, None, None, 0, None,
+                           None, None, 0, None) == _capi.ERR_UNSUPPORTED
+    assert lib.mcx_extract(fake, 0, 1 << 15, 1 << 15, 4, 1.0, None, None, None, None, 0, None,
+                           None, None, 0, None) == _capi.ERR_UNSUPPORTED
     assert lib.mcx_classify(None, 0, 4, 4, 4, None, None, None) == _capi.ERR_ARG
     # differentiable integration: the backward handles one pass of <= 512 frames;
     # the checks return before any device work
