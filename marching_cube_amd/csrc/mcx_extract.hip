@@ -532,7 +532,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4)))
 void count4_kernel(ExtractParams P) {
   __shared__ uint8_t s_nv[256];
   __shared__ __attribute__((aligned(16))) uint32_t s_rec[4][256 * kRecWords];  // per wave: one chunk's records
+  __shared__ uint32_t s_wg_used;  // records taken from the workgroup's region
   s_nv[threadIdx.x] = dNumVerts[threadIdx.x];
+  if (threadIdx.x == 0) s_wg_used = 0;
   clear_scan_descriptors(P);
   __syncthreads();
   const int lane = threadIdx.x & 63;
@@ -559,7 +561,13 @@ void count4_kernel(ExtractParams P) {
   const int X = V.X;
   const int64_t XY = (int64_t)X * V.Y;
   const int64_t p0 = (int64_t)r * kTile;
-  const int64_t rec_base = col * P.quota;
+  // The quotas of the workgroup's (up to) four consecutive columns form one
+  // region shared by its waves: a tile takes its records with an LDS atomic
+  // (no wait on the vector memory counter, so the prefetched planes stay in
+  // flight), and only when the region is full from the shared overflow pool.
+  const int64_t col0 = col - wave;
+  const int64_t wg_base = col0 * P.quota;
+  const uint32_t wg_size = (uint32_t)(min((int64_t)4, P.ncols - col0) * P.quota);
 
   // byte k of cube_mask[c] = 0xff when cube k of the lane exists: in-plane
   // index p + k inside the plane, not in the last row and not the last of
@@ -638,10 +646,8 @@ void count4_kernel(ExtractParams P) {
   asm volatile("s_waitcnt vmcnt(12)" : "+v"(A.u[0]), "+v"(A.u[1]), "+v"(A.w[0]), "+v"(A.w[1]), "+v"(A.xu), "+v"(A.xw) :: "memory");
   plane_signs(A);
 
-  int32_t used = 0;  // records of the column's quota taken
   uint64_t nzmask = 0;  // planes zb + i of this column whose tile has vertices
   int zb = z0;
-  const uint32_t quota = (uint32_t)P.quota;
   uint32_t t = (uint32_t)(V.tile_begin + (int64_t)z0 * V.R + r);  // this step's tile
   const uint32_t tR = (uint32_t)V.R;
   // The plane two steps ahead is loaded at the end of a step, after the
@@ -736,11 +742,14 @@ void count4_kernel(ExtractParams P) {
       for (int k = 0; k < 4; ++k) nrec32 += __popcll(m[c][k]);
     const int64_t nrec = nrec32;
 
-    // ---- destination of the tile's records
-    int64_t rbase = rec_base + used;
-    bool dense = false, in_quota = true;
-    if ((uint32_t)nrec32 > quota - (uint32_t)used) {
-      in_quota = false;
+    // ---- destination of the tile's records (a failed take leaves the
+    // counter past the region: later tiles of the workgroup use the pool)
+    uint32_t got = 0;
+    if (lane == 0) got = atomicAdd(&s_wg_used, (uint32_t)nrec32);
+    got = __builtin_amdgcn_readfirstlane(got);
+    int64_t rbase = wg_base + got;
+    bool dense = false;
+    if (got + (uint32_t)nrec32 > wg_size) {
       unsigned long long ob = 0;
       if (lane == 0) ob = atomicAdd(reinterpret_cast<unsigned long long*>(&P.hdr->ovf_next),
                                     (unsigned long long)nrec);
@@ -851,7 +860,6 @@ void count4_kernel(ExtractParams P) {
       P.tile_rec[t] = rbase;
       if (dense) atomicAdd(&P.hdr->dense_acc, 1u);
     }
-    if (in_quota) used += nrec32;
     load_plane(z + 3, L);
     t += tR;
     // append the column's non-empty tiles to the list, 64 planes at a time
