@@ -21,7 +21,7 @@ import torch
 
 from . import _capi, mesh
 
-COLUMNS = (16, 32, 64, 96, 128)
+COLUMNS = (16, 32, 48, 56, 64, 96, 128)
 EMIT_GROUPS = (4, 6, 8, 16, 32)
 BATCH_SLABS = (16, 32, 64)
 

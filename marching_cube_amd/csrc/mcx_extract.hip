@@ -62,7 +62,7 @@ constexpr int kEmitThreads = 256;
 // single volumes, K3 persistent workgroups per CU, planes per column of
 // batched plans (taken when a plan is created).
 struct Tuning {
-  int columns_per_cu = 64;
+  int columns_per_cu = 56;
   int emit_groups_per_cu = 16;
   int batch_slab = 64;
 };

@@ -101,7 +101,7 @@ def test_tuning_set_get_and_bounds():
     lib = _capi.load()
     keys = (_capi.TUNE_COLUMNS_PER_CU, _capi.TUNE_EMIT_GROUPS_PER_CU, _capi.TUNE_BATCH_SLAB)
     saved = [lib.mcx_get_tuning(k) for k in keys]
-    assert saved == [64, 16, 64]  # defaults
+    assert saved == [56, 16, 64]  # defaults
     try:
         assert lib.mcx_set_tuning(_capi.TUNE_COLUMNS_PER_CU, 32) == _capi.OK
         assert lib.mcx_get_tuning(_capi.TUNE_COLUMNS_PER_CU) == 32
