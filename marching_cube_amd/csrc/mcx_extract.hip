@@ -479,9 +479,10 @@ __device__ __forceinline__ uint32_t min_u16x2(uint32_t a, uint32_t b) {
 // 8 XCDs: ids b, b + 8, ... run on XCD b % 8): neighbouring columns share a
 // row through that XCD's L2, and the runs rotate over the XCDs so that every
 // XCD gets work from every slab.  Ids of an incomplete last round map to
-// themselves.
+// themselves.  Runs of 4 (16 columns) measured fastest at 512^3: 141 us for
+// K1 against 142 (1, no remap), 141-142 (2), 144-146 (8) and 150 (16, 32).
 #ifndef MCX_XCD_GROUP
-#define MCX_XCD_GROUP 8
+#define MCX_XCD_GROUP 4
 #endif
 constexpr uint32_t kXcdGroup = MCX_XCD_GROUP;
 __device__ __forceinline__ uint32_t xcd_block() {
