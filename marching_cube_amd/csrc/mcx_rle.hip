@@ -121,10 +121,27 @@ int scan_excl(int64_t* a, int64_t n, int64_t* tmp, hipStream_t s) {
 }
 
 // ---------------------------------------------------------------- encode
+typedef uint32_t u32x4a4 __attribute__((ext_vector_type(4), aligned(4)));  // 4-byte aligned
+
 // grid (blocks per slice, nz): change points among 1024 voxels of slice z.
+// Bit k: voxel i0 + k starts a run (the first of the slice, or a tsdf that
+// differs from the voxel before it).
 __device__ __forceinline__ uint32_t change_flags(const uint32_t* __restrict__ sl, int64_t XY,
                                                  int64_t i0) {
   uint32_t f = 0;
+  if (i0 + 4 <= XY) {
+    // the four voxels in one 128-bit load (the slice is 4-byte aligned,
+    // which the vector memory path accepts), the one before in another
+    const u32x4a4 v = *reinterpret_cast<const u32x4a4*>(sl + i0);
+    int prev = i0 > 0 ? tsdf_of(sl[i0 - 1]) : 0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const int t = tsdf_of(v[k]);
+      if (i0 + k == 0 || t != prev) f |= 1u << k;
+      prev = t;
+    }
+    return f;
+  }
   int prev = i0 > 0 && i0 - 1 < XY ? tsdf_of(sl[i0 - 1]) : 0;
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
