@@ -430,8 +430,12 @@ __global__ __launch_bounds__(256) void dec_fill_kernel(const SliceInfo* info, in
                                                        const uint32_t* tile_first,
                                                        const uint32_t* err, uint32_t* out,
                                                        int vec) {
-  __shared__ uint32_t s_start[kFillStage];
+  // staged runs: start - vb clamped to [0, kFillTile] (16 bits, so six
+  // workgroups fit a CU's LDS; for every voxel q >= vb of the tile,
+  // start <= q exactly when the clamped value <= q - vb), and voxel value
+  __shared__ uint16_t s_start[kFillStage];
   __shared__ uint32_t s_val[kFillStage];
+  static_assert(kFillTile <= 0xffff, "tile offsets in 16 bits");
   if (*err) return;  // malformed input: the call reports it, outputs are undefined
   const int tid = threadIdx.x;
   const int j = blockIdx.y, t = blockIdx.x;
@@ -450,7 +454,7 @@ __global__ __launch_bounds__(256) void dec_fill_kernel(const SliceInfo* info, in
       const uint32_t s = rs[ri];
       in = ri == r0 || (int64_t)s < ve;
       if (in) {
-        s_start[round * 256 + tid] = s;
+        s_start[round * 256 + tid] = (uint16_t)min(max((int64_t)s - vb, (int64_t)0), (int64_t)kFillTile);
         s_val[round * 256 + tid] = rv[ri];
       }
     }
@@ -461,10 +465,11 @@ __global__ __launch_bounds__(256) void dec_fill_kernel(const SliceInfo* info, in
   __syncthreads();
   uint32_t* o = out + (int64_t)j * XY;
   for (int64_t q = vb + 4 * tid; q < ve; q += 4 * 256) {
+    const int qr = (int)(q - vb);
     int lo = 0, hi = m - 1;  // last staged record with start <= q
     while (lo < hi) {
       const int mid = (lo + hi + 1) >> 1;
-      if ((int64_t)s_start[mid] <= q)
+      if ((int)s_start[mid] <= qr)
         lo = mid;
       else
         hi = mid - 1;
@@ -473,7 +478,7 @@ __global__ __launch_bounds__(256) void dec_fill_kernel(const SliceInfo* info, in
     int idx = lo;
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
-      while (idx + 1 < m && (int64_t)s_start[idx + 1] <= q + e) ++idx;
+      while (idx + 1 < m && (int)s_start[idx + 1] <= qr + e) ++idx;
       v4[e] = s_val[idx];
     }
     if (vec && q + 3 < ve) {
