@@ -9,10 +9,12 @@
 // canonical order (the cubes interpolate the edge in different directions, so
 // their fp32 positions can differ in the last bit).
 //
-// W1 insert  hash table (open addressing, power of two >= 2V slots): each
-//            vertex claims its key's slot (atomicCAS) and records its index
-//            with atomicMin, so the slot ends with the first occurrence
-//            whatever the order of the atomics.
+// W1 insert  hash table (open addressing, power of two >= 2V slots): the
+//            first lane of a wave holding a key (the smallest index of the
+//            key in the wave; vertices come in cube order, so a key often
+//            repeats within 64 vertices) claims the key's slot (atomicCAS)
+//            and records its index with atomicMin, so the slot ends with the
+//            first occurrence whatever the order of the atomics.
 // W2 flag    each vertex looks its key up: rep = first occurrence; unique
 //            vertices (rep == v) get flag 1; per-block flag sums.
 // W3 scan    exclusive scan of the block sums (one workgroup).
@@ -49,8 +51,19 @@ __global__ __launch_bounds__(256) void weld_insert_kernel(const int64_t* __restr
                                                           int64_t V, uint64_t* keys,
                                                           uint32_t* vals, uint64_t mask) {
   const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (v >= V) return;
-  const uint64_t key = (uint64_t)ekey[v];
+  const uint64_t key = v < V ? (uint64_t)ekey[v] : kEmptyKey;
+  // a lower lane of the wave with the same key inserts for this one (lanes
+  // past V are the highest of the last wave, so no live lane compares with
+  // them)
+  const int lane = (int)(threadIdx.x & 63);
+  const uint32_t klo = (uint32_t)key, khi = (uint32_t)(key >> 32);
+  bool dup = false;
+  for (int i = 0; i < 63; ++i) {
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)klo, i);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)khi, i);
+    dup |= i < lane && lo == klo && hi == khi;
+  }
+  if (v >= V || dup) return;
   uint64_t h = mix64(key) & mask;
   // at most V distinct keys in >= 2V slots: the probe ends
   for (;;) {
