@@ -98,6 +98,50 @@ def test_welded_sphere_is_closed_manifold(dev):
     assert (faces[:, 0] != faces[:, 1]).sum() > 0.99 * F  # few degenerate triangles
 
 
+@pytest.mark.parametrize("pattern,V", [("one_key", 1000), ("few", 1), ("few", 63), ("few", 65),
+                                       ("few", 4097), ("pairs", 640), ("runs", 100003),
+                                       ("mesh_like", 300001)])
+def test_weld_synthetic_keys(dev, pattern, V):
+    """mcx_weld on its own (C ABI) with keys that repeat inside a wave, across
+    waves and at wave boundaries, and sizes that are not multiples of 64:
+    unique vertices in first-occurrence order and every index as the numpy
+    restatement (np.unique first occurrences)."""
+    print(f"[case] weld keys {pattern} V={V}", flush=True)
+    from marching_cube_amd import _capi
+    import ctypes
+    rng = np.random.default_rng(V)
+    if pattern == "one_key":
+        keys = np.full(V, 123456789, np.int64)
+    elif pattern == "few":
+        keys = rng.integers(0, 50, V).astype(np.int64)
+    elif pattern == "pairs":
+        # lane 63 of a wave and lane 0 of the next share a key; so do lanes
+        # 62 and 1 of one wave (the later lane is the repeat)
+        keys = np.arange(V, dtype=np.int64) * 7
+        for w in range(V // 64 - 1):
+            keys[64 * w + 64] = keys[64 * w + 63]
+            keys[64 * w + 62] = keys[64 * w + 1]
+    elif pattern == "runs":
+        keys = np.repeat(rng.integers(0, 1 << 40, V // 5 + 1), 5)[:V].astype(np.int64)
+    else:  # about six occurrences per key, spread over the array
+        keys = (rng.integers(0, V // 6, V) * 3 + 1).astype(np.int64)
+    pos = rng.standard_normal((V, 3)).astype(np.float32)
+    lib = _capi.load()
+    k = torch.from_numpy(keys).to(dev)
+    p = torch.from_numpy(pos).to(dev)
+    upos = torch.empty((V, 3), dtype=torch.float32, device=dev)
+    idx = torch.empty(V, dtype=torch.int32, device=dev)
+    wws = torch.empty(max(4096, lib.mcx_weld_workspace_size(V)), dtype=torch.uint8, device=dev)
+    nu = ctypes.c_int64(0)
+    _capi.check(lib.mcx_weld(mesh._ptr(k), mesh._ptr(p), 3, V, mesh._ptr(upos), mesh._ptr(idx),
+                             None, ctypes.byref(nu), mesh._ptr(wws), wws.numel(),
+                             mesh._stream_handle(dev)), "mcx_weld")
+    rupos, ridx = ref_weld(keys, pos)
+    assert nu.value == len(rupos)
+    assert upos[: nu.value].cpu().numpy().tobytes() == rupos.tobytes()
+    assert idx.cpu().numpy().tobytes() == ridx.tobytes()
+
+
 def test_weld_empty(dev):
     f = synth.pack_short2(np.zeros((6, 7, 8)), np.zeros((6, 7, 8)))
     upos, faces = mesh.indexed_mesh(torch.from_numpy(f.view(np.int32)).to(dev), 0.1)
