@@ -21,8 +21,8 @@
 //
 // Layout: one thread per pixel; each wave covers an 8 x 8 pixel tile so the
 // 64 rays of a wave march through neighbouring voxels (shared cache lines);
-// workgroups of 16 x 16 pixels.  Each sample reads eight 16-bit tsdf halves
-// (global_load_sshort).  Bit-exact with the CPU oracle: every fused
+// workgroups of 16 x 16 pixels.  Each sample reads its eight corner words as
+// four x-neighbour pairs (one 64-bit load each).  Bit-exact with the CPU oracle: every fused
 // multiply-add is explicit and the build disables contraction.
 #include <hip/hip_runtime.h>
 
@@ -62,19 +62,29 @@ __device__ __forceinline__ float unpack_tsdf(int s) {
   return fmaf(e, r, q);
 }
 
+// two consecutive voxel words (4-byte aligned, which the vector memory path
+// accepts for a 64-bit load)
+typedef uint32_t u32x2a4 __attribute__((ext_vector_type(2), aligned(4), may_alias));
+
 // Corner tsdf values in the reference's order V000 V100 V010 V001 V101 V011
-// V110 V111 for the cell whose lower corner is word `base`.
+// V110 V111 for the cell whose lower corner is word `base`: the words of
+// (x, x + 1) in the rows (y, z), (y + 1, z), (y, z + 1), (y + 1, z + 1).
 __device__ __forceinline__ void load_corners(const int16_t* __restrict__ v16, int64_t base,
                                              int64_t X, int64_t XY, int* s) {
-  const int16_t* p = v16 + 2 * base;
-  s[0] = p[0];
-  s[1] = p[2];
-  s[2] = p[2 * X];
-  s[3] = p[2 * XY];
-  s[4] = p[2 * XY + 2];
-  s[5] = p[2 * (XY + X)];
-  s[6] = p[2 * X + 2];
-  s[7] = p[2 * (XY + X) + 2];
+  const uint32_t* w = reinterpret_cast<const uint32_t*>(v16) + base;
+  auto pair = [](const uint32_t* q) { return *reinterpret_cast<const u32x2a4*>(q); };
+  const u32x2a4 a = pair(w);           // V000 V100
+  const u32x2a4 b = pair(w + X);       // V010 V110
+  const u32x2a4 c = pair(w + XY);      // V001 V101
+  const u32x2a4 d = pair(w + XY + X);  // V011 V111
+  s[0] = (int16_t)a[0];
+  s[1] = (int16_t)a[1];
+  s[2] = (int16_t)b[0];
+  s[3] = (int16_t)c[0];
+  s[4] = (int16_t)c[1];
+  s[5] = (int16_t)d[0];
+  s[6] = (int16_t)b[1];
+  s[7] = (int16_t)d[1];
 }
 
 __device__ __forceinline__ float sample_sdf(const RaycastArgs& A, int64_t XY, float px, float py,
