@@ -126,6 +126,8 @@ def bench_diff(torch, dev, dim, steps, warmup):
     fdiff_ms, _ = _event_ms(torch, fwd_diff, steps, warmup)
     fb_ms, _ = _event_ms(torch, fwd_bwd, steps, warmup)
     bwd_ms, _ = _event_ms(torch, bwd, steps, warmup)
+    ws.check()
+    _capi.check(lib.mcx_workspace_status(_ptr(ex.ws), ex.ws.numel(), stream), "diff workspace")
     ref = ex.backward(vol, grad_pos)
     assert gvol.cpu().numpy().tobytes() == ref.cpu().numpy().tobytes()  # deterministic
     # end-to-end autograd step (forward, loss, backward) with a reused extractor
@@ -178,6 +180,8 @@ def bench_batch(torch, dev, count, steps, warmup):
 
     b_ms, _ = _event_ms(torch, batched, steps, warmup)
     p_ms, _ = _event_ms(torch, per_volume, max(2, steps // 4), 1)
+    plan.check()  # device-side status of every asynchronous call above
+    ws.check()
     assert vv.cpu().numpy().tobytes() == vv2.cpu().numpy().tobytes()
     nvox = int(sum(X * Y * Z for X, Y, Z in lib_sizes))
     return {"workload": f"batch{count}_shapes_16_128", "volumes": count, "voxels": nvox,
@@ -339,6 +343,7 @@ def bench_fusion_graph(torch, dev, nvol, steps):
     e_ms, _ = _event_ms(torch, lambda i: g.step(), 2, 1)
     g.capture()
     g_ms, _ = _event_ms(torch, lambda i: g.replay(), max(3, steps // 4), 1)
+    g.check()
     nvox = int(sum(a * b * c for a, b, c in shapes))
     return {"workload": f"fusion{nvol}_64_192_diff_graph", "volumes": nvol, "voxels": nvox,
             "frames_per_volume": K, "frame_size": [W, H],
@@ -420,6 +425,7 @@ def main():
         evs[i][1].record()
     torch.cuda.synchronize()
     wall = time.perf_counter() - t_start
+    ws.check()  # device-side status of the asynchronous calls
     assert int(d_count.item()) == count
     kernel_ms = sorted(a.elapsed_time(b) for a, b in evs)
     avg_kernel_ms = sum(kernel_ms) / len(kernel_ms)

@@ -66,6 +66,16 @@ size_t mcx_extract_workspace_size(int X, int Y, int Z);
 size_t mcx_extract_workspace_size_ex(int X, int Y, int Z, int64_t records);
 /* Zero a workspace once before its first use (stream-ordered fill kernel). */
 int mcx_workspace_init(void* d_ws, size_t ws_bytes, void* stream);
+/* Device-side status of the extraction calls made on a workspace since the
+ * last report (no counterpart in the reference, whose kernels cannot fail
+ * this way).  The scan's look-back waits are bounded; a wait that reaches its
+ * bound (a broken invariant, not expected in normal operation) marks the
+ * workspace and makes the call publish -1 as its vertex count (*d_count, and
+ * every d_vbase entry of a batched call).  Synchronises the stream, clears
+ * the mark and returns MCX_ERR_INTERNAL if it was set, else MCX_OK.  Calls
+ * made with h_count report (and clear) the mark themselves.  Meant for
+ * callers that pass no h_count: asynchronous loops and captured graphs. */
+int mcx_workspace_status(void* d_ws, size_t ws_bytes, void* stream);
 
 /* ---- extraction (replaces MarchingCubes::run, cu_marching_cubes_v1.cu:417-467:
  *      getOccupiedVoxels :185-206, computeOffsetsAndTotalVertexes :208-225,
@@ -131,7 +141,8 @@ int mcx_extract_batched(const mcx_batch* plan, mcx_vertex_voxel* d_vv, float* d_
  *      first occurrence), d_indices (int32 per input vertex: its unique
  *      vertex; consecutive triples are the triangles) and the unique count
  *      (d_nunique device int64 and/or h_nunique host, which synchronises).
- *      Deterministic.  V < 2^31.  Workspace: mcx_weld_workspace_size(V). */
+ *      Deterministic.  V < 2^31.  Keys may be any int64 (equal keys weld;
+ *      extraction keys are >= 0).  Workspace: mcx_weld_workspace_size(V). */
 size_t mcx_weld_workspace_size(int64_t V);
 int mcx_weld(const int64_t* d_edge_key, const float* d_pos, int64_t stride_floats, int64_t V,
              float* d_upos, int32_t* d_indices, int64_t* d_nunique, int64_t* h_nunique,

@@ -42,6 +42,7 @@ SIGNATURES = {
     "mcx_extract_workspace_size": (_sz, [_i32, _i32, _i32]),
     "mcx_extract_workspace_size_ex": (_sz, [_i32, _i32, _i32, _i64]),
     "mcx_workspace_init": (_i32, [_vp, _sz, _vp]),
+    "mcx_workspace_status": (_i32, [_vp, _sz, _vp]),
     "mcx_extract": (_i32, [_vp, _i32, _i32, _i32, _i32, _f32, _vp, _vp, _vp, _vp, _i64, _vp,
                            _vp, _vp, _sz, _vp]),
     "mcx_classify": (_i32, [_vp, _i32, _i32, _i32, _i32, _vp, _vp, _vp]),

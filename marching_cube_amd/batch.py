@@ -48,6 +48,14 @@ class BatchPlan:
                                                _stream_handle(self.device)), "workspace_init")
         return self.ws
 
+    def check(self):
+        """Raise McxError if a batched extraction reported a device-side error
+        since the last check (synchronises the stream)."""
+        if self.ws is not None:
+            _capi.check(_capi.load().mcx_workspace_status(_ptr(self.ws), self.ws.numel(),
+                                                          _stream_handle(self.device)),
+                        "batched extraction workspace")
+
     def extract_into(self, *, vv=None, pos=None, v2vox=None, vbase=None, cap=None, d_count=None,
                      sync=True):
         lib = _capi.load()

@@ -1,6 +1,7 @@
 // mcx_core.hip -- library-level C ABI: version, status strings, tables,
 // workspace initialisation and the stream-ordered fill kernels.
 #include <hip/hip_runtime.h>
+#include <stddef.h>
 #include <string.h>
 
 #include "mc_tables.inc"
@@ -73,6 +74,13 @@ int fill_bytes_zero(void* d_ptr, size_t n_bytes, hipStream_t s) {
   return MCX_OK;
 }
 
+int take_error(void* d_ws, hipStream_t s) {
+  int rc = fill_u32(static_cast<char*>(d_ws) + offsetof(WsHeader, error), 0u, 1, s);
+  if (rc) return rc;
+  MCX_HIP(hipStreamSynchronize(s));
+  return MCX_ERR_INTERNAL;
+}
+
 }  // namespace mcx
 
 using namespace mcx;
@@ -104,4 +112,14 @@ extern "C" int mcx_get_tables(uint64_t* case_table, uint8_t* num_verts) {
 extern "C" int mcx_workspace_init(void* d_ws, size_t ws_bytes, void* stream) {
   if (!d_ws || ws_bytes < kWsHeader + 16) return MCX_ERR_WORKSPACE;
   return fill_bytes_zero(d_ws, ws_bytes, as_stream(stream));
+}
+
+extern "C" int mcx_workspace_status(void* d_ws, size_t ws_bytes, void* stream) {
+  if (!d_ws || ws_bytes < kWsHeader) return MCX_ERR_WORKSPACE;
+  hipStream_t s = as_stream(stream);
+  uint32_t err = 0;
+  MCX_HIP(hipMemcpyAsync(&err, static_cast<char*>(d_ws) + offsetof(WsHeader, error), sizeof(err),
+                         hipMemcpyDeviceToHost, s));
+  MCX_HIP(hipStreamSynchronize(s));
+  return err ? take_error(d_ws, s) : MCX_OK;
 }

@@ -44,6 +44,7 @@
 
 #include "mc_tables.inc"
 #include "mcx_internal.h"
+#include "mcx_scan_part.h"
 
 namespace mcx {
 namespace {
@@ -247,7 +248,10 @@ __device__ __forceinline__ int volume_of_tile(const ExtractParams& P, int64_t t)
 __device__ __forceinline__ void clear_scan_descriptors(const ExtractParams& P) {
   if (blockIdx.x == 0) {
     for (int64_t i = threadIdx.x; i < 3 * P.cap_desc; i += blockDim.x) P.desc[i] = 0;
-    if (threadIdx.x == 0) P.hdr->dense_n = 0;
+    if (threadIdx.x == 0) {
+      P.hdr->dense_n = 0;
+      P.hdr->scan_ticket = 0;
+    }
   }
 }
 
@@ -898,12 +902,20 @@ void count4_kernel(ExtractParams P) {
 // are compacted into clist with their first record (lrs) and the exclusive
 // scan of their record counts (lpre), and units[k] is set to the clist entry
 // holding record 64 k.  The three scans have descriptor sets of their own,
-// looked back by waves 0, 1 and 2.
+// looked back by waves 0, 1 and 2.  A workgroup's block of tiles is the
+// ticket it takes when it starts (not its blockIdx), so the blocks it waits
+// for belong to workgroups that are already running.  The workgroup with the
+// last ticket publishes the totals (vertex count, list entries, list
+// records) from its inclusive prefixes, the only writer of each.
 __global__ __launch_bounds__(256) void scan_kernel(ExtractParams P) {
   __shared__ int64_t s_wave[3][4];
   __shared__ int64_t s_excl[3];
+  __shared__ uint32_t s_ticket;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int64_t blk = blockIdx.x;
+  if (tid == 0) s_ticket = atomicAdd(&P.hdr->scan_ticket, 1u);
+  __syncthreads();
+  const int64_t blk = s_ticket;
+  const bool last_blk = blk == (int64_t)gridDim.x - 1;
   const int64_t i0 = blk * kScanItems + (int64_t)tid * kScanPer;
   // read before this block publishes anything: the last block resets the
   // counter once every block has published
@@ -941,10 +953,8 @@ __global__ __launch_bounds__(256) void scan_kernel(ExtractParams P) {
 
   // ---- list: this block's contiguous range, c entries per thread
   const bool lst = P.list_scan && nz > 0;
-  const int64_t L = lst ? ((int64_t)nz + gridDim.x - 1) / gridDim.x : 0;  // <= kScanItems
-  const int c = (int)((L + 255) / 256);                                     // <= kScanPer
-  const int64_t lb = min(blk * L, (int64_t)nz), le = min(lb + L, (int64_t)nz);
-  const int64_t j0 = lb + (int64_t)tid * c;
+  const ListPart part = list_part(lst ? (int64_t)nz : 0, gridDim.x, blk);  // c <= kScanPer
+  const int64_t j0 = part.lb + (int64_t)tid * part.c;
   uint32_t tl[kScanPer], vl[kScanPer];
   int32_t n[kScanPer];
   int64_t rsl[kScanPer];
@@ -965,7 +975,7 @@ __global__ __launch_bounds__(256) void scan_kernel(ExtractParams P) {
     }
 #pragma unroll
     for (int k = 0; k < kScanPer; ++k) {
-      if (k < c && j0 + k < le) {
+      if (list_slot(part, tid, k) >= 0) {
         if (n[k] < 0) {  // the staged path (rare): off the list
           P.dense_list[atomicAdd(&P.hdr->dense_n, 1u)] = tl[k];
           n[k] = 0;
@@ -1001,15 +1011,6 @@ __global__ __launch_bounds__(256) void scan_kernel(ExtractParams P) {
     if (lane == 0) {
       if (blk > 0) st_agent(P.desc + blk, kStatInc | (uint64_t)(excl + total0));
       s_excl[0] = excl;
-      if (blk == (int64_t)gridDim.x - 1) {
-        P.hdr->count = excl + total0;
-        if (P.d_count) *P.d_count = excl + total0;
-        P.hdr->dense_last = P.hdr->dense_acc;  // K1 has finished: hand the count over
-        P.hdr->dense_acc = 0;
-        P.hdr->nz_total = P.hdr->nz_count;     // the emit kernel's work list
-        P.hdr->nz_count = 0;
-        P.hdr->ovf_next = 0;
-      }
     }
   } else if (lst && wave < 3) {
     uint64_t* desc = P.desc + wave * P.cap_desc;
@@ -1025,6 +1026,25 @@ __global__ __launch_bounds__(256) void scan_kernel(ExtractParams P) {
     if (lane == 0) s_excl[wave] = excl;
   }
   __syncthreads();
+  if (last_blk && tid == 0) {
+    // every other workgroup has published (the look-backs ended), so every
+    // one has read nz_count: the counters of the count kernel are handed over
+    // and reset for the next call
+    const bool bad = __hip_atomic_load(&P.hdr->error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0;
+    const int64_t cnt = bad ? -1 : s_excl[0] + total0;
+    P.hdr->count = cnt;
+    if (P.d_count) *P.d_count = cnt;
+    P.hdr->dense_last = P.hdr->dense_acc;  // K1 has finished: hand the count over
+    P.hdr->dense_acc = 0;
+    P.hdr->nz_total = nz;                  // the emit kernel's work list
+    P.hdr->nz_count = 0;
+    P.hdr->ovf_next = 0;
+    if (lst) {
+      const int64_t nl = s_excl[2] + total2;  // list entries with records
+      P.hdr->list_n = (uint32_t)nl;
+      P.lpre[nl] = s_excl[1] + total1;        // records of the whole list
+    }
+  }
   int64_t run = s_excl[0] + texcl0;
 #pragma unroll
   for (int k = 0; k < kScanPer; ++k) {
@@ -1045,10 +1065,6 @@ __global__ __launch_bounds__(256) void scan_kernel(ExtractParams P) {
         r += n[k];
         ++e;
       }
-      if (k < c && j0 + k == (int64_t)nz - 1) {  // the list's last entry: its totals
-        P.lpre[e] = r;
-        P.hdr->list_n = (uint32_t)e;
-      }
     }
   }
 }
@@ -1058,6 +1074,10 @@ __global__ __launch_bounds__(256) void vbase_kernel(ExtractParams P) {
   const int b = blockIdx.x * blockDim.x + threadIdx.x;
   if (b > P.nvol) return;
   const int64_t total = P.hdr->count;
+  if (total < 0 || P.hdr->error) {  // the scan reported an error: every base is -1
+    P.vbase[b] = -1;
+    return;
+  }
   if (b == P.nvol) {
     P.vbase[b] = total;
     return;
@@ -1360,6 +1380,12 @@ void emit_units_kernel(ExtractParams P) {
   const int lane = threadIdx.x & 63;
   const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
   s_mark[wv][lane] = 0;
+  if (blockIdx.x == 0 && threadIdx.x == 0 && P.hdr->error) {
+    // a look-back of the scan (any workgroup) hit its wait limit: the
+    // offsets are not to be trusted, and the count says so
+    P.hdr->count = -1;
+    if (P.d_count) *P.d_count = -1;
+  }
   const VolDesc& V = P.vol0;  // single volumes
   const int mdim = BATCH ? kNodeTable + 1 : max(V.X, max(V.Y, V.Z));
   if (threadIdx.x == 0) s_node_ok = mdim <= kNodeTable;
@@ -1960,7 +1986,7 @@ extern "C" int mcx_extract_ex(const void* d_vol, int elem, int X, int Y, int Z, 
     MCX_HIP(hipMemcpyAsync(&hh, d_ws, sizeof(hh), hipMemcpyDeviceToHost, s));
     MCX_HIP(hipStreamSynchronize(s));
     *h_count = hh.count;
-    if (hh.error) return MCX_ERR_INTERNAL;
+    if (hh.error) return take_error(d_ws, s);
     if (hh.count > cap) return MCX_ERR_CAPACITY;
   }
   return MCX_OK;
@@ -2023,7 +2049,7 @@ extern "C" int mcx_extract_diff(const float* d_vol, int X, int Y, int Z, float h
     rc = read_header(d_ws, &hh, s);
     if (rc) return rc;
     *h_count = hh.count;
-    if (hh.error) return MCX_ERR_INTERNAL;
+    if (hh.error) return take_error(d_ws, s);
     if (hh.dense_last) return MCX_ERR_WORKSPACE;  // backward needs every record
     if (hh.count > cap) return MCX_ERR_CAPACITY;
   }
@@ -2210,7 +2236,7 @@ extern "C" int mcx_extract_batched(const mcx_batch* plan, mcx_vertex_voxel* d_vv
     rc = read_header(d_ws, &hh, s);
     if (rc) return rc;
     *h_count = hh.count;
-    if (hh.error) return MCX_ERR_INTERNAL;
+    if (hh.error) return take_error(d_ws, s);
     if (hh.count > cap) return MCX_ERR_CAPACITY;
   }
   return MCX_OK;

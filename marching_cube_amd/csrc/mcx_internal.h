@@ -54,16 +54,21 @@ inline hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s);
 int fill_u32(void* d_ptr, uint32_t value, size_t n_words, hipStream_t s);
 int fill_bytes_zero(void* d_ptr, size_t n_bytes, hipStream_t s);
 
+// A workspace's error word was found set: clear it (stream-ordered fill,
+// then a synchronisation) and return MCX_ERR_INTERNAL, so that the error is
+// reported once.
+int take_error(void* d_ws, hipStream_t s);
+
 // ------------------------------------------------------ workspace layout
-// [0, 256)   header: u32 dense_acc, u32 dense_last, u32 (unused), u32 error, i64 count, ...
+// [0, 256)   header: u32 dense_acc, u32 dense_last, u32 scan_ticket, u32 error, i64 count, ...
 // [256, ...) u64 scan look-back descriptors (zeroed by each call's count kernel)
 constexpr size_t kWsHeader = 256;
 struct WsHeader {
   uint32_t dense_acc;   // tiles of the running call that took the staged path
   uint32_t dense_last;  // the same count for the last completed call
-  uint32_t unused;
-  uint32_t error;       // sticky: a look-back that timed out (until re-init)
-  int64_t count;
+  uint32_t scan_ticket; // scan workgroups started by the running call (their order)
+  uint32_t error;       // a look-back wait that hit its limit since the last report
+  int64_t count;        // vertices of the last call; -1 when its scan reported an error
   uint32_t nz_count;    // non-empty tiles appended by the running count kernel
   uint32_t nz_total;    // the same, frozen by the scan for the emit kernel
   uint64_t ovf_next;    // records taken from the shared overflow pool
@@ -195,10 +200,13 @@ __device__ __forceinline__ void st_agent(uint64_t* p, uint64_t v) {
 // the sum of all tiles before t.  Lane l inspects descriptors
 // j - l*W - w (w < W), so one round trip to memory covers 64*W tiles and the
 // chain of inclusive prefixes advances faster than tiles are issued.
-// Tiles are numbered in dispatch order and publish their aggregate before
-// they look back themselves, so every wait ends once the predecessors run; a
-// wait beyond kSpinLimit sets *err and counts the tile as inclusive 0, so the
-// wave always exits.
+// Tiles are numbered by a ticket each workgroup takes when it starts (an
+// atomic counter), so every predecessor of a waiting tile is already running,
+// and every tile publishes its aggregate before it looks back itself: every
+// wait ends.  A wait beyond kSpinLimit (a broken invariant, never expected)
+// sets *err and counts the tile as inclusive 0, so the wave still exits; the
+// caller then publishes -1 as its vertex count and the error stays set in the
+// workspace until reported (mcx_workspace_status or a synchronous call).
 template <int W>
 __device__ __forceinline__ int64_t lookback(const uint64_t* desc, int64_t t, uint32_t* err) {
   const int lane = lane_id();

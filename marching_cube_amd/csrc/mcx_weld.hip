@@ -29,6 +29,9 @@
 namespace mcx {
 namespace {
 
+// An empty slot holds kEmptyKey.  A vertex whose key equals it (-1 as int64:
+// never a grid-edge key, but the entry point accepts any int64) is kept in
+// the extra value slot T past the table, so that every key works.
 constexpr uint64_t kEmptyKey = ~0ull;
 constexpr int kWeldBlock = 1024;  // vertices per flag block (256 threads x 4)
 
@@ -42,6 +45,7 @@ __device__ __forceinline__ uint64_t mix64(uint64_t k) {
 }
 
 __device__ __forceinline__ uint64_t find_slot(const uint64_t* keys, uint64_t mask, uint64_t key) {
+  if (key == kEmptyKey) return mask + 1;  // its own slot past the table
   uint64_t h = mix64(key) & mask;
   while (keys[h] != key) h = (h + 1) & mask;  // the key is present (inserted by W1)
   return h;
@@ -64,6 +68,10 @@ __global__ __launch_bounds__(256) void weld_insert_kernel(const int64_t* __restr
     dup |= i < lane && lo == klo && hi == khi;
   }
   if (v >= V || dup) return;
+  if (key == kEmptyKey) {
+    atomicMin(vals + mask + 1, (uint32_t)v);
+    return;
+  }
   uint64_t h = mix64(key) & mask;
   // at most V distinct keys in >= 2V slots: the probe ends
   for (;;) {
@@ -169,7 +177,10 @@ __global__ __launch_bounds__(256) void weld_place_kernel(int64_t V, const uint32
 __global__ __launch_bounds__(256) void weld_index_kernel(int64_t V, const uint32_t* rep,
                                                          const uint32_t* uidx, int32_t* idx) {
   const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (v < V) idx[v] = (int32_t)uidx[rep[v]];
+  if (v < V) {
+    const uint32_t r = rep[v];  // the key's first occurrence: r <= v
+    idx[v] = r < (uint64_t)V ? (int32_t)uidx[r] : -1;
+  }
 }
 
 __global__ __launch_bounds__(256) void fill_u64_kernel(uint64_t* p, int64_t n, uint64_t val) {
@@ -196,7 +207,8 @@ extern "C" size_t mcx_weld_workspace_size(int64_t V) {
   if (V < 0) return 0;
   const uint64_t T = table_slots(V);
   const int64_t nb = (V + kWeldBlock - 1) / kWeldBlock;
-  return al256(T * 8) + al256(T * 4) + 3 * al256((size_t)V * 4 + 4) + al256((size_t)(nb + 1) * 4);
+  return al256(T * 8) + al256((T + 1) * 4) + 3 * al256((size_t)V * 4 + 4) +
+         al256((size_t)(nb + 1) * 4);
 }
 
 extern "C" int mcx_weld(const int64_t* d_edge_key, const float* d_pos, int64_t stride_floats,
@@ -220,7 +232,7 @@ extern "C" int mcx_weld(const int64_t* d_edge_key, const float* d_pos, int64_t s
   uint64_t* keys = reinterpret_cast<uint64_t*>(b);
   b += al256(T * 8);
   uint32_t* vals = reinterpret_cast<uint32_t*>(b);
-  b += al256(T * 4);
+  b += al256((T + 1) * 4);
   uint32_t* rep = reinterpret_cast<uint32_t*>(b);
   b += al256((size_t)V * 4 + 4);
   uint32_t* uidx = reinterpret_cast<uint32_t*>(b);
@@ -232,7 +244,7 @@ extern "C" int mcx_weld(const int64_t* d_edge_key, const float* d_pos, int64_t s
   hipLaunchKernelGGL(fill_u64_kernel, dim3((unsigned)fb), dim3(256), 0, s, keys, (int64_t)T,
                      kEmptyKey);
   MCX_HIP(hipGetLastError());
-  int rc = fill_u32(vals, 0xffffffffu, (size_t)T, s);
+  int rc = fill_u32(vals, 0xffffffffu, (size_t)T + 1, s);
   if (rc) return rc;
   const uint64_t mask = T - 1;
   hipLaunchKernelGGL(weld_insert_kernel, dim3(grid_for(V, 256)), dim3(256), 0, s, d_edge_key, V,

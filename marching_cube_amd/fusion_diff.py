@@ -243,3 +243,10 @@ class FusionGraph:
 
     def replay(self):
         self.graph.replay()
+
+    def check(self):
+        """Raise McxError if an extraction of the pass reported a device-side
+        error since the last check (synchronises the stream)."""
+        _capi.check(_capi.load().mcx_workspace_status(_ptr(self.ws), self.ws.numel(),
+                                                      _stream_handle(self.dev)),
+                    "fusion graph extraction workspace")

@@ -77,6 +77,14 @@ class Workspace:
     def ptr(self):
         return _ptr(self.buf)
 
+    def check(self):
+        """Raise McxError if an extraction on this workspace reported a
+        device-side error since the last check (synchronises the stream; for
+        calls made with sync=False and for graph replays)."""
+        _capi.check(_capi.load().mcx_workspace_status(self.ptr, self.nbytes,
+                                                      _stream_handle(self.device)),
+                    "extraction workspace")
+
     @property
     def nbytes(self):
         return self.buf.numel()

@@ -100,7 +100,7 @@ def test_welded_sphere_is_closed_manifold(dev):
 
 @pytest.mark.parametrize("pattern,V", [("one_key", 1000), ("few", 1), ("few", 63), ("few", 65),
                                        ("few", 4097), ("pairs", 640), ("runs", 100003),
-                                       ("mesh_like", 300001)])
+                                       ("mesh_like", 300001), ("negative", 5000)])
 def test_weld_synthetic_keys(dev, pattern, V):
     """mcx_weld on its own (C ABI) with keys that repeat inside a wave, across
     waves and at wave boundaries, and sizes that are not multiples of 64:
@@ -123,6 +123,12 @@ def test_weld_synthetic_keys(dev, pattern, V):
             keys[64 * w + 62] = keys[64 * w + 1]
     elif pattern == "runs":
         keys = np.repeat(rng.integers(0, 1 << 40, V // 5 + 1), 5)[:V].astype(np.int64)
+    elif pattern == "negative":
+        # -1 (the bit pattern of an empty table slot, kept in a slot of its
+        # own), the int64 extremes and other negative keys among small ones
+        pool = np.array([-1, -2, np.iinfo(np.int64).min, np.iinfo(np.int64).max, 0, 1, 2, 3,
+                         -(1 << 40)], np.int64)
+        keys = pool[rng.integers(0, len(pool), V)]
     else:  # about six occurrences per key, spread over the array
         keys = (rng.integers(0, V // 6, V) * 3 + 1).astype(np.int64)
     pos = rng.standard_normal((V, 3)).astype(np.float32)
