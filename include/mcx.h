@@ -106,6 +106,27 @@ int mcx_extract_ex(const void* d_vol, int elem, int X, int Y, int Z, float h,
                    int16_t* d_ver_voxel, int64_t* d_edge_key, int64_t cap, int64_t* d_count,
                    int64_t* h_count, void* d_ws, size_t ws_bytes, void* stream);
 
+/* ---- extraction without allocation (no counterpart in the reference) -----
+ * For callers that cannot allocate device memory: code under hipGraph
+ * capture, tight serving loops, harnesses that forbid hipMalloc.
+ * mcx_extract_static = mcx_extract without a workspace argument: the
+ * library's own device array of 64 MiB per GPU (MCX_STATIC_SCRATCH_BYTES,
+ * loaded with the library's code, held until the process ends) is the
+ * workspace.  The call allocates nothing and clears the array's counters
+ * with a fill kernel.  mcx_extract_static_limit(X, Y, Z) is 0 for volumes
+ * whose tile tables leave no room for 65,536 records in the array (those
+ * calls return MCX_ERR_UNSUPPORTED; 512^3 fits), else the array's size;
+ * tiles whose records do not fit the rest are re-read from the volume (same
+ * output, slower).  Every call on a GPU shares that GPU's array and leaves
+ * its contents behind: calls that overlap on one GPU (other streams, other
+ * threads) race, calls ordered on one stream do not.  A device-side error
+ * shows as -1 in *d_count (MCX_ERR_INTERNAL when h_count is passed). */
+size_t mcx_extract_static_limit(int X, int Y, int Z);
+int mcx_extract_static(const void* d_vol, int elem, int X, int Y, int Z, float h,
+                       mcx_vertex_voxel* d_vv, float* d_pos, int32_t* d_v2vox,
+                       int16_t* d_ver_voxel, int64_t cap, int64_t* d_count, int64_t* h_count,
+                       void* stream);
+
 /* Per-cube classification (the per-cube part of OccupiedVoxels,
  * cu_marching_cubes_v1.cu:120-131): d_ci / d_nv (X*Y*Z bytes each, indexed by
  * the cube's min-corner voxel; 0 for cubes that do not exist). */

@@ -46,6 +46,9 @@ SIGNATURES = {
     "mcx_extract": (_i32, [_vp, _i32, _i32, _i32, _i32, _f32, _vp, _vp, _vp, _vp, _i64, _vp,
                            _vp, _vp, _sz, _vp]),
     "mcx_classify": (_i32, [_vp, _i32, _i32, _i32, _i32, _vp, _vp, _vp]),
+    "mcx_extract_static_limit": (_sz, [_i32, _i32, _i32]),
+    "mcx_extract_static": (_i32, [_vp, _i32, _i32, _i32, _i32, _f32, _vp, _vp, _vp, _vp, _i64,
+                                  _vp, _vp, _vp]),
     "mcx_batch_create": (_i32, [_vp, _i32, _vp, _vp, _vp, _i32]),
     "mcx_batch_destroy": (None, [_vp]),
     "mcx_batch_tiles": (_i64, [_vp]),

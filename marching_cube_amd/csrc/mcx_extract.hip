@@ -40,6 +40,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <mutex>
 #include <vector>
 
 #include "mc_tables.inc"
@@ -1990,6 +1991,63 @@ extern "C" int mcx_extract_ex(const void* d_vol, int elem, int X, int Y, int Z, 
     if (hh.count > cap) return MCX_ERR_CAPACITY;
   }
   return MCX_OK;
+}
+
+// ---- extraction without allocation (no counterpart in the reference) ----
+// For callers that cannot allocate device memory (code under graph capture,
+// tight serving loops): the workspace is this file-scope device array, one
+// copy per GPU, held until the process ends.  Host code never passes the
+// array's name to a kernel (on the host it only stands for the symbol): its
+// device address is looked up once per device with hipGetSymbolAddress and
+// kept below.  Every call on a device uses the same array, so calls that
+// overlap on one GPU (different streams) race; calls ordered on one stream
+// do not.
+namespace {
+constexpr size_t kStaticScratchBytes = MCX_STATIC_SCRATCH_BYTES;
+__device__ __align__(256) uint8_t scratch[kStaticScratchBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of this device's scratch array (looked up on first use).
+int static_scratch(void** out) {
+  int dev = 0;
+  MCX_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) return MCX_ERR_UNSUPPORTED;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    MCX_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) return MCX_ERR_HIP;
+    g_scratch_ptr[dev] = p;
+  }
+  *out = g_scratch_ptr[dev];
+  return MCX_OK;
+}
+}  // namespace
+
+extern "C" size_t mcx_extract_static_limit(int X, int Y, int Z) {
+  if (X < 1 || Y < 1 || Z < 1 || !size_ok(X, Y, Z)) return 0;
+  const size_t need = fixed_ws_bytes(tiles_of(X, Y, Z)) + rec_area_bytes(kStaticMinRecords);
+  return need <= kStaticScratchBytes ? kStaticScratchBytes : 0;
+}
+
+extern "C" int mcx_extract_static(const void* d_vol, int elem, int X, int Y, int Z, float h,
+                                  mcx_vertex_voxel* d_vv, float* d_pos, int32_t* d_v2vox,
+                                  int16_t* d_ver_voxel, int64_t cap, int64_t* d_count,
+                                  int64_t* h_count, void* stream) {
+  if (!d_vol || X < 1 || Y < 1 || Z < 1 || cap < 0) return MCX_ERR_ARG;
+  if (!mcx_extract_static_limit(X, Y, Z)) return MCX_ERR_UNSUPPORTED;
+  void* ws = nullptr;
+  int rc = static_scratch(&ws);
+  if (rc) return rc;
+  // the header's counters must start at zero: cleared by a fill kernel
+  // (no memset node, no allocation); the rest is written before it is read
+  rc = fill_bytes_zero(ws, kWsHeader, as_stream(stream));
+  if (rc) return rc;
+  return mcx_extract_ex(d_vol, elem, X, Y, Z, h, d_vv, d_pos, d_v2vox, d_ver_voxel, nullptr, cap,
+                        d_count, h_count, ws, kStaticScratchBytes, stream);
 }
 
 extern "C" int mcx_classify(const void* d_vol, int elem, int X, int Y, int Z, uint8_t* d_ci,

@@ -63,6 +63,10 @@ int take_error(void* d_ws, hipStream_t s);
 // [0, 256)   header: u32 dense_acc, u32 dense_last, u32 scan_ticket, u32 error, i64 count, ...
 // [256, ...) u64 scan look-back descriptors (zeroed by each call's count kernel)
 constexpr size_t kWsHeader = 256;
+// The device array behind mcx_extract_static (bytes per GPU) and the fewest
+// records it must leave room for beside the tile tables.
+#define MCX_STATIC_SCRATCH_BYTES (64ull << 20)
+constexpr int64_t kStaticMinRecords = 65536;
 struct WsHeader {
   uint32_t dense_acc;   // tiles of the running call that took the staged path
   uint32_t dense_last;  // the same count for the last completed call
