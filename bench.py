@@ -22,6 +22,8 @@ import os
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
@@ -36,7 +38,9 @@ def parse():
     ap.add_argument("--frames", type=int, default=300)
     ap.add_argument("--dim", type=int, default=512)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--cpu-threads", type=int, default=16)
+    ap.add_argument("--cpu-threads", type=int, default=0,
+                    help="threads of the all-cores CPU baseline (0 = every core this process "
+                         "may use: its CPU affinity, capped by OMP_NUM_THREADS when set)")
     ap.add_argument("--diff-dim", type=int, default=256,
                     help="side of the fp32 bumps field of the forward/backward leg (0 = skip)")
     ap.add_argument("--batch-count", type=int, default=3000,
@@ -55,6 +59,50 @@ def parse():
     ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "extract_kinfu512_traffic.json"),
                     help="JSON with measured HBM bytes per launch (from rocprofv3 --pmc)")
     return ap.parse_args()
+
+
+def host_threads(requested=0):
+    """Threads for the all-cores CPU baseline: `requested`, or every core this
+    process may run on (its affinity set), capped by OMP_NUM_THREADS."""
+    if requested > 0:
+        return requested
+    n = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    omp = os.environ.get("OMP_NUM_THREADS", "")
+    if omp.isdigit() and int(omp) > 0:
+        n = min(n, int(omp))
+    return max(1, n)
+
+
+def cpu_extract_times(vols, hs, nthreads, reps):
+    """Seconds per pass of the oracle's extraction (vertex_voxel output) over
+    every volume of `vols`: median of `reps` passes after one warm-up pass."""
+    from oracle import oracle as O
+    for v, h in zip(vols, hs):
+        O.extract(v, float(h), want=("vv",), nthreads=nthreads)
+    times, n = [], 0
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        n = 0
+        for v, h in zip(vols, hs):
+            n += O.extract(v, float(h), want=("vv",), nthreads=nthreads)["n"]
+        times.append(time.perf_counter() - t0)
+    return sorted(times)[len(times) // 2], n
+
+
+def cpu_baseline_entry(vols, hs, threads_all, reps_single, reps_all, sample):
+    """CPU baseline of an extraction workload: single thread and all cores
+    (OpenMP over each volume's planes), Gvoxel/s and Mtri/s of the sample."""
+    nvox = int(sum(int(np.prod(v.shape[:3])) for v in vols))
+    t1, n1 = cpu_extract_times(vols, hs, 1, reps_single)
+    ta, na = cpu_extract_times(vols, hs, threads_all, reps_all)
+    assert n1 == na
+    return {"value": round(nvox / ta / 1e9, 4), "unit": "Gvoxel/s", "cores": threads_all,
+            "kind": "port", "mtri_per_s": round(na / 3 / ta / 1e6, 2),
+            "seconds_per_pass": round(ta, 4),
+            "single_thread": {"value": round(nvox / t1 / 1e9, 4), "unit": "Gvoxel/s",
+                              "cores": 1, "mtri_per_s": round(n1 / 3 / t1 / 1e6, 2),
+                              "seconds_per_pass": round(t1, 4)},
+            "nproc": os.cpu_count(), "sample": sample}
 
 
 def _event_ms(torch, fn, steps, warmup):
@@ -191,6 +239,37 @@ def bench_batch(torch, dev, count, steps, warmup):
             "batched_mtri_per_s": round(n / 3 / (b_ms * 1e-3) / 1e6, 1)}
 
 
+def bench_c1(torch, dev, steps, warmup, threads_all, no_cpu):
+    """The reference's CPU-path case: 128^3 short2 sphere (r 0.3 m in a 1 m
+    cube, trunc 3 voxels), vertex_voxel output.  GPU: one extraction call per
+    step between HIP events (the 8 MiB volume stays in cache, so this is not
+    an HBM figure); CPU: the oracle, single thread and all cores."""
+    from marching_cube_amd import mesh, synth
+    vol_np, h = synth.sphere_volume(128)
+    vol = torch.from_numpy(vol_np.view(np.int32)).to(dev)
+    ws = mesh.Workspace(dev)
+    n = mesh.extract_into(vol, h, cap=0, workspace=ws)
+    vv = torch.empty((n, mesh.VV_WORDS), dtype=torch.int32, device=dev)
+    dc = torch.zeros(1, dtype=torch.int64, device=dev)
+
+    def step(i):
+        mesh.extract_into(vol, h, vv=vv, cap=n, d_count=dc, workspace=ws, sync=False)
+
+    avg, med = _event_ms(torch, step, max(steps, 20), warmup)
+    ws.check()
+    assert int(dc.item()) == n
+    nvox = 128 ** 3
+    out = {"workload": "sphere128_extract", "vertices": n, "ms": round(avg, 4),
+           "ms_median": round(med, 4), "gvoxel_per_s": round(nvox / (avg * 1e-3) / 1e9, 2),
+           "mtri_per_s": round(n / 3 / (avg * 1e-3) / 1e6, 1), "cpu_baseline": None}
+    if not no_cpu:
+        out["cpu_baseline"] = cpu_baseline_entry(
+            [vol_np], [h], threads_all, 5, 9,
+            "the whole 128^3 volume; single thread: median of 5 passes, all cores: median of "
+            "9, each after 1 warm-up pass")
+    return out
+
+
 def bench_raycast(torch, dev, vol, h, cfg, views, steps, warmup, cpu_threads, no_cpu):
     """Depth + normal maps (640x480, the frames' camera) ray-marched through
     the fused volume from `views` poses of the orbit, one launch per view,
@@ -218,7 +297,7 @@ def bench_raycast(torch, dev, vol, h, cfg, views, steps, warmup, cpu_threads, no
     if not no_cpu:
         from oracle import oracle as O
         host = vol.cpu().numpy().view(np.uint32)
-        nt = max(1, min(cpu_threads, os.cpu_count() or 1))
+        nt = host_threads(cpu_threads)
         O.raycast(host, h, intr, poses[0], W, H, origin=0.5, nthreads=nt)
         tc = time.perf_counter()
         O.raycast(host, h, intr, poses[0], W, H, origin=0.5, nthreads=nt)
@@ -477,27 +556,22 @@ def main():
     if ray_leg:
         print(f"[bench] raycast {ray_leg}", file=sys.stderr, flush=True)
 
-    # ---- CPU baseline: the oracle's extraction on the host cores
+    # ---- CPU baseline: the oracle's extraction on the host cores, single
+    # thread and all cores (the reference's CPU path is only declared,
+    # MyMarchingCube.h:4-10; the oracle restates its GPU semantics)
     cpu = None
     if not args.no_cpu:
-        from oracle import oracle as O
-        import numpy as np
         host = vols[0].cpu().numpy().view(np.uint32)
-        nt = max(1, min(args.cpu_threads, os.cpu_count() or 1))
-        O.extract(host, h, want=("vv",), nthreads=nt)  # warm-up
-        times = []
-        n_cpu = None
-        for _ in range(3):
-            tc = time.perf_counter()
-            r = O.extract(host, h, want=("vv",), nthreads=nt)
-            times.append(time.perf_counter() - tc)
-            n_cpu = r["n"]
-        tmed = sorted(times)[1]
-        assert n_cpu == count, (n_cpu, count)
-        cpu = {"value": round(nvox / tmed / 1e9, 4), "unit": "Gvoxel/s", "cores": nt,
-               "kind": "port",
-               "sample": f"full {dim}^3 volume (same input), vertex_voxel output, median of 3 "
-                         f"after 1 warm-up, {tmed:.3f} s per pass, {nt} OpenMP threads"}
+        nt = host_threads(args.cpu_threads)
+        cpu = cpu_baseline_entry(
+            [host], [h], nt, 3, 5,
+            f"full {dim}^3 volume (same input), vertex_voxel output; single thread: median of 3 "
+            f"passes, all cores ({nt} OpenMP threads over z planes): median of 5, each after "
+            f"1 warm-up pass")
+        print(f"[bench] cpu baseline {cpu}", file=sys.stderr, flush=True)
+    c1_leg = bench_c1(torch, dev, args.steps, args.warmup, host_threads(args.cpu_threads),
+                      args.no_cpu)
+    print(f"[bench] c1 {c1_leg}", file=sys.stderr, flush=True)
 
     if args.calib_copy:
         src = torch.empty(128 << 20, dtype=torch.int32, device=dev)
@@ -528,6 +602,7 @@ def main():
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic},
         "cpu_baseline": cpu,
+        "sphere128": c1_leg,
         "vertices": count,
         "triangles": count // 3,
         "mtri_per_s": round(count / 3 / (ms_per_step * 1e-3) / 1e6, 1),

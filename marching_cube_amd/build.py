@@ -4,13 +4,19 @@
 
 Every HIP source under csrc/ is compiled with hipcc for gfx950 only.  Flags:
   -Xclang -target-feature -Xclang -packed-fp32-ops  no packed-FP32 instructions
+      (the device pass honours it; the host pass warns that it does not know
+      the feature; check_no_packed_fp32 disassembles the built device code and
+      fails the build on any v_pk_add/mul/fma_f32)
   -ffp-contract=off                       every fused multiply-add is explicit
   -fhip-fp32-correctly-rounded-divide-sqrt  IEEE fp32 division (as nvcc's default)
 """
 import glob
 import os
+import re
+import shutil
 import subprocess
 import sys
+import tempfile
 
 PKG = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(PKG)
@@ -26,6 +32,37 @@ HIP_FLAGS = [
     "-ffp-contract=off", "-fhip-fp32-correctly-rounded-divide-sqrt",
     "-Wall", "-Wno-unused-function",
 ]
+
+
+LLVM_BIN = os.environ.get("LLVM_BIN", "/opt/rocm/llvm/bin")
+PACKED_FP32 = re.compile(r"\bv_pk_(add|mul|fma)_f32\b")
+
+
+def check_no_packed_fp32(lib):
+    """Disassemble the gfx950 code objects of a library built here and fail
+    on any packed FP32 add / multiply / fma (the build disables them with
+    -packed-fp32-ops; this checks that the device compiler honoured it).
+    Returns the number of instructions checked."""
+    with tempfile.TemporaryDirectory() as tmp:
+        copy = os.path.join(tmp, os.path.basename(lib))
+        shutil.copy(lib, copy)
+        # writes one file per bundle entry beside the copy
+        subprocess.run([os.path.join(LLVM_BIN, "llvm-objdump"), "--offloading", copy],
+                       check=True, capture_output=True, cwd=tmp)
+        objs = [f for f in os.listdir(tmp) if "amdgcn" in f and "gfx950" in f]
+        if not objs:
+            raise RuntimeError(f"no gfx950 code object found in {lib}")
+        ninsn = 0
+        for f in sorted(objs):
+            asm = subprocess.run([os.path.join(LLVM_BIN, "llvm-objdump"), "-d",
+                                  os.path.join(tmp, f)], check=True, capture_output=True,
+                                 text=True).stdout
+            bad = [ln.strip() for ln in asm.splitlines() if PACKED_FP32.search(ln)]
+            if bad:
+                raise RuntimeError(f"{lib}: packed FP32 instructions in the device code: "
+                                   f"{bad[:3]}")
+            ninsn += sum(1 for ln in asm.splitlines() if ln.startswith("\t"))
+    return ninsn
 
 
 def _run(cmd):
@@ -53,6 +90,8 @@ def build_lib(jobs=8):
             raise RuntimeError("hipcc failed")
     tmp = LIB + ".tmp"
     _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs])
+    n = check_no_packed_fp32(tmp)
+    print(f"device code: {n} instructions, no packed FP32 add/mul/fma", flush=True)
     os.replace(tmp, LIB)
     return LIB
 
