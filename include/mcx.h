@@ -153,6 +153,68 @@ int mcx_extract_batched(const mcx_batch* plan, mcx_vertex_voxel* d_vv, float* d_
                         int32_t* d_v2vox, int64_t cap, int64_t* d_vbase, int64_t* d_count,
                         int64_t* h_count, void* d_ws, size_t ws_bytes, void* stream);
 
+/* Voxels of all volumes of a plan (the length of a batch's concatenated
+ * per-voxel arrays: volume b's voxels follow those of volumes 0..b-1). */
+int64_t mcx_batch_voxels(const mcx_batch* plan);
+
+/* ---- batched differentiable fusion (no counterpart in the reference) ----
+ * A plan of fp32 volumes (elem MCX_ELEM_F32) whose fp32 TSDF lives in one
+ * concatenated array d_tsdf (volume b at d_tsdf + the voxels of volumes
+ * 0..b-1; d_vols[b] of mcx_batch_create must point there), with int32
+ * weights d_weight in the same layout.  Every volume is fused from K depth
+ * frames of W x H of its own (d_depth [nvol][K][H][W] floats, metres =
+ * value * depth_scale; d_poses [nvol][K][12]) with one set of intrinsics,
+ * the update of mcx_integrate_tsdf.  Each call is one launch sequence over
+ * all volumes and only launches kernels (capturable in a hipGraph).
+ * mcx_integrate_tsdf_batched: from_init != 0 starts every voxel from
+ *   (init_value, 0) without reading the arrays (else from their contents).
+ *   K <= 512.  Workspace: mcx_integrate_workspace_size(W, H).
+ * mcx_extract_diff_batched: mcx_extract_diff of every volume (float3
+ *   positions, vertex ranges in d_vbase as mcx_extract_batched), keeping in
+ *   the workspace what the backward needs.
+ * mcx_extract_backward_batched: dL/dtsdf (d_grad_vol, the concatenated
+ *   layout) from dL/dpos, with the workspace of the preceding
+ *   mcx_extract_diff_batched of this plan and d_scratch of
+ *   mcx_batch_backward_scratch_size bytes; d_maxabs (optional, device
+ *   uint32) receives max |dL/dtsdf| as float bits.  Fixed-order sums.
+ * mcx_integrate_tsdf_backward_batched: dL/d(depth value) [nvol][K][H][W]
+ *   from dL/dtsdf, as mcx_integrate_tsdf_backward per volume; d_weight0: the
+ *   weights before the fused frames (NULL: all 0); d_maxabs: max |dL/dtsdf|
+ *   bits from the previous call, or NULL to compute it.  Workspace:
+ *   mcx_integrate_tsdf_backward_batched_workspace_size. */
+int mcx_integrate_tsdf_batched(const mcx_batch* plan, float* d_tsdf, int32_t* d_weight,
+                               int from_init, float init_value, float trunc, int max_w,
+                               const float* intr, const float* d_poses, const float* d_depth,
+                               float depth_scale, int W, int H, int K, void* d_ws,
+                               size_t ws_bytes, void* stream);
+int mcx_extract_diff_batched(const mcx_batch* plan, float* d_pos, int64_t cap, int64_t* d_vbase,
+                             int64_t* d_count, int64_t* h_count, void* d_ws, size_t ws_bytes,
+                             void* stream);
+size_t mcx_batch_backward_scratch_size(const mcx_batch* plan, size_t ws_bytes);
+int mcx_extract_backward_batched(const mcx_batch* plan, const float* d_grad_pos,
+                                 float* d_grad_vol, uint32_t* d_maxabs, void* d_ws,
+                                 size_t ws_bytes, void* d_scratch, size_t scratch_bytes,
+                                 void* stream);
+size_t mcx_integrate_tsdf_backward_batched_workspace_size(const mcx_batch* plan, int W, int H,
+                                                          int K);
+int mcx_integrate_tsdf_backward_batched(const mcx_batch* plan, const float* d_grad_tsdf,
+                                        const int32_t* d_weight0, const uint32_t* d_maxabs,
+                                        float trunc, int max_w, const float* intr,
+                                        const float* d_poses, const float* d_depth,
+                                        float depth_scale, int W, int H, int K,
+                                        float* d_grad_depth, void* d_ws, size_t ws_bytes,
+                                        void* stream);
+/* Chamfer-style loss of a batch of meshes: vertex v of volume b (rows
+ * [d_vbase[b], d_vbase[b+1]) of d_pos, float3) to the nearest of volume b's
+ * M targets (d_targets [nvol][M][3]): d_loss[b] = sum of squared distances
+ * (fixed summation order), d_grad_pos = 2 (p - nearest) per row (0 for rows
+ * past the total, up to cap rows).  M <= 1024.  Workspace:
+ * mcx_chamfer_workspace_size(cap). */
+size_t mcx_chamfer_workspace_size(int64_t cap);
+int mcx_chamfer_batched(const float* d_pos, const int64_t* d_vbase, int nvol,
+                        const float* d_targets, int M, int64_t cap, float* d_loss,
+                        float* d_grad_pos, void* d_ws, size_t ws_bytes, void* stream);
+
 /* ---- indexed mesh (replaces the OBJ writer's de-duplication by printed
  *      coordinates, cu_kinect_fusion_v1.cu:3668-3719, with exact sharing by
  *      grid edge).  Input: the V vertices of an extraction (positions with a
@@ -230,6 +292,17 @@ int mcx_integrate_f32(void* d_vol, int X, int Y, int Z, float h, float trunc, in
                       const float* intr, const float* d_poses, const float* d_depth,
                       float depth_scale, int W, int H, int K, void* d_ws, size_t ws_bytes,
                       void* stream);
+/* Both with the reference's two truncation distances (cu_kinect_fusion_v1.cu:
+ * 455-463, the volume's min_t and max_t): a voxel is skipped where sdf <
+ * -trunc_neg, and t = sdf > 0 ? min(sdf / trunc_pos, 1) : sdf / trunc_neg
+ * (equal distances give the update above, bit for bit).  depth_kind:
+ * MCX_DEPTH_U16 (uint16 frames) or MCX_DEPTH_F32 (float frames). */
+#define MCX_DEPTH_U16 0
+#define MCX_DEPTH_F32 1
+int mcx_integrate_ex(void* d_vol, int X, int Y, int Z, float h, float trunc_neg, float trunc_pos,
+                     int max_w, const float* intr, const float* d_poses, const void* d_depth,
+                     int depth_kind, float depth_scale, int W, int H, int K, void* d_ws,
+                     size_t ws_bytes, void* stream);
 
 /* ---- fp32 TSDF volumes, differentiable integration (no counterpart in the
  *      reference).  tsdf (float) and weight (int32) in separate [z][y][x]

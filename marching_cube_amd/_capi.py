@@ -23,6 +23,9 @@ ERR_FORMAT = 7
 ELEM_SHORT2 = 0
 ELEM_F32 = 1
 
+DEPTH_U16 = 0
+DEPTH_F32 = 1
+
 TUNE_COLUMNS_PER_CU = 0
 TUNE_EMIT_GROUPS_PER_CU = 1
 TUNE_BATCH_SLAB = 2
@@ -88,6 +91,20 @@ SIGNATURES = {
                               _vp]),
     "mcx_rle_decode_workspace_size": (_sz, [_i64, _i32, _i32, _i32]),
     "mcx_rle_decode": (_i32, [_vp, _i64, _vp, _i32, _i32, _i32, _vp, _vp, _sz, _vp]),
+    "mcx_integrate_ex": (_i32, [_vp, _i32, _i32, _i32, _f32, _f32, _f32, _i32, _vp, _vp, _vp,
+                                _i32, _f32, _i32, _i32, _i32, _vp, _sz, _vp]),
+    "mcx_batch_voxels": (_i64, [_vp]),
+    "mcx_integrate_tsdf_batched": (_i32, [_vp, _vp, _vp, _i32, _f32, _f32, _i32, _vp, _vp, _vp,
+                                          _f32, _i32, _i32, _i32, _vp, _sz, _vp]),
+    "mcx_extract_diff_batched": (_i32, [_vp, _vp, _i64, _vp, _vp, _vp, _vp, _sz, _vp]),
+    "mcx_batch_backward_scratch_size": (_sz, [_vp, _sz]),
+    "mcx_extract_backward_batched": (_i32, [_vp, _vp, _vp, _vp, _vp, _sz, _vp, _sz, _vp]),
+    "mcx_integrate_tsdf_backward_batched_workspace_size": (_sz, [_vp, _i32, _i32, _i32]),
+    "mcx_integrate_tsdf_backward_batched": (_i32, [_vp, _vp, _vp, _vp, _f32, _i32, _vp, _vp,
+                                                   _vp, _f32, _i32, _i32, _i32, _vp, _vp, _sz,
+                                                   _vp]),
+    "mcx_chamfer_workspace_size": (_sz, [_i64]),
+    "mcx_chamfer_batched": (_i32, [_vp, _vp, _i32, _vp, _i32, _i64, _vp, _vp, _vp, _sz, _vp]),
     "mcx_synth_depth": (_i32, [_vp, _i32, _i32, _i32, _vp, _vp, _vp, _i32, _f32, _f32, _u32,
                                _f32, _vp]),
 }

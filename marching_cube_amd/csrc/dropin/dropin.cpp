@@ -216,10 +216,12 @@ void _cu_voxel_volumetric_integration(cu_voxel v, float_camera_intrinsic ci,
   cu_array<float> dpose;
   dpose.upload(pose, 12);
   cu_array<unsigned char> ws(mcx_integrate_workspace_size(size.x, size.y));
-  check_mcx(mcx_integrate_f32(v.data, v.size.x, v.size.y, v.size.z, v.grid_s, v.max_t,
-                              CU_VOXEL_MAX_W_INT, intr, dpose.ptr(), d_depth_ptr, 1.0f, size.x,
-                              size.y, 1, ws.ptr(), ws.size(), nullptr),
-            "mcx_integrate_f32");
+  // the volume's two truncation distances: min_t behind the surface, max_t
+  // in front of it (cu_kinect_fusion_v1.cu:455-463)
+  check_mcx(mcx_integrate_ex(v.data, v.size.x, v.size.y, v.size.z, v.grid_s, v.min_t, v.max_t,
+                             CU_VOXEL_MAX_W_INT, intr, dpose.ptr(), d_depth_ptr, MCX_DEPTH_F32,
+                             1.0f, size.x, size.y, 1, ws.ptr(), ws.size(), nullptr),
+            "mcx_integrate_ex");
   mcx_detail::check_hip(hipDeviceSynchronize(), "_cu_voxel_volumetric_integration");
 }
 

@@ -48,6 +48,21 @@
 #include "mcx_scan_part.h"
 
 namespace mcx {
+
+struct VolDesc {
+  const uint32_t* vol;   // X*Y*Z raw voxels (short2 bits or fp32 bits), [z][y][x]
+  int X, Y, Z;
+  int R;                 // tiles per plane = ceil(X*Y / kTile)
+  float h;
+  int64_t tile_begin;    // first tile of this volume in the launch
+  int16_t* ver_voxel;    // optional
+  FastDiv divX, divR;
+};
+
+struct Column {          // one K1 wave: volume, in-plane tile range, planes [z0, z1)
+  int vol, r, z0, z1;
+};
+
 namespace {
 
 __constant__ unsigned long long dCase[256] = MCX_CASE_TABLE_INIT;
@@ -115,23 +130,10 @@ constexpr SlotTable make_slot_table() {
 __constant__ SlotTable dSlot = make_slot_table();
 
 
-struct VolDesc {
-  const uint32_t* vol;   // X*Y*Z raw voxels (short2 bits or fp32 bits), [z][y][x]
-  int X, Y, Z;
-  int R;                 // tiles per plane = ceil(X*Y / kTile)
-  float h;
-  int64_t tile_begin;    // first tile of this volume in the launch
-  int16_t* ver_voxel;    // optional
-  FastDiv divX, divR;
-};
-
-struct Column {          // one K1 wave: volume, in-plane tile range, planes [z0, z1)
-  int vol, r, z0, z1;
-};
-
 struct ExtractParams {
   VolDesc vol0;          // single-volume launches
   const VolDesc* vols;   // batched launches: [nvol] in device memory
+  const BatchVol* bvol;  // batched launches: [nvol] per-voxel layout (backward)
   const Column* cols;    // batched launches: [ncols]
   int nvol;
   int64_t ncols;
@@ -1626,17 +1628,30 @@ __global__ __launch_bounds__(kEmitThreads) void contrib_kernel(ExtractParams P, 
 // cubes read their contribution (record index = tile's first record + rank
 // of the cube's bit).
 constexpr int kGatherSpan = 1024;  // voxels per workgroup (4 per thread)
+// Batched launches: workgroup blk belongs to the volume whose gather range
+// [gblk_begin, next) holds it, and writes that volume's part of the
+// concatenated gradient (from vox_begin).  With maxg, the largest |dL/dv| of
+// the launch goes there as float bits (atomicMax; order-independent).
+template <bool BATCH>
 __global__ __launch_bounds__(256) void gather_kernel(ExtractParams P, const float* contrib,
-                                                     float* gvol) {
+                                                     float* gvol, uint32_t* maxg) {
   __shared__ uint64_t s_mask[2][2][4][kCPW];  // [dz][dy][tile slot][word]
   __shared__ uint32_t s_pre[2][2][4][kCPW];   // set bits of the tile's earlier words
   __shared__ int64_t s_rec[2][2][4];          // first record of the tile
-  const VolDesc& V = P.vol0;
+  __shared__ uint32_t s_max[4];
+  int vi = 0;
+  uint32_t blk = blockIdx.x;
+  if constexpr (BATCH) {
+    vi = batch_volume_of<&BatchVol::gblk_begin>(P.bvol, P.nvol, blockIdx.x);
+    blk = (uint32_t)(blockIdx.x - P.bvol[vi].gblk_begin);
+    gvol += P.bvol[vi].vox_begin;
+  }
+  const VolDesc& V = vol_at<BATCH>(P, vi);
   const int X = V.X, Y = V.Y, Z = V.Z;
   const uint32_t XY = (uint32_t)X * (uint32_t)Y;  // < 2^30 (plane_ok)
   const uint32_t per_plane = (XY + kGatherSpan - 1) / kGatherSpan;
-  const int z = (int)(blockIdx.x / per_plane);
-  const uint32_t q0 = (blockIdx.x - (uint32_t)z * per_plane) * kGatherSpan;
+  const int z = (int)(blk / per_plane);
+  const uint32_t q0 = (blk - (uint32_t)z * per_plane) * kGatherSpan;
   const int tid = threadIdx.x;
   // first tile of each window (cube ids below 0 are never looked up)
   int base[2];
@@ -1671,6 +1686,7 @@ __global__ __launch_bounds__(256) void gather_kernel(ExtractParams P, const floa
   }
   __syncthreads();
   float* const gz = gvol + (int64_t)z * XY;
+  uint32_t gmax = 0;  // largest |g| bits of this thread
 #pragma unroll
   for (int j = 0; j < kGatherSpan / 256; ++j) {
     const uint32_t q = q0 + (uint32_t)(j * 256 + tid);
@@ -1693,6 +1709,20 @@ __global__ __launch_bounds__(256) void gather_kernel(ExtractParams P, const floa
       g += contrib[(s_rec[dz][dy][slot] + rank) * 8 + k];
     }
     gz[q] = g;
+    gmax = max(gmax, __float_as_uint(g) & 0x7fffffffu);
+  }
+  if (maxg) {
+    // one atomic per workgroup, only where it can raise the maximum (the
+    // plain read is a filter: a stale value only lets an atomic through)
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) gmax = max(gmax, (uint32_t)__shfl_xor((int)gmax, o));
+    if ((tid & 63) == 0) s_max[tid >> 6] = gmax;
+    __syncthreads();
+    if (tid == 0) {
+      const uint32_t m = max(max(s_max[0], s_max[1]), max(s_max[2], s_max[3]));
+      if (m > __hip_atomic_load(maxg, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+        atomicMax(maxg, m);
+    }
   }
 }
 
@@ -2145,7 +2175,8 @@ int backward_impl(const float* d_vol, int X, int Y, int Z, float h, const float*
   MCX_HIP(hipGetLastError());
   const int64_t g2 = ((int64_t)X * Y + kGatherSpan - 1) / kGatherSpan * Z;
   if (g2 >= ((int64_t)1 << 31)) return MCX_ERR_UNSUPPORTED;
-  hipLaunchKernelGGL(gather_kernel, dim3((unsigned)g2), dim3(256), 0, s, P, contrib, d_grad_vol);
+  hipLaunchKernelGGL(gather_kernel<false>, dim3((unsigned)g2), dim3(256), 0, s, P, contrib,
+                     d_grad_vol, nullptr);
   MCX_HIP(hipGetLastError());
   return MCX_OK;
 }
@@ -2171,26 +2202,14 @@ extern "C" int mcx_extract_backward_recompute(const float* d_vol, int X, int Y, 
 // A plan holds the per-volume descriptors and the K1 column list in device
 // memory (uploaded once); mcx_extract_batched then only launches kernels, so
 // it can be captured in a hipGraph and replayed.
-struct mcx_batch {
-  int nvol = 0;
-  int elem = MCX_ELEM_SHORT2;
-  int maxX = 0;
-  bool wide = true;  // every volume suits the 128-bit count kernel
-  int slab = 64;     // planes per column
-  int64_t n_tiles = 0;
-  int64_t ncols = 0;
-  int64_t cubes = 0;
-  VolDesc* d_vols = nullptr;
-  Column* d_cols = nullptr;
-};
-
 extern "C" int mcx_batch_create(mcx_batch** out, int nvol, const int* dims_xyz, const float* h,
                                 const void* const* d_vols, int elem) {
   if (!out || nvol < 1 || !dims_xyz || !h || !d_vols) return MCX_ERR_ARG;
   if (elem != MCX_ELEM_SHORT2 && elem != MCX_ELEM_F32) return MCX_ERR_ARG;
   std::vector<VolDesc> vols(nvol);
+  std::vector<BatchVol> bvol(nvol);
   std::vector<Column> cols;
-  int64_t tb = 0, cubes = 0;
+  int64_t tb = 0, cubes = 0, voxels = 0, blocks = 0, gblocks = 0;
   int maxX = 1;
   bool wide = true;
   for (int b = 0; b < nvol; ++b) {
@@ -2198,6 +2217,11 @@ extern "C" int mcx_batch_create(mcx_batch** out, int nvol, const int* dims_xyz, 
     if (X < 1 || Y < 1 || Z < 1 || !d_vols[b]) return MCX_ERR_ARG;
     if (!size_ok(X, Y, Z)) return MCX_ERR_UNSUPPORTED;
     vols[b] = make_vol(d_vols[b], X, Y, Z, h[b], nullptr, tb);
+    const int64_t nv = (int64_t)X * Y * Z;
+    bvol[b] = BatchVol{X, Y, Z, h[b], voxels, blocks, gblocks, FastDiv((uint32_t)X)};
+    voxels += nv;
+    blocks += (nv + 255) / 256;
+    gblocks += ((int64_t)X * Y + kGatherSpan - 1) / kGatherSpan * Z;
     const int64_t nt = tiles_of(X, Y, Z);
     if (nt > 0) {
       // one column per in-plane range, slabs of at most 64 planes
@@ -2221,15 +2245,23 @@ extern "C" int mcx_batch_create(mcx_batch** out, int nvol, const int* dims_xyz, 
   p->n_tiles = tb;
   p->ncols = (int64_t)cols.size();
   p->cubes = cubes;
+  p->voxels = voxels;
+  p->blocks = blocks;
+  p->gather_blocks = gblocks;
   hipError_t e = hipMalloc(reinterpret_cast<void**>(&p->d_vols), sizeof(VolDesc) * nvol);
+  if (e == hipSuccess)
+    e = hipMalloc(reinterpret_cast<void**>(&p->d_bvol), sizeof(BatchVol) * nvol);
   if (e == hipSuccess && !cols.empty())
     e = hipMalloc(reinterpret_cast<void**>(&p->d_cols), sizeof(Column) * cols.size());
   if (e == hipSuccess)
     e = hipMemcpy(p->d_vols, vols.data(), sizeof(VolDesc) * nvol, hipMemcpyHostToDevice);
+  if (e == hipSuccess)
+    e = hipMemcpy(p->d_bvol, bvol.data(), sizeof(BatchVol) * nvol, hipMemcpyHostToDevice);
   if (e == hipSuccess && !cols.empty())
     e = hipMemcpy(p->d_cols, cols.data(), sizeof(Column) * cols.size(), hipMemcpyHostToDevice);
   if (e != hipSuccess) {
     (void)hipFree(p->d_vols);
+    (void)hipFree(p->d_bvol);
     (void)hipFree(p->d_cols);
     delete p;
     return hip_status(e);
@@ -2241,9 +2273,12 @@ extern "C" int mcx_batch_create(mcx_batch** out, int nvol, const int* dims_xyz, 
 extern "C" void mcx_batch_destroy(mcx_batch* p) {
   if (!p) return;
   (void)hipFree(p->d_vols);
+  (void)hipFree(p->d_bvol);
   (void)hipFree(p->d_cols);
   delete p;
 }
+
+extern "C" int64_t mcx_batch_voxels(const mcx_batch* p) { return p ? p->voxels : 0; }
 
 extern "C" int64_t mcx_batch_tiles(const mcx_batch* p) { return p ? p->n_tiles : 0; }
 
@@ -2253,6 +2288,23 @@ extern "C" size_t mcx_batch_workspace_size(const mcx_batch* p, int64_t records) 
   return fixed_ws_bytes(p->n_tiles) + rec_area_bytes(records);
 }
 
+namespace {
+// Launch parameters of a batched call: workspace layout and the plan's tables.
+int prepare_batch(ExtractParams& P, const mcx_batch* plan, void* d_ws, size_t ws_bytes) {
+  int rc = layout_ws(d_ws, ws_bytes, plan->n_tiles, P);
+  if (rc) return rc;
+  P.vols = plan->d_vols;
+  P.bvol = plan->d_bvol;
+  P.cols = plan->d_cols;
+  P.nvol = plan->nvol;
+  P.n_tiles = plan->n_tiles;
+  P.ncols = plan->ncols;
+  P.slab_max = plan->slab;
+  P.wide = plan->wide;
+  return MCX_OK;
+}
+}  // namespace
+
 extern "C" int mcx_extract_batched(const mcx_batch* plan, mcx_vertex_voxel* d_vv, float* d_pos,
                                    int32_t* d_v2vox, int64_t cap, int64_t* d_vbase,
                                    int64_t* d_count, int64_t* h_count, void* d_ws,
@@ -2261,15 +2313,8 @@ extern "C" int mcx_extract_batched(const mcx_batch* plan, mcx_vertex_voxel* d_vv
   if (plan->elem == MCX_ELEM_F32 && d_vv) return MCX_ERR_ARG;
   hipStream_t s = as_stream(stream);
   ExtractParams P{};
-  int rc = layout_ws(d_ws, ws_bytes, plan->n_tiles, P);
+  int rc = prepare_batch(P, plan, d_ws, ws_bytes);
   if (rc) return rc;
-  P.vols = plan->d_vols;
-  P.cols = plan->d_cols;
-  P.nvol = plan->nvol;
-  P.n_tiles = plan->n_tiles;
-  P.ncols = plan->ncols;
-  P.slab_max = plan->slab;
-  P.wide = plan->wide;
   P.vv = d_vv;
   P.pos = d_pos;
   P.v2vox = d_v2vox;
@@ -2297,6 +2342,85 @@ extern "C" int mcx_extract_batched(const mcx_batch* plan, mcx_vertex_voxel* d_vv
     if (hh.error) return take_error(d_ws, s);
     if (hh.count > cap) return MCX_ERR_CAPACITY;
   }
+  return MCX_OK;
+}
+
+// ---- batched differentiable extraction (no counterpart in the reference) -
+extern "C" int mcx_extract_diff_batched(const mcx_batch* plan, float* d_pos, int64_t cap,
+                                        int64_t* d_vbase, int64_t* d_count, int64_t* h_count,
+                                        void* d_ws, size_t ws_bytes, void* stream) {
+  if (!plan || !d_vbase || cap < 0) return MCX_ERR_ARG;
+  if (plan->elem != MCX_ELEM_F32) return MCX_ERR_ARG;
+  hipStream_t s = as_stream(stream);
+  ExtractParams P{};
+  int rc = prepare_batch(P, plan, d_ws, ws_bytes);
+  if (rc) return rc;
+  P.pos = d_pos;
+  P.cap = cap;
+  P.d_count = d_count;
+  P.vbase = d_vbase;
+  P.write_masks = true;
+  if (plan->n_tiles == 0) {
+    rc = fill_bytes_zero(d_vbase, sizeof(int64_t) * (plan->nvol + 1), s);
+    if (!rc && d_count) rc = fill_bytes_zero(d_count, sizeof(int64_t), s);
+    if (rc) return rc;
+    if (h_count) {
+      MCX_HIP(hipStreamSynchronize(s));
+      *h_count = 0;
+    }
+    return MCX_OK;
+  }
+  rc = run_extract(P, MCX_ELEM_F32, true, plan->maxX, s);
+  if (rc) return rc;
+  if (h_count) {
+    WsHeader hh;
+    rc = read_header(d_ws, &hh, s);
+    if (rc) return rc;
+    *h_count = hh.count;
+    if (hh.error) return take_error(d_ws, s);
+    if (hh.dense_last) return MCX_ERR_WORKSPACE;  // the backward needs every record
+    if (hh.count > cap) return MCX_ERR_CAPACITY;
+  }
+  return MCX_OK;
+}
+
+extern "C" size_t mcx_batch_backward_scratch_size(const mcx_batch* plan, size_t ws_bytes) {
+  if (!plan || ws_bytes < fixed_ws_bytes(plan->n_tiles)) return 0;
+  ExtractParams P{};
+  char dummy;
+  if (layout_ws(&dummy, ws_bytes, plan->n_tiles, P)) return 0;
+  return (size_t)P.rec_cap * 8 * sizeof(float);
+}
+
+extern "C" int mcx_extract_backward_batched(const mcx_batch* plan, const float* d_grad_pos,
+                                            float* d_grad_vol, uint32_t* d_maxabs, void* d_ws,
+                                            size_t ws_bytes, void* d_scratch,
+                                            size_t scratch_bytes, void* stream) {
+  if (!plan || !d_grad_vol) return MCX_ERR_ARG;
+  if (plan->elem != MCX_ELEM_F32) return MCX_ERR_ARG;
+  hipStream_t s = as_stream(stream);
+  int rc = MCX_OK;
+  if (d_maxabs) {
+    rc = fill_u32(d_maxabs, 0u, 1, s);
+    if (rc) return rc;
+  }
+  if (plan->n_tiles == 0)
+    return fill_bytes_zero(d_grad_vol, (size_t)plan->voxels * sizeof(float), s);
+  if (!d_grad_pos) return MCX_ERR_ARG;
+  ExtractParams P{};
+  rc = prepare_batch(P, plan, d_ws, ws_bytes);
+  if (rc) return rc;
+  if (!d_scratch || scratch_bytes < (size_t)P.rec_cap * 8 * sizeof(float)) return MCX_ERR_WORKSPACE;
+  if (plan->gather_blocks >= ((int64_t)1 << 31)) return MCX_ERR_UNSUPPORTED;
+  float* contrib = static_cast<float*>(d_scratch);
+  const unsigned g1 =
+      (unsigned)((plan->n_tiles + kTilesPerContribGroup - 1) / kTilesPerContribGroup);
+  hipLaunchKernelGGL(contrib_kernel<true>, dim3(g1), dim3(kEmitThreads), 0, s, P, d_grad_pos,
+                     contrib);
+  MCX_HIP(hipGetLastError());
+  hipLaunchKernelGGL(gather_kernel<true>, dim3((unsigned)plan->gather_blocks), dim3(256), 0, s, P,
+                     contrib, d_grad_vol, d_maxabs);
+  MCX_HIP(hipGetLastError());
   return MCX_OK;
 }
 

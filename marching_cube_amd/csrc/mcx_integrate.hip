@@ -18,9 +18,14 @@
 //   skip q.z < FLT_MIN; u = rint(fx q.x / q.z + cx), v likewise (round half even)
 //   skip u,v outside [0, W-2) x [0, H-2)                 (reference :449)
 //   zval = depth * scale * sqrt(((u-cx)/fx)^2 + ((v-cy)/fy)^2 + 1)  (ray distance)
-//   skip zval <= FLT_MIN (dropout); sdf = zval - |q|; skip sdf < -trunc
-//   tsdf = clamp(sdf / trunc, -1, 1); running average with integer weights
-//   capped at max_w; tsdf stored as trunc(t * 32767) clamped to +-32767.
+//   skip zval <= FLT_MIN (dropout); sdf = zval - |q|; skip sdf < -trunc_neg
+//   tsdf = sdf > 0 ? min(sdf / trunc, 1) : sdf / trunc_neg (:455-463, min_t
+//   and max_t of the volume; one truncation gives clamp(sdf / trunc, -1, 1));
+//   running average with integer weights capped at max_w; tsdf stored as
+//   trunc(t * 32767) clamped to +-32767.
+// A workgroup first marks the frames that none of its voxels can observe
+// (frame_skip_flags) and passes over them; the per-voxel arithmetic and its
+// results are unchanged.
 #include <float.h>
 #include <hip/hip_runtime.h>
 
@@ -54,7 +59,8 @@ __global__ __launch_bounds__(256) void rayfac_kernel(float* out, int W, int H, f
 struct IntegrateParams {
   uint32_t* vol;
   int X, Y, Z;
-  float h, trunc;
+  float h, trunc;        // trunc: the positive side (max_t)
+  float trunc_neg;       // the negative side (min_t) of the packed integration
   int max_w;
   float fx, fy, cx, cy;
   const float* poses;      // [K][12] device
@@ -65,25 +71,93 @@ struct IntegrateParams {
   FastDiv divX, divY;
 };
 
-template <typename DT>
-__global__ __launch_bounds__(256) void integrate_kernel(IntegrateParams P) {
-  __shared__ float s_pose[kFramesPerPass * 12];
-  for (int i = threadIdx.x; i < P.K * 12; i += blockDim.x) s_pose[i] = P.poses[i];
+// Frames that no voxel of a workgroup can observe, as bits of s_skip (one per
+// frame).  The workgroup's voxels, linear indices [i0, i1] of one X x Y x Z
+// volume, lie in a box of voxel centres; frame k is skipped when the whole
+// box is behind its camera (every corner at camera depth below -1e-4 m, so
+// every voxel's fp32 q.z is below FLT_MIN), or in front of it (every corner
+// deeper than 1 cm) and projecting past one edge of the pixel window by more
+// than a pixel (plus 1e-5 of the coordinate): the image of a convex box in
+// front of the camera is the convex hull of its corners' images.  Corners are
+// projected in double precision from the voxels' own fp32 centres; a voxel's
+// fp32 projection differs from the exact one by far less than the margin, so
+// every voxel of a skipped frame would have taken its q.z or window skip.
+// Called by every thread of the workgroup (two barriers inside).
+__device__ void frame_skip_flags(const float* s_pose, int K, const IntegrateParams& P,
+                                 int64_t i0, int64_t i1, uint32_t* s_skip) {
+  const int64_t XY = (int64_t)P.X * P.Y;
+  const int64_t z0 = i0 / XY, z1 = i1 / XY;
+  int64_t x0 = 0, x1 = P.X - 1, y0 = 0, y1 = P.Y - 1;
+  if (z0 == z1) {
+    const int64_t r0 = i0 - z0 * XY, r1 = i1 - z1 * XY;
+    y0 = r0 / P.X;
+    y1 = r1 / P.X;
+    if (y0 == y1) {
+      x0 = r0 - y0 * P.X;
+      x1 = r1 - y1 * P.X;
+    }
+  }
+  const double cx[2] = {(double)(((float)x0 + 0.5f) * P.h), (double)(((float)x1 + 0.5f) * P.h)};
+  const double cy[2] = {(double)(((float)y0 + 0.5f) * P.h), (double)(((float)y1 + 0.5f) * P.h)};
+  const double cz[2] = {(double)(((float)z0 + 0.5f) * P.h), (double)(((float)z1 + 0.5f) * P.h)};
+  for (int k0 = 0; k0 < K; k0 += blockDim.x) {
+    const int k = k0 + (int)threadIdx.x;
+    bool skip = false;
+    if (k < K) {
+      const float* R = s_pose + 12 * k;
+      double qzmin = 1e300, qzmax = -1e300, umin = 1e300, umax = -1e300, vmin = 1e300,
+             vmax = -1e300;
+#pragma unroll
+      for (int c = 0; c < 8; ++c) {
+        const double dx = cx[c & 1] - R[9], dy = cy[(c >> 1) & 1] - R[10], dz = cz[c >> 2] - R[11];
+        const double qx = (double)R[0] * dx + (double)R[3] * dy + (double)R[6] * dz;
+        const double qy = (double)R[1] * dx + (double)R[4] * dy + (double)R[7] * dz;
+        const double qz = (double)R[2] * dx + (double)R[5] * dy + (double)R[8] * dz;
+        qzmin = fmin(qzmin, qz);
+        qzmax = fmax(qzmax, qz);
+        const double u = (double)P.fx * (qx / qz) + (double)P.cx;
+        const double v = (double)P.fy * (qy / qz) + (double)P.cy;
+        umin = fmin(umin, u);
+        umax = fmax(umax, u);
+        vmin = fmin(vmin, v);
+        vmax = fmax(vmax, v);
+      }
+      if (qzmax < -1e-4) {
+        skip = true;
+      } else if (qzmin > 0.01) {
+        const double tol = 1.0 + 1e-5 * fmax(fmax(fabs(umin), fabs(umax)), fmax(fabs(vmin), fabs(vmax)));
+        // a voxel is observed only at rint(u) in [0, W - 2), rint(v) in [0, H - 2)
+        skip = umax < -0.5 - tol || umin > (double)P.W - 2.5 + tol || vmax < -0.5 - tol ||
+               vmin > (double)P.H - 2.5 + tol;
+      }
+    }
+    const uint64_t m = __ballot(skip);
+    if ((threadIdx.x & 63) == 0 && k0 + (int)threadIdx.x < K) {
+      const int w = (k0 + (int)threadIdx.x) >> 5;
+      s_skip[w] = (uint32_t)m;
+      if (k0 + (int)threadIdx.x + 32 < K) s_skip[w + 1] = (uint32_t)(m >> 32);
+    }
+  }
   __syncthreads();
-  const int64_t n = (int64_t)P.X * P.Y * P.Z;
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const uint32_t xy = (uint32_t)((int64_t)P.X * P.Y);
-  const int z = (int)(i / xy);
-  const uint32_t rem = (uint32_t)(i - (int64_t)z * xy);
-  const uint32_t y = P.divX.div(rem);
-  const int x = (int)(rem - y * (uint32_t)P.X);
-  const float px = ((float)x + 0.5f) * P.h;
-  const float py = ((float)(int)y + 0.5f) * P.h;
-  const float pz = ((float)z + 0.5f) * P.h;
-  uint32_t cur = P.vol[i];
+}
+
+__device__ __forceinline__ bool skipped(const uint32_t* s_skip, int k) {
+  return (s_skip[k >> 5] >> (k & 31)) & 1u;
+}
+
+// Packed short2 update of voxel i by frames [0, K) (the oracle's
+// update_voxel).  The voxel is read at its first update and written only if
+// some frame updated it: outside every frustum and behind every surface it
+// costs no memory traffic.
+template <typename DT>
+__device__ __forceinline__ void integrate_voxel(const IntegrateParams& P, const float* s_pose,
+                                                const uint32_t* s_skip, float px, float py,
+                                                float pz, int64_t i) {
   const int64_t frame = (int64_t)P.W * P.H;
+  uint32_t cur = 0;
+  bool have = false;
   for (int k = 0; k < P.K; ++k) {
+    if (skipped(s_skip, k)) continue;
     const float* R = s_pose + 12 * k;
     const float dx = px - R[9], dy = py - R[10], dz = pz - R[11];
     const float qx = __fmaf_rn(R[6], dz, __fmaf_rn(R[3], dy, R[0] * dx));
@@ -102,9 +176,12 @@ __global__ __launch_bounds__(256) void integrate_kernel(IntegrateParams P) {
     if (!(zval > FLT_MIN)) continue;
     const float qn = sqrtf(__fmaf_rn(qz, qz, __fmaf_rn(qy, qy, qx * qx)));
     const float sdf = zval - qn;
-    if (!(sdf >= -P.trunc)) continue;
-    float tsdf = sdf / P.trunc;
-    tsdf = fminf(fmaxf(tsdf, -1.0f), 1.0f);
+    if (!(sdf >= -P.trunc_neg)) continue;
+    const float tsdf = sdf > 0.0f ? fminf(sdf / P.trunc, 1.0f) : sdf / P.trunc_neg;
+    if (!have) {
+      cur = P.vol[i];
+      have = true;
+    }
     const int16_t ti = (int16_t)(cur & 0xffffu);
     const int wi = (int)(cur >> 16);
     const float told = (float)ti / 32767.0f;
@@ -114,7 +191,29 @@ __global__ __launch_bounds__(256) void integrate_kernel(IntegrateParams P) {
     fixedp = fixedp > 32767 ? 32767 : (fixedp < -32767 ? -32767 : fixedp);
     cur = (uint32_t)(uint16_t)(int16_t)fixedp | ((uint32_t)wnew << 16);
   }
-  P.vol[i] = cur;
+  if (have) P.vol[i] = cur;
+}
+
+template <typename DT>
+__global__ __launch_bounds__(256) void integrate_kernel(IntegrateParams P) {
+  __shared__ float s_pose[kFramesPerPass * 12];
+  __shared__ uint32_t s_skip[kFramesPerPass / 32];
+  for (int i = threadIdx.x; i < P.K * 12; i += blockDim.x) s_pose[i] = P.poses[i];
+  __syncthreads();
+  const int64_t n = (int64_t)P.X * P.Y * P.Z;
+  const int64_t ib = (int64_t)blockIdx.x * blockDim.x;
+  frame_skip_flags(s_pose, P.K, P, ib, min(ib + (int64_t)blockDim.x, n) - 1, s_skip);
+  const int64_t i = ib + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t xy = (uint32_t)((int64_t)P.X * P.Y);
+  const int z = (int)(i / xy);
+  const uint32_t rem = (uint32_t)(i - (int64_t)z * xy);
+  const uint32_t y = P.divX.div(rem);
+  const int x = (int)(rem - y * (uint32_t)P.X);
+  const float px = ((float)x + 0.5f) * P.h;
+  const float py = ((float)(int)y + 0.5f) * P.h;
+  const float pz = ((float)z + 0.5f) * P.h;
+  integrate_voxel<DT>(P, s_pose, s_skip, px, py, pz, i);
 }
 
 // ---- fp32 TSDF volumes (differentiable form; no counterpart in the reference)
@@ -153,33 +252,64 @@ __device__ __forceinline__ bool observe(const IntegrateParams& P, const float* R
   return true;
 }
 
-__global__ __launch_bounds__(256) void integrate_tsdf_kernel(IntegrateParams P, float* tsdf,
-                                                             int32_t* wvol) {
-  __shared__ float s_pose[kFramesPerPass * 12];
-  for (int i = threadIdx.x; i < P.K * 12; i += blockDim.x) s_pose[i] = P.poses[i];
-  __syncthreads();
-  const int64_t n = (int64_t)P.X * P.Y * P.Z;
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
+// fp32 update of one voxel by frames [0, K).  With tsdf / wvol, (f, w) is
+// read from them at the first update (none: no memory traffic) and *changed
+// says whether any frame updated it; without, (f, w) holds the start value.
+__device__ __forceinline__ void integrate_tsdf_voxel(const IntegrateParams& P, const float* s_pose,
+                                                     const uint32_t* s_skip, float px, float py,
+                                                     float pz, float& f, int& w,
+                                                     const float* tsdf, const int32_t* wvol,
+                                                     bool* changed) {
+  bool have = tsdf == nullptr;
+  for (int k = 0; k < P.K; ++k) {
+    if (skipped(s_skip, k)) continue;
+    int pix;
+    float t, dtdd;
+    if (!observe<float>(P, s_pose + 12 * k, k, px, py, pz, &pix, &t, &dtdd)) continue;
+    if (!have) {
+      f = *tsdf;
+      w = *wvol;
+      have = true;
+    }
+    *changed = true;
+    f = __fmaf_rn(f, (float)w, t) / (float)(w + 1);
+    w = w + 1 < P.max_w ? w + 1 : P.max_w;
+  }
+}
+
+__device__ __forceinline__ void voxel_centre(const IntegrateParams& P, int64_t i, float* px,
+                                             float* py, float* pz) {
   const uint32_t xy = (uint32_t)((int64_t)P.X * P.Y);
   const int z = (int)(i / xy);
   const uint32_t rem = (uint32_t)(i - (int64_t)z * xy);
   const uint32_t y = P.divX.div(rem);
   const int x = (int)(rem - y * (uint32_t)P.X);
-  const float px = ((float)x + 0.5f) * P.h;
-  const float py = ((float)(int)y + 0.5f) * P.h;
-  const float pz = ((float)z + 0.5f) * P.h;
-  float f = tsdf[i];
-  int w = wvol[i];
-  for (int k = 0; k < P.K; ++k) {
-    int pix;
-    float t, dtdd;
-    if (!observe<float>(P, s_pose + 12 * k, k, px, py, pz, &pix, &t, &dtdd)) continue;
-    f = __fmaf_rn(f, (float)w, t) / (float)(w + 1);
-    w = w + 1 < P.max_w ? w + 1 : P.max_w;
+  *px = ((float)x + 0.5f) * P.h;
+  *py = ((float)(int)y + 0.5f) * P.h;
+  *pz = ((float)z + 0.5f) * P.h;
+}
+
+__global__ __launch_bounds__(256) void integrate_tsdf_kernel(IntegrateParams P, float* tsdf,
+                                                             int32_t* wvol) {
+  __shared__ float s_pose[kFramesPerPass * 12];
+  __shared__ uint32_t s_skip[kFramesPerPass / 32];
+  for (int i = threadIdx.x; i < P.K * 12; i += blockDim.x) s_pose[i] = P.poses[i];
+  __syncthreads();
+  const int64_t n = (int64_t)P.X * P.Y * P.Z;
+  const int64_t ib = (int64_t)blockIdx.x * blockDim.x;
+  frame_skip_flags(s_pose, P.K, P, ib, min(ib + (int64_t)blockDim.x, n) - 1, s_skip);
+  const int64_t i = ib + threadIdx.x;
+  if (i >= n) return;
+  float px, py, pz;
+  voxel_centre(P, i, &px, &py, &pz);
+  float f = 0.0f;
+  int w = 0;
+  bool changed = false;
+  integrate_tsdf_voxel(P, s_pose, s_skip, px, py, pz, f, w, tsdf + i, wvol + i, &changed);
+  if (changed) {
+    tsdf[i] = f;
+    wvol[i] = w;
   }
-  tsdf[i] = f;
-  wvol[i] = w;
 }
 
 // Deterministic max of |g| over n floats: per-block max, then an integer
@@ -205,37 +335,22 @@ __device__ __forceinline__ double grad_scale(uint32_t maxg_bits, float max_dtdd)
   return bound > 0.0 ? 4398046511104.0 / bound : 1.0;  // 2^42 / bound
 }
 
-// Backward of integrate_tsdf_kernel for the K frames of one call: for every
-// voxel with dL/df != 0, recompute its observations; observation i (weight
-// before it wb_i = min(w0 + i, max_w)) has df/dt_i = 1/(wb_i + 1) * prod over
-// later observations j of wb_j / (wb_j + 1) (closed form: the uncapped part
+// Backward of the fp32 update of one voxel for the K frames of one call:
+// recompute its observations; observation i (weight before it wb_i =
+// min(w0 + i, max_w)) has df/dt_i = 1/(wb_i + 1) * prod over later
+// observations j of wb_j / (wb_j + 1) (closed form: the uncapped part
 // telescopes, the capped part is a power of max_w / (max_w + 1)); the
-// contribution g * df/dt_i * dt_i/dd goes to the observed pixel as an int64
-// fixed-point atomic add (integer sums do not depend on the order).
-__global__ __launch_bounds__(256) void integrate_tsdf_backward_kernel(
-    IntegrateParams P, const float* __restrict__ g, const int32_t* __restrict__ w0vol,
-    const uint32_t* maxg, float max_dtdd, unsigned long long* acc) {
-  __shared__ float s_pose[kFramesPerPass * 12];
-  for (int i = threadIdx.x; i < P.K * 12; i += blockDim.x) s_pose[i] = P.poses[i];
-  __syncthreads();
-  const int64_t n = (int64_t)P.X * P.Y * P.Z;
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const float gi = g[i];
-  if (gi == 0.0f) return;
-  const double scale = grad_scale(*maxg, max_dtdd);
-  const uint32_t xy = (uint32_t)((int64_t)P.X * P.Y);
-  const int z = (int)(i / xy);
-  const uint32_t rem = (uint32_t)(i - (int64_t)z * xy);
-  const uint32_t y = P.divX.div(rem);
-  const int x = (int)(rem - y * (uint32_t)P.X);
-  const float px = ((float)x + 0.5f) * P.h;
-  const float py = ((float)(int)y + 0.5f) * P.h;
-  const float pz = ((float)z + 0.5f) * P.h;
+// contribution g * df/dt_i * dt_i/dd goes to the observed pixel of acc (K
+// frames of W x H) as an int64 fixed-point atomic add (integer sums do not
+// depend on the order).
+__device__ __forceinline__ void backward_voxel(const IntegrateParams& P, const float* s_pose,
+                                               const uint32_t* s_skip, float px, float py,
+                                               float pz, float gi, int w0, double scale,
+                                               unsigned long long* acc) {
   const int M = P.max_w;
-  const int w0 = w0vol ? w0vol[i] : 0;
   int nobs = 0;
   for (int k = 0; k < P.K; ++k) {
+    if (skipped(s_skip, k)) continue;
     int pix;
     float t, dtdd;
     if (observe<float>(P, s_pose + 12 * k, k, px, py, pz, &pix, &t, &dtdd)) ++nobs;
@@ -244,6 +359,7 @@ __global__ __launch_bounds__(256) void integrate_tsdf_backward_kernel(
   const double cap_a = (double)M / (double)(M + 1);
   int o = 0;
   for (int k = 0; k < P.K; ++k) {
+    if (skipped(s_skip, k)) continue;
     int pix;
     float t, dtdd;
     if (!observe<float>(P, s_pose + 12 * k, k, px, py, pz, &pix, &t, &dtdd)) continue;
@@ -261,6 +377,109 @@ __global__ __launch_bounds__(256) void integrate_tsdf_backward_kernel(
     }
     ++o;
   }
+}
+
+__global__ __launch_bounds__(256) void integrate_tsdf_backward_kernel(
+    IntegrateParams P, const float* __restrict__ g, const int32_t* __restrict__ w0vol,
+    const uint32_t* maxg, float max_dtdd, unsigned long long* acc) {
+  __shared__ float s_pose[kFramesPerPass * 12];
+  __shared__ uint32_t s_skip[kFramesPerPass / 32];
+  for (int i = threadIdx.x; i < P.K * 12; i += blockDim.x) s_pose[i] = P.poses[i];
+  __syncthreads();
+  const int64_t n = (int64_t)P.X * P.Y * P.Z;
+  const int64_t ib = (int64_t)blockIdx.x * blockDim.x;
+  frame_skip_flags(s_pose, P.K, P, ib, min(ib + (int64_t)blockDim.x, n) - 1, s_skip);
+  const int64_t i = ib + threadIdx.x;
+  if (i >= n) return;
+  const float gi = g[i];
+  if (gi == 0.0f) return;
+  float px, py, pz;
+  voxel_centre(P, i, &px, &py, &pz);
+  backward_voxel(P, s_pose, s_skip, px, py, pz, gi, w0vol ? w0vol[i] : 0,
+                 grad_scale(*maxg, max_dtdd), acc);
+}
+
+// ---- batches of fp32 volumes (the per-volume layout of a batch plan:
+// concatenated per-voxel arrays, K frames of W x H and K poses per volume,
+// one set of intrinsics); one workgroup = 256 voxels of one volume.
+struct BatchIntegrateParams {
+  IntegrateParams base;  // intrinsics, truncation, frames; depth / poses of volume 0
+  const BatchVol* bvol;
+  int nvol;
+  int from_init;         // start every voxel from (init_value, 0) without reading it
+  float init_value;
+};
+
+// This workgroup's volume, its launch parameters and its poses in LDS.
+__device__ __forceinline__ int batch_setup(const BatchIntegrateParams& B, IntegrateParams& P,
+                                           float* s_pose, int64_t* vox_begin, int64_t* i0) {
+  const int vi = batch_volume_of<&BatchVol::blk_begin>(B.bvol, B.nvol, blockIdx.x);
+  const BatchVol& V = B.bvol[vi];
+  P = B.base;
+  P.X = V.X;
+  P.Y = V.Y;
+  P.Z = V.Z;
+  P.h = V.h;
+  P.divX = V.divX;
+  const int64_t frames = (int64_t)P.K * P.W * P.H;
+  P.depth = static_cast<const float*>(B.base.depth) + (int64_t)vi * frames;
+  const float* poses = B.base.poses + (int64_t)vi * P.K * 12;
+  for (int i = threadIdx.x; i < P.K * 12; i += blockDim.x) s_pose[i] = poses[i];
+  *vox_begin = V.vox_begin;
+  *i0 = (blockIdx.x - V.blk_begin) * 256;
+  __syncthreads();
+  return vi;
+}
+
+__global__ __launch_bounds__(256) void integrate_tsdf_batched_kernel(BatchIntegrateParams B,
+                                                                     float* tsdf, int32_t* wvol) {
+  __shared__ float s_pose[kFramesPerPass * 12];
+  __shared__ uint32_t s_skip[kFramesPerPass / 32];
+  IntegrateParams P;
+  int64_t vb, ib;
+  batch_setup(B, P, s_pose, &vb, &ib);
+  const int64_t n = (int64_t)P.X * P.Y * P.Z;
+  frame_skip_flags(s_pose, P.K, P, ib, min(ib + 256, n) - 1, s_skip);
+  const int64_t i = ib + threadIdx.x;
+  if (i >= n) return;
+  float px, py, pz;
+  voxel_centre(P, i, &px, &py, &pz);
+  float f = B.init_value;
+  int w = 0;
+  bool changed = false;
+  if (B.from_init) {
+    // every voxel is written: the start value where no frame observes it
+    integrate_tsdf_voxel(P, s_pose, s_skip, px, py, pz, f, w, nullptr, nullptr, &changed);
+    tsdf[vb + i] = f;
+    wvol[vb + i] = w;
+  } else {
+    integrate_tsdf_voxel(P, s_pose, s_skip, px, py, pz, f, w, tsdf + vb + i, wvol + vb + i,
+                         &changed);
+    if (changed) {
+      tsdf[vb + i] = f;
+      wvol[vb + i] = w;
+    }
+  }
+}
+
+__global__ __launch_bounds__(256) void integrate_tsdf_backward_batched_kernel(
+    BatchIntegrateParams B, const float* __restrict__ g, const int32_t* __restrict__ w0vol,
+    const uint32_t* maxg, float max_dtdd, unsigned long long* acc) {
+  __shared__ float s_pose[kFramesPerPass * 12];
+  __shared__ uint32_t s_skip[kFramesPerPass / 32];
+  IntegrateParams P;
+  int64_t vb, ib;
+  const int vi = batch_setup(B, P, s_pose, &vb, &ib);
+  const int64_t n = (int64_t)P.X * P.Y * P.Z;
+  frame_skip_flags(s_pose, P.K, P, ib, min(ib + 256, n) - 1, s_skip);
+  const int64_t i = ib + threadIdx.x;
+  if (i >= n) return;
+  const float gi = g[vb + i];
+  if (gi == 0.0f) return;
+  float px, py, pz;
+  voxel_centre(P, i, &px, &py, &pz);
+  backward_voxel(P, s_pose, s_skip, px, py, pz, gi, w0vol ? w0vol[vb + i] : 0,
+                 grad_scale(*maxg, max_dtdd), acc + (int64_t)vi * P.K * P.W * P.H);
 }
 
 __global__ __launch_bounds__(256) void fixed_to_float_kernel(const unsigned long long* acc,
@@ -403,6 +622,7 @@ int prepare_integrate(IntegrateParams& P, int X, int Y, int Z, float h, float tr
   P.Z = Z;
   P.h = h;
   P.trunc = trunc;
+  P.trunc_neg = trunc;
   P.max_w = max_w;
   P.fx = intr[0];
   P.fy = intr[1];
@@ -421,10 +641,11 @@ int prepare_integrate(IntegrateParams& P, int X, int Y, int Z, float h, float tr
 }
 
 template <typename DT>
-int integrate_impl(void* d_vol, int X, int Y, int Z, float h, float trunc, int max_w,
-                   const float* intr, const float* d_poses, const DT* d_depth, float depth_scale,
-                   int W, int H, int K, void* d_ws, size_t ws_bytes, void* stream) {
-  if (!d_vol) return MCX_ERR_ARG;
+int integrate_impl(void* d_vol, int X, int Y, int Z, float h, float trunc_neg, float trunc,
+                   int max_w, const float* intr, const float* d_poses, const DT* d_depth,
+                   float depth_scale, int W, int H, int K, void* d_ws, size_t ws_bytes,
+                   void* stream) {
+  if (!d_vol || !(trunc_neg > 0.f)) return MCX_ERR_ARG;
   if (K > 0 && (!d_poses || !d_depth)) return MCX_ERR_ARG;
   if (!d_ws || ws_bytes < mcx_integrate_workspace_size(W, H)) return MCX_ERR_WORKSPACE;
   if (K == 0) return MCX_OK;
@@ -434,6 +655,7 @@ int integrate_impl(void* d_vol, int X, int Y, int Z, float h, float trunc, int m
                              static_cast<float*>(d_ws), s);
   if (rc) return rc;
   P.vol = static_cast<uint32_t*>(d_vol);
+  P.trunc_neg = trunc_neg;
   const int64_t n = (int64_t)X * Y * Z;
   for (int k0 = 0; k0 < K; k0 += kFramesPerPass) {
     P.K = K - k0 < kFramesPerPass ? K - k0 : kFramesPerPass;
@@ -464,7 +686,7 @@ extern "C" int mcx_integrate(void* d_vol, int X, int Y, int Z, float h, float tr
                              const float* intr, const float* d_poses, const uint16_t* d_depth,
                              float depth_scale, int W, int H, int K, void* d_ws, size_t ws_bytes,
                              void* stream) {
-  return integrate_impl<uint16_t>(d_vol, X, Y, Z, h, trunc, max_w, intr, d_poses, d_depth,
+  return integrate_impl<uint16_t>(d_vol, X, Y, Z, h, trunc, trunc, max_w, intr, d_poses, d_depth,
                                   depth_scale, W, H, K, d_ws, ws_bytes, stream);
 }
 
@@ -472,8 +694,24 @@ extern "C" int mcx_integrate_f32(void* d_vol, int X, int Y, int Z, float h, floa
                                  int max_w, const float* intr, const float* d_poses,
                                  const float* d_depth, float depth_scale, int W, int H, int K,
                                  void* d_ws, size_t ws_bytes, void* stream) {
-  return integrate_impl<float>(d_vol, X, Y, Z, h, trunc, max_w, intr, d_poses, d_depth,
+  return integrate_impl<float>(d_vol, X, Y, Z, h, trunc, trunc, max_w, intr, d_poses, d_depth,
                                depth_scale, W, H, K, d_ws, ws_bytes, stream);
+}
+
+extern "C" int mcx_integrate_ex(void* d_vol, int X, int Y, int Z, float h, float trunc_neg,
+                                float trunc_pos, int max_w, const float* intr,
+                                const float* d_poses, const void* d_depth, int depth_kind,
+                                float depth_scale, int W, int H, int K, void* d_ws,
+                                size_t ws_bytes, void* stream) {
+  if (depth_kind == MCX_DEPTH_U16)
+    return integrate_impl<uint16_t>(d_vol, X, Y, Z, h, trunc_neg, trunc_pos, max_w, intr,
+                                    d_poses, static_cast<const uint16_t*>(d_depth), depth_scale,
+                                    W, H, K, d_ws, ws_bytes, stream);
+  if (depth_kind == MCX_DEPTH_F32)
+    return integrate_impl<float>(d_vol, X, Y, Z, h, trunc_neg, trunc_pos, max_w, intr, d_poses,
+                                 static_cast<const float*>(d_depth), depth_scale, W, H, K, d_ws,
+                                 ws_bytes, stream);
+  return MCX_ERR_ARG;
 }
 
 extern "C" int mcx_synth_depth(uint16_t* d_depth, int W, int H, int K, const float* intr,
@@ -574,6 +812,108 @@ extern "C" int mcx_integrate_tsdf_backward(const float* d_grad_tsdf, const int32
   MCX_HIP(hipGetLastError());
   hipLaunchKernelGGL(fixed_to_float_kernel, dim3(blocks_for(npix)), dim3(256), 0, s, acc, npix,
                      maxg, mdt, d_grad_depth);
+  MCX_HIP(hipGetLastError());
+  return MCX_OK;
+}
+
+// ---- batched fp32 fusion (no counterpart in the reference) ----------------
+extern "C" int mcx_integrate_tsdf_batched(const mcx_batch* plan, float* d_tsdf, int32_t* d_weight,
+                                          int from_init, float init_value, float trunc,
+                                          int max_w, const float* intr, const float* d_poses,
+                                          const float* d_depth, float depth_scale, int W, int H,
+                                          int K, void* d_ws, size_t ws_bytes, void* stream) {
+  if (!plan || !d_tsdf || !d_weight) return MCX_ERR_ARG;
+  if (plan->elem != MCX_ELEM_F32) return MCX_ERR_ARG;
+  if (K > 0 && (!d_poses || !d_depth)) return MCX_ERR_ARG;
+  if (K > kFramesPerPass) return MCX_ERR_UNSUPPORTED;
+  if (!d_ws || ws_bytes < mcx_integrate_workspace_size(W, H)) return MCX_ERR_WORKSPACE;
+  hipStream_t s = as_stream(stream);
+  if (K == 0) {
+    if (!from_init) return MCX_OK;
+    hipLaunchKernelGGL(fill_f32_i32_kernel, dim3(blocks_for(plan->voxels) < 65536u
+                                                      ? blocks_for(plan->voxels) : 65536u),
+                       dim3(256), 0, s, d_tsdf, d_weight, plan->voxels, init_value, 0);
+    MCX_HIP(hipGetLastError());
+    return MCX_OK;
+  }
+  if (plan->blocks >= ((int64_t)1 << 31)) return MCX_ERR_UNSUPPORTED;
+  BatchIntegrateParams B;
+  int rc = prepare_integrate(B.base, 1, 1, 1, 1.0f, trunc, max_w, intr, depth_scale, W, H, K,
+                             static_cast<float*>(d_ws), s);
+  if (rc) return rc;
+  B.base.K = K;
+  B.base.poses = d_poses;
+  B.base.depth = d_depth;
+  B.bvol = plan->d_bvol;
+  B.nvol = plan->nvol;
+  B.from_init = from_init ? 1 : 0;
+  B.init_value = init_value;
+  hipLaunchKernelGGL(integrate_tsdf_batched_kernel, dim3((unsigned)plan->blocks), dim3(256), 0, s,
+                     B, d_tsdf, d_weight);
+  MCX_HIP(hipGetLastError());
+  return MCX_OK;
+}
+
+extern "C" size_t mcx_integrate_tsdf_backward_batched_workspace_size(const mcx_batch* plan, int W,
+                                                                     int H, int K) {
+  if (!plan || W < 1 || H < 1 || K < 0) return 0;
+  return al256((size_t)W * H * sizeof(float)) + 256 + (size_t)plan->nvol * K * W * H * 8;
+}
+
+extern "C" int mcx_integrate_tsdf_backward_batched(const mcx_batch* plan,
+                                                   const float* d_grad_tsdf,
+                                                   const int32_t* d_weight0,
+                                                   const uint32_t* d_maxabs, float trunc,
+                                                   int max_w, const float* intr,
+                                                   const float* d_poses, const float* d_depth,
+                                                   float depth_scale, int W, int H, int K,
+                                                   float* d_grad_depth, void* d_ws,
+                                                   size_t ws_bytes, void* stream) {
+  if (!plan || !d_grad_tsdf) return MCX_ERR_ARG;
+  if (plan->elem != MCX_ELEM_F32) return MCX_ERR_ARG;
+  if (K > 0 && (!d_poses || !d_depth || !d_grad_depth)) return MCX_ERR_ARG;
+  if (!d_ws || ws_bytes < mcx_integrate_tsdf_backward_batched_workspace_size(plan, W, H, K))
+    return MCX_ERR_WORKSPACE;
+  if (K == 0) return MCX_OK;
+  if (K > kFramesPerPass) return MCX_ERR_UNSUPPORTED;
+  if (plan->blocks >= ((int64_t)1 << 31)) return MCX_ERR_UNSUPPORTED;
+  hipStream_t s = as_stream(stream);
+  char* b = static_cast<char*>(d_ws);
+  float* rayfac = reinterpret_cast<float*>(b);
+  uint32_t* maxg = reinterpret_cast<uint32_t*>(b + al256((size_t)W * H * sizeof(float)));
+  unsigned long long* acc = reinterpret_cast<unsigned long long*>(
+      b + al256((size_t)W * H * sizeof(float)) + 256);
+  BatchIntegrateParams B;
+  int rc = prepare_integrate(B.base, 1, 1, 1, 1.0f, trunc, max_w, intr, depth_scale, W, H, K,
+                             rayfac, s);
+  if (rc) return rc;
+  const int64_t npix = (int64_t)plan->nvol * K * W * H;
+  if (d_maxabs) {
+    rc = fill_bytes_zero(acc, (size_t)npix * 8, s);  // the accumulators
+    if (rc) return rc;
+  } else {
+    rc = fill_bytes_zero(maxg, 256 + (size_t)npix * 8, s);  // max and accumulators
+    if (rc) return rc;
+    int64_t mb = (plan->voxels + 255) / 256;
+    if (mb > 4096) mb = 4096;
+    hipLaunchKernelGGL(maxabs_kernel, dim3((unsigned)mb), dim3(256), 0, s, d_grad_tsdf,
+                       plan->voxels, maxg);
+    MCX_HIP(hipGetLastError());
+  }
+  const uint32_t* mg = d_maxabs ? d_maxabs : maxg;
+  const float mdt = max_dtdd(intr, depth_scale, trunc, W, H);
+  B.base.K = K;
+  B.base.poses = d_poses;
+  B.base.depth = d_depth;
+  B.bvol = plan->d_bvol;
+  B.nvol = plan->nvol;
+  B.from_init = 0;
+  B.init_value = 0.0f;
+  hipLaunchKernelGGL(integrate_tsdf_backward_batched_kernel, dim3((unsigned)plan->blocks),
+                     dim3(256), 0, s, B, d_grad_tsdf, d_weight0, mg, mdt, acc);
+  MCX_HIP(hipGetLastError());
+  hipLaunchKernelGGL(fixed_to_float_kernel, dim3(blocks_for(npix)), dim3(256), 0, s, acc, npix,
+                     mg, mdt, d_grad_depth);
   MCX_HIP(hipGetLastError());
   return MCX_OK;
 }

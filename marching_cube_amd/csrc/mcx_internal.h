@@ -255,4 +255,54 @@ __device__ __forceinline__ int64_t lookback(const uint64_t* desc, int64_t t, uin
   return acc;
 }
 
+// ------------------------------------------------------------ batch plans
+// Per-volume table of a batch plan (device memory), shared by the batched
+// extraction, MC backward and integration kernels.  Per-voxel arrays of a
+// batch (fp32 TSDF, weights, gradients) are concatenated in volume order:
+// volume b's voxels start at vox_begin.  Workgroups of the per-voxel kernels
+// take 256 consecutive voxels of one volume (blk_begin: the volume's first
+// such workgroup); the batched gather takes 1024 voxels of one plane
+// (gblk_begin).
+struct BatchVol {
+  int X, Y, Z;
+  float h;
+  int64_t vox_begin;
+  int64_t blk_begin;
+  int64_t gblk_begin;
+  FastDiv divX;
+};
+
+struct VolDesc;  // extraction descriptors (mcx_extract.hip)
+struct Column;
+
+// Volume of workgroup `blk` (the last b with begin[b] <= blk), by binary
+// search over a BatchVol table; the member selects which begin.
+template <int64_t BatchVol::*BEGIN>
+__device__ __forceinline__ int batch_volume_of(const BatchVol* bv, int nvol, int64_t blk) {
+  int lo = 0, hi = nvol - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (bv[mid].*BEGIN <= blk) lo = mid; else hi = mid - 1;
+  }
+  return lo;
+}
+
 }  // namespace mcx
+
+// A batch of volumes (opaque in the C ABI): built once by mcx_batch_create.
+struct mcx_batch {
+  int nvol = 0;
+  int elem = MCX_ELEM_SHORT2;
+  int maxX = 0;
+  bool wide = true;  // every volume suits the 128-bit count kernel
+  int slab = 64;     // planes per column
+  int64_t n_tiles = 0;
+  int64_t ncols = 0;
+  int64_t cubes = 0;
+  int64_t voxels = 0;       // all volumes' voxels
+  int64_t blocks = 0;       // 256-voxel workgroups of the per-voxel kernels
+  int64_t gather_blocks = 0;
+  mcx::VolDesc* d_vols = nullptr;
+  mcx::Column* d_cols = nullptr;
+  mcx::BatchVol* d_bvol = nullptr;
+};

@@ -12,8 +12,9 @@ window; see mcx_integrate.hip), stored unpacked in fp32 with int32 weights:
   int64 fixed-point sums).
 * ``FusionGraph`` -- a batch of volumes, each fused from its own K frames,
   extracted, scored with a Chamfer-style vertex loss, and differentiated back
-  to every depth pixel, with the whole sequence (integrate -> extract -> loss
-  -> backward) recorded once in a HIP graph and replayed.
+  to every depth pixel, every stage one batched launch sequence, the whole
+  pass (integrate -> extract -> loss -> backward) recorded once in a HIP
+  graph and replayed.
 """
 import ctypes
 
@@ -128,104 +129,137 @@ def chamfer_grad(pos, targets, count):
 
 
 class FusionGraph:
-    """B volumes (own shape, grid size, K frames of H x W each, own targets):
-    integrate -> extract -> Chamfer loss -> backward to dL/d(depth), recorded
-    in one HIP graph.
+    """B volumes (own shape and grid size; K frames of H x W each, M targets
+    each, the same K, H, W and M for all): integrate -> extract -> Chamfer
+    loss -> backward to dL/d(depth), each stage one launch sequence over the
+    whole batch (batched integration, batched differentiable extraction,
+    batched loss, batched MC backward, batched integration backward), the
+    whole pass recorded once in a HIP graph and replayed.
 
-    Static buffers: ``depth[b]`` (K, H, W) float metres, ``poses[b]`` (K, 12),
-    ``targets[b]`` (M, 3); outputs ``loss`` (B,), ``grad_depth[b]``,
-    ``count[b]``.  Every volume starts from (tsdf_init_value, 0): -1 makes
-    never-observed voxels (object interiors behind the truncation band) count
-    as inside, so they add no surface.  The
-    extraction workspace is shared (the backward recomputes the forward's
-    records), sized by an eager run; ``cap`` bounds the vertices per volume.
+    Static buffers: ``depth`` (B, K, H, W) float metres, ``poses`` (B, K, 12),
+    ``targets`` (B, M, 3) (``depth[b]``... also as per-volume views); outputs
+    ``loss`` (B,), ``grad_depth`` (B, K, H, W), ``count`` (B,), ``vbase``
+    (B + 1: volume b's vertices are rows vbase[b]:vbase[b+1] of ``pos``).
+    Every volume starts from (tsdf_init_value, 0): -1 makes never-observed
+    voxels (object interiors behind the truncation band) count as inside, so
+    they add no surface.  The fp32 TSDF of all volumes is one concatenated
+    array (volume b after volumes 0..b-1); the extraction workspace is sized
+    by an eager run; ``cap`` bounds the vertices of the whole batch.
     """
 
     def __init__(self, shapes, hs, depth, poses, targets, intr, trunc, max_w=128, cap=None,
                  tsdf_init_value=-1.0):
+        from .batch import BatchPlan
         lib = _capi.load()
         self.B = len(shapes)
         self.dev = depth[0].device
         self.shapes = [tuple(int(a) for a in s) for s in shapes]
         self.hs = [float(h) for h in hs]
-        self.depth = [d.contiguous() for d in depth]
-        self.poses = [p.contiguous().to(torch.float32) for p in poses]
-        self.targets = [t.contiguous().to(torch.float32) for t in targets]
+        if len({tuple(d.shape) for d in depth}) != 1 or len({tuple(t.shape) for t in targets}) != 1:
+            raise ValueError("every volume needs the same K x H x W frames and M targets")
+        self.K, self.H, self.W = (int(a) for a in depth[0].shape)
+        self.M = int(targets[0].shape[0])
+        self.depth_all = torch.stack([d.to(torch.float32) for d in depth]).contiguous()
+        self.poses_all = torch.stack([p.to(torch.float32).reshape(self.K, 12)
+                                      for p in poses]).contiguous()
+        self.targets_all = torch.stack([t.to(torch.float32) for t in targets]).contiguous()
+        self.depth = list(self.depth_all.unbind(0))
+        self.poses = list(self.poses_all.unbind(0))
+        self.targets = list(self.targets_all.unbind(0))
+        self.intr = _intr_arr(intr)
+        self.trunc, self.max_w = float(trunc), int(max_w)
         self.init_value = float(tsdf_init_value)
-        self.integ = [_Integrator(s, d.shape[2], d.shape[1], d.shape[0], h, trunc, max_w, intr,
-                                  self.dev) for s, h, d in zip(self.shapes, self.hs, self.depth)]
-        self.tsdf = [torch.empty(s, dtype=torch.float32, device=self.dev) for s in self.shapes]
-        self.w = [torch.empty(s, dtype=torch.int32, device=self.dev) for s in self.shapes]
-        self.w0 = [torch.zeros(s, dtype=torch.int32, device=self.dev) for s in self.shapes]
-        self.gtsdf = [torch.empty(s, dtype=torch.float32, device=self.dev) for s in self.shapes]
-        self.grad_depth = [torch.empty_like(d) for d in self.depth]
-        self.count = torch.zeros(self.B, dtype=torch.int64, device=self.dev)
+        nvox = [Z * Y * X for Z, Y, X in self.shapes]
+        self.vox_begin = np.concatenate([[0], np.cumsum(nvox)]).astype(np.int64)
+        total = int(self.vox_begin[-1])
+        self.tsdf_all = torch.empty(total, dtype=torch.float32, device=self.dev)
+        self.w_all = torch.empty(total, dtype=torch.int32, device=self.dev)
+        self.gtsdf_all = torch.empty(total, dtype=torch.float32, device=self.dev)
+        self.tsdf = [self.tsdf_all[self.vox_begin[b]:self.vox_begin[b + 1]].view(self.shapes[b])
+                     for b in range(self.B)]
+        self.w = [self.w_all[self.vox_begin[b]:self.vox_begin[b + 1]].view(self.shapes[b])
+                  for b in range(self.B)]
+        self.gtsdf = [self.gtsdf_all[self.vox_begin[b]:self.vox_begin[b + 1]].view(self.shapes[b])
+                      for b in range(self.B)]
+        self.plan = BatchPlan(self.tsdf, self.hs)
+        if lib.mcx_batch_voxels(self.plan.handle) != total:
+            raise RuntimeError("batch plan voxel count does not match the concatenated TSDF")
+        self.grad_depth_all = torch.empty_like(self.depth_all)
+        self.grad_depth = list(self.grad_depth_all.unbind(0))
+        self.vbase = torch.zeros(self.B + 1, dtype=torch.int64, device=self.dev)
+        self.total = torch.zeros(1, dtype=torch.int64, device=self.dev)
         self.loss = torch.zeros(self.B, dtype=torch.float32, device=self.dev)
-        # extraction workspace and output capacity from an eager run
-        cubes = max((X - 1) * (Y - 1) * (Z - 1) for Z, Y, X in self.shapes)
-        records = max(65536, cubes // 8)
-        need_cap = 0
-        for b in range(self.B):
-            self._integrate(b)
-            Z, Y, X = self.shapes[b]
-            for _ in range(3):
-                ws_bytes = lib.mcx_extract_workspace_size_ex(X, Y, Z, records)
-                ws = torch.empty(max(ws_bytes, 4096), dtype=torch.uint8, device=self.dev)
-                _capi.check(lib.mcx_workspace_init(_ptr(ws), ws.numel(), _stream_handle(self.dev)),
-                            "workspace_init")
-                hc = ctypes.c_int64(0)
-                st = lib.mcx_extract_diff(_ptr(self.tsdf[b]), X, Y, Z, self.hs[b], None, 0, None,
-                                          ctypes.byref(hc), _ptr(ws), ws.numel(),
-                                          _stream_handle(self.dev))
-                if st == _capi.ERR_WORKSPACE:
-                    records = 2 * records
-                    continue
-                if st not in (_capi.OK, _capi.ERR_CAPACITY):
-                    _capi.check(st, "mcx_extract_diff")
-                break
-            need_cap = max(need_cap, int(hc.value))
-        self.cap = int(cap or max(4096, int(need_cap * 1.25)))
-        self.cap = (self.cap + 1023) // 1024 * 1024  # whole blocks of the loss sum
-        self.ws_bytes = max(lib.mcx_extract_workspace_size_ex(X, Y, Z, records)
-                            for Z, Y, X in self.shapes)
-        self.ws = torch.empty(self.ws_bytes, dtype=torch.uint8, device=self.dev)
-        _capi.check(lib.mcx_workspace_init(_ptr(self.ws), self.ws.numel(),
-                                           _stream_handle(self.dev)), "workspace_init")
-        self.pos = [torch.zeros((self.cap, 3), dtype=torch.float32, device=self.dev)
-                    for _ in range(self.B)]
-        self.gpos = [torch.zeros((self.cap, 3), dtype=torch.float32, device=self.dev)
-                     for _ in range(self.B)]
-        sb = max(lib.mcx_backward_scratch_size(X, Y, Z, self.ws_bytes) for Z, Y, X in self.shapes)
+        self.maxabs = torch.zeros(1, dtype=torch.int32, device=self.dev)
+        self.ws_f = torch.empty(max(4096, lib.mcx_integrate_workspace_size(self.W, self.H)),
+                                dtype=torch.uint8, device=self.dev)
+        self.ws_b = torch.empty(max(4096, lib.mcx_integrate_tsdf_backward_batched_workspace_size(
+            self.plan.handle, self.W, self.H, self.K)), dtype=torch.uint8, device=self.dev)
+        # extraction workspace (every record kept: the backward needs them) and
+        # output capacity from an eager run
+        s = _stream_handle(self.dev)
+        self._integrate()
+        records = max(65536, sum((X - 1) * (Y - 1) * (Z - 1) for Z, Y, X in self.shapes) // 8)
+        for _ in range(4):
+            ws_bytes = max(4096, lib.mcx_batch_workspace_size(self.plan.handle, records))
+            self.ws = torch.empty(ws_bytes, dtype=torch.uint8, device=self.dev)
+            _capi.check(lib.mcx_workspace_init(_ptr(self.ws), self.ws.numel(), s),
+                        "workspace_init")
+            hc = ctypes.c_int64(0)
+            st = lib.mcx_extract_diff_batched(self.plan.handle, None, 0, _ptr(self.vbase), None,
+                                              ctypes.byref(hc), _ptr(self.ws), self.ws.numel(), s)
+            if st == _capi.ERR_WORKSPACE:
+                records *= 2
+                continue
+            if st not in (_capi.OK, _capi.ERR_CAPACITY):
+                _capi.check(st, "mcx_extract_diff_batched")
+            break
+        else:
+            raise RuntimeError("FusionGraph: record scratch still too small")
+        self.cap = int(cap or max(4096, int(hc.value * 1.25)))
+        self.pos = torch.zeros((self.cap, 3), dtype=torch.float32, device=self.dev)
+        self.gpos = torch.zeros((self.cap, 3), dtype=torch.float32, device=self.dev)
+        self.ws_l = torch.empty(max(4096, lib.mcx_chamfer_workspace_size(self.cap)),
+                                dtype=torch.uint8, device=self.dev)
+        sb = lib.mcx_batch_backward_scratch_size(self.plan.handle, self.ws.numel())
         self.scratch = torch.empty(max(sb, 16), dtype=torch.uint8, device=self.dev)
         self.graph = None
 
-    def _integrate(self, b):
-        _capi.check(_capi.load().mcx_tsdf_init(_ptr(self.tsdf[b]), _ptr(self.w[b]),
-                                               self.tsdf[b].numel(), self.init_value,
-                                               _stream_handle(self.dev)), "mcx_tsdf_init")
-        self.integ[b].forward(self.tsdf[b], self.w[b], self.poses[b], self.depth[b])
+    @property
+    def count(self):
+        """Vertices per volume (B,), from the vertex ranges of the last pass."""
+        return self.vbase[1:] - self.vbase[:-1]
+
+    def _integrate(self):
+        _capi.check(_capi.load().mcx_integrate_tsdf_batched(
+            self.plan.handle, _ptr(self.tsdf_all), _ptr(self.w_all), 1, self.init_value,
+            self.trunc, self.max_w, self.intr, _ptr(self.poses_all), _ptr(self.depth_all), 1.0,
+            self.W, self.H, self.K, _ptr(self.ws_f), self.ws_f.numel(),
+            _stream_handle(self.dev)), "mcx_integrate_tsdf_batched")
 
     def step(self):
         """One pass over every volume (eager, or recorded when capturing).
         Launch-only: no allocation, no host synchronisation."""
         lib = _capi.load()
         s = _stream_handle(self.dev)
-        for b in range(self.B):
-            Z, Y, X = self.shapes[b]
-            self._integrate(b)
-            _capi.check(lib.mcx_extract_diff(_ptr(self.tsdf[b]), X, Y, Z, self.hs[b],
-                                             _ptr(self.pos[b]), self.cap,
-                                             _ptr(self.count[b:b + 1]), None, _ptr(self.ws),
-                                             self.ws.numel(), s), "mcx_extract_diff")
-            n = self.count[b]
-            self.loss[b:b + 1].copy_(chamfer_loss(self.pos[b], self.targets[b], n).reshape(1))
-            self.gpos[b].copy_(chamfer_grad(self.pos[b], self.targets[b], n))
-            _capi.check(lib.mcx_extract_backward_recompute(
-                _ptr(self.tsdf[b]), X, Y, Z, self.hs[b], _ptr(self.gpos[b]), _ptr(self.gtsdf[b]),
-                _ptr(self.ws), self.ws.numel(), _ptr(self.scratch), self.scratch.numel(), s),
-                "mcx_extract_backward_recompute")
-            self.integ[b].backward(self.gtsdf[b], self.w0[b], self.poses[b], self.depth[b],
-                                   self.grad_depth[b])
+        h = self.plan.handle
+        self._integrate()
+        _capi.check(lib.mcx_extract_diff_batched(h, _ptr(self.pos), self.cap, _ptr(self.vbase),
+                                                 _ptr(self.total), None, _ptr(self.ws),
+                                                 self.ws.numel(), s), "mcx_extract_diff_batched")
+        _capi.check(lib.mcx_chamfer_batched(_ptr(self.pos), _ptr(self.vbase), self.B,
+                                            _ptr(self.targets_all), self.M, self.cap,
+                                            _ptr(self.loss), _ptr(self.gpos), _ptr(self.ws_l),
+                                            self.ws_l.numel(), s), "mcx_chamfer_batched")
+        _capi.check(lib.mcx_extract_backward_batched(h, _ptr(self.gpos), _ptr(self.gtsdf_all),
+                                                     _ptr(self.maxabs), _ptr(self.ws),
+                                                     self.ws.numel(), _ptr(self.scratch),
+                                                     self.scratch.numel(), s),
+                    "mcx_extract_backward_batched")
+        _capi.check(lib.mcx_integrate_tsdf_backward_batched(
+            h, _ptr(self.gtsdf_all), None, _ptr(self.maxabs), self.trunc, self.max_w, self.intr,
+            _ptr(self.poses_all), _ptr(self.depth_all), 1.0, self.W, self.H, self.K,
+            _ptr(self.grad_depth_all), _ptr(self.ws_b), self.ws_b.numel(), s),
+            "mcx_integrate_tsdf_backward_batched")
 
     def capture(self):
         """Record step() in a HIP graph (after one eager warm-up pass)."""
@@ -246,7 +280,11 @@ class FusionGraph:
 
     def check(self):
         """Raise McxError if an extraction of the pass reported a device-side
-        error since the last check (synchronises the stream)."""
+        error since the last check (synchronises the stream), or if the vertices
+        exceeded the capacity."""
         _capi.check(_capi.load().mcx_workspace_status(_ptr(self.ws), self.ws.numel(),
                                                       _stream_handle(self.dev)),
                     "fusion graph extraction workspace")
+        if int(self.total.item()) > self.cap:
+            raise _capi.McxError(_capi.ERR_CAPACITY, f"FusionGraph: {int(self.total.item())} "
+                                                     f"vertices exceed the capacity {self.cap}")

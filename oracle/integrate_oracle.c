@@ -13,8 +13,8 @@
  * Kept from the reference: ray-distance sdf (depth scaled by
  * sqrt(|K^-1 [u v]|^2 + 1) minus |q|, :420-424, :456-458), nearest-pixel lookup
  * with round-half-even and the [0, W-2) x [0, H-2) pixel window (:447-449),
- * the q.z < FLT_MIN and depth <= FLT_MIN skips (:446, :455), the
- * sdf >= -trunc test (:459) and init to (-32766, 0) (:375-376).
+ * the q.z < FLT_MIN and depth <= FLT_MIN skips (:446, :455), the two
+ * truncation distances (:455-463) and init to (-32766, 0) (:375-376).
  * Changed (documented in DESIGN.md): voxel centres at (i + 0.5) h so that
  * integration and extraction (getNodeCoo) agree; integer weights.
  *
@@ -27,6 +27,7 @@
  */
 #include <float.h>
 #include <math.h>
+#include <stddef.h>
 #include <stdint.h>
 
 #include "mc_oracle.h"
@@ -37,10 +38,16 @@ void orc_volume_init(uint32_t* vol, int64_t n) {
   for (int64_t i = 0; i < n; ++i) vol[i] = v;
 }
 
+/* One observation of a packed voxel.  depth_f32 selects float frames (the
+ * reference's float *d_depth_ptr, :534) over uint16 ones.  Truncation as the
+ * reference (:455-463, the volume's min_t and max_t): skip sdf < -trunc_neg,
+ * t = sdf > 0 ? min(sdf / trunc_pos, 1) : sdf / trunc_neg; with equal
+ * distances this is clamp(sdf / trunc, -1, 1) (sdf >= -trunc gives
+ * sdf / trunc >= -1 exactly). */
 static inline uint32_t update_voxel(uint32_t cur, float px, float py, float pz,
-                                    float trunc, int max_w, const float* intr,
-                                    const float* P, const uint16_t* depth,
-                                    float depth_scale, int W, int H) {
+                                    float trunc_neg, float trunc_pos, int max_w,
+                                    const float* intr, const float* P, const void* depth,
+                                    int depth_f32, float depth_scale, int W, int H) {
   float fx = intr[0], fy = intr[1], cx = intr[2], cy = intr[3];
   float dx = px - P[9], dy = py - P[10], dz = pz - P[11];
   /* q = R^T d */
@@ -53,17 +60,22 @@ static inline uint32_t update_voxel(uint32_t cur, float px, float py, float pz,
   if (!(fabsf(u) < 1e9f) || !(fabsf(v) < 1e9f)) return cur;
   int ui = (int)rintf(u), vi = (int)rintf(v);
   if (ui < 0 || vi < 0 || ui >= W - 2 || vi >= H - 2) return cur;
-  uint16_t dm = depth[(int64_t)vi * W + ui];
-  float d = (float)dm * depth_scale;
+  int64_t pix = (int64_t)vi * W + ui;
+  float dm = depth_f32 ? ((const float*)depth)[pix] : (float)((const uint16_t*)depth)[pix];
+  float d = dm * depth_scale;
   float ax = ((float)ui - cx) / fx, ay = ((float)vi - cy) / fy;
   float zval = d * sqrtf(fmaf(ay, ay, ax * ax) + 1.0f);
   if (!(zval > FLT_MIN)) return cur;
   float qn = sqrtf(fmaf(qz, qz, fmaf(qy, qy, qx * qx)));
   float sdf = zval - qn;
-  if (!(sdf >= -trunc)) return cur;
-  float tsdf = sdf / trunc;
-  if (tsdf > 1.0f) tsdf = 1.0f;
-  if (tsdf < -1.0f) tsdf = -1.0f;
+  if (!(sdf >= -trunc_neg)) return cur;
+  float tsdf;
+  if (sdf > 0.0f) {
+    tsdf = sdf / trunc_pos;
+    if (tsdf > 1.0f) tsdf = 1.0f;
+  } else {
+    tsdf = sdf / trunc_neg;
+  }
   int16_t ti = (int16_t)(cur & 0xffffu);
   int wi = (int)(cur >> 16);
   float told = (float)ti / 32767.0f;
@@ -76,10 +88,11 @@ static inline uint32_t update_voxel(uint32_t cur, float px, float py, float pz,
   return (uint32_t)(uint16_t)(int16_t)fixedp | ((uint32_t)wnew << 16);
 }
 
-void orc_integrate(uint32_t* vol, int X, int Y, int Z, float h, float trunc,
-                   int max_w, const float* intr, const float* poses,
-                   const uint16_t* depth, float depth_scale, int W, int H,
-                   int nframes) {
+void orc_integrate_ex(uint32_t* vol, int X, int Y, int Z, float h, float trunc_neg,
+                      float trunc_pos, int max_w, const float* intr, const float* poses,
+                      const void* depth, int depth_f32, float depth_scale, int W, int H,
+                      int nframes) {
+  const size_t esize = depth_f32 ? sizeof(float) : sizeof(uint16_t);
 #pragma omp parallel for schedule(static)
   for (int z = 0; z < Z; ++z)
     for (int y = 0; y < Y; ++y)
@@ -89,10 +102,19 @@ void orc_integrate(uint32_t* vol, int X, int Y, int Z, float h, float trunc,
               pz = ((float)z + 0.5f) * h;
         uint32_t cur = vol[i];
         for (int k = 0; k < nframes; ++k)
-          cur = update_voxel(cur, px, py, pz, trunc, max_w, intr, poses + 12 * k,
-                             depth + (int64_t)k * W * H, depth_scale, W, H);
+          cur = update_voxel(cur, px, py, pz, trunc_neg, trunc_pos, max_w, intr, poses + 12 * k,
+                             (const char*)depth + (size_t)k * W * H * esize, depth_f32,
+                             depth_scale, W, H);
         vol[i] = cur;
       }
+}
+
+void orc_integrate(uint32_t* vol, int X, int Y, int Z, float h, float trunc,
+                   int max_w, const float* intr, const float* poses,
+                   const uint16_t* depth, float depth_scale, int W, int H,
+                   int nframes) {
+  orc_integrate_ex(vol, X, Y, Z, h, trunc, trunc, max_w, intr, poses, depth, 0, depth_scale, W,
+                   H, nframes);
 }
 
 /* ---- fp32 TSDF volumes (differentiable form): the same observation stored

@@ -88,6 +88,10 @@ void orc_integrate_tsdf(float* tsdf, int32_t* w, int X, int Y, int Z, float h, f
 void orc_integrate_tsdf_f64(double* tsdf, int32_t* w, int X, int Y, int Z, float h, float trunc,
                             int max_w, const float* intr, const float* poses,
                             const double* depth, double depth_scale, int W, int H, int nframes);
+void orc_integrate_ex(uint32_t* vol, int X, int Y, int Z, float h, float trunc_neg,
+                      float trunc_pos, int max_w, const float* intr, const float* poses,
+                      const void* depth, int depth_f32, float depth_scale, int W, int H,
+                      int nframes);
 void orc_integrate(uint32_t* vol, int X, int Y, int Z, float h, float trunc,
                    int max_w, const float* intr, const float* poses,
                    const uint16_t* depth, float depth_scale, int W, int H,

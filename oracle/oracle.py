@@ -52,6 +52,9 @@ def lib():
         L.orc_integrate_tsdf_f64.restype = None
         L.orc_integrate_tsdf_f64.argtypes = [vp, vp, i32, i32, i32, f32, f32, i32, vp, vp, vp,
                                              ctypes.c_double, i32, i32, i32]
+        L.orc_integrate_ex.restype = None
+        L.orc_integrate_ex.argtypes = [vp, i32, i32, i32, f32, f32, f32, i32, vp, vp, vp, i32,
+                                       f32, i32, i32, i32]
         L.orc_integrate.restype = None
         L.orc_integrate.argtypes = [vp, i32, i32, i32, f32, f32, i32, vp, vp, vp, f32,
                                     i32, i32, i32]
@@ -189,6 +192,22 @@ def integrate(vol, h, trunc, max_w, intr, poses, depth, depth_scale=0.001):
     assert poses.shape[0] == K
     lib().orc_integrate(_ptr(vol), X, Y, Z, h, trunc, max_w, _ptr(intr), _ptr(poses),
                         _ptr(depth), depth_scale, W, H, K)
+    return vol
+
+
+def integrate_ex(vol, h, trunc_neg, trunc_pos, max_w, intr, poses, depth, depth_scale=0.001):
+    """In-place integration with two truncation distances (the reference's
+    min_t / max_t) of uint16 or float32 depth frames (K,H,W)."""
+    assert vol.dtype == np.uint32 and vol.flags.c_contiguous
+    Z, Y, X = vol.shape
+    intr = np.ascontiguousarray(intr, np.float32)
+    poses = np.ascontiguousarray(poses, np.float32).reshape(-1, 12)
+    f32 = depth.dtype == np.float32
+    depth = np.ascontiguousarray(depth, np.float32 if f32 else np.uint16)
+    K, H, W = depth.shape
+    assert poses.shape[0] == K
+    lib().orc_integrate_ex(_ptr(vol), X, Y, Z, h, trunc_neg, trunc_pos, max_w, _ptr(intr),
+                           _ptr(poses), _ptr(depth), int(f32), depth_scale, W, H, K)
     return vol
 
 

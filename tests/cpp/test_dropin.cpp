@@ -74,10 +74,11 @@ int main(int argc, char** argv) {
   cu_voxel_rlc_to_mesh(argv[7], const_cast<char*>((out + "/rlc.ply").c_str()), std::atoi(argv[8]));
 
   // 5. cu_voxel_new + one integrated float depth frame: a camera at
-  // (0.8, 0.8, -0.5) looking along +z at a wall 1.2 m away
+  // (0.8, 0.8, -0.5) looking along +z at a wall 1.2 m away; truncation
+  // min_t 0.2 m behind the surface, max_t 0.1 m in front of it
   cu_voxel tv;
   tv.data = nullptr;
-  cu_voxel_new(&tv, make_int3(16, 16, 16), 0.1f, 0.1f, 0.1f);
+  cu_voxel_new(&tv, make_int3(16, 16, 16), 0.2f, 0.1f, 0.1f);
   float_camera_intrinsic ci{};
   ci.fc = make_float2(40.f, 40.f);
   ci.cc = make_float2(15.5f, 11.5f);

@@ -86,15 +86,17 @@ def test_dropin_api(dev, tmp_path):
     rr = O.extract(small, 0.003, want=("pos",))
     assert pr.tobytes() == rr["pos"].tobytes() and nfr == rr["n"] // 3
 
-    # integration of one 1.2 m float depth frame into a 16^3 volume
-    assert np.float32(1200) * np.float32(0.001) == np.float32(1.2)
+    # integration of one 1.2 m float depth frame into a 16^3 volume with the
+    # volume's two truncation distances (min_t 0.2 behind, max_t 0.1 in front)
     ref_vol = O.volume_init((16, 16, 16))
     pose = np.array([1, 0, 0, 0, 1, 0, 0, 0, 1, 0.8, 0.8, -0.5], np.float32)
-    depth = np.full((1, 24, 32), 1200, np.uint16)
-    O.integrate(ref_vol, 0.1, 0.1, 128, np.array([40, 40, 15.5, 11.5], np.float32), pose[None],
-                depth, 0.001)
+    depth = np.full((1, 24, 32), 1.2, np.float32)
+    O.integrate_ex(ref_vol, 0.1, 0.2, 0.1, 128, np.array([40, 40, 15.5, 11.5], np.float32),
+                   pose[None], depth, 1.0)
     fused = np.fromfile(tmp_path / "fused.bin", dtype=np.uint32).reshape(16, 16, 16)
-    assert (synth.unpack_short2(ref_vol)[1] > 0).any()
+    t_ref, w_ref = synth.unpack_short2(ref_vol)
+    assert (w_ref > 0).any()
+    assert ((t_ref < -16383) & (w_ref > 0)).any()  # beyond max_t behind: kept by min_t
     assert fused.tobytes() == ref_vol.tobytes()
 
     # raycast of the fused volume from the integrating camera

@@ -145,3 +145,37 @@ def test_tsdf_f64_matches_f32_and_derivative_closed_form():
     fm[0, 15, 20] -= dl
     fd = (f64(fp)[z, 3, 3] - f64(fm)[z, 3, 3]) / (2 * dl)
     assert abs(fd - 1.0 / (2 * TRUNC)) < 1e-6 / TRUNC
+
+
+def test_split_truncation_axis_voxels():
+    """The reference's two truncation distances (cu_kinect_fusion_v1.cu:455-463):
+    voxels behind the surface by more than trunc_neg are skipped, t = sdf /
+    trunc_neg behind it and min(sdf / trunc_pos, 1) in front; float frames
+    of the same values give the same volume as uint16 frames."""
+    D, TN, TP = 1.43, 0.25, 0.12
+    frames = np.full((1, H, W), round(D * 1000), np.uint16)
+    poses = axis_pose()[None]
+    vol = O.volume_init((N, N, N))
+    O.integrate_ex(vol, HV, TN, TP, 128, INTR, poses, frames)
+    t, w = synth.unpack_short2(vol)
+    seen = 0
+    for z in range(N):
+        sdf = D - ((z + 0.5) * HV + 1.0)
+        if sdf < -TN:
+            assert t[z, 3, 3] == -32766 and w[z, 3, 3] == 0, z
+            continue
+        tt = min(sdf / TP, 1.0) if sdf > 0 else sdf / TN
+        assert w[z, 3, 3] == 1, z
+        assert abs(int(t[z, 3, 3]) - int(tt * 32767)) <= 1, z
+        seen += 1
+    assert seen >= 4
+    # a voxel the single-distance rule skips (sdf < -TP) is kept here
+    assert any(D - ((z + 0.5) * HV + 1.0) < -TP and w[z, 3, 3] == 1 for z in range(N))
+    volf = O.volume_init((N, N, N))
+    O.integrate_ex(volf, HV, TN, TP, 128, INTR, poses, frames.astype(np.float32))
+    assert volf.tobytes() == vol.tobytes()
+    # equal distances reproduce the single-distance form bit for bit
+    a, b = O.volume_init((N, N, N)), O.volume_init((N, N, N))
+    O.integrate(a, HV, TRUNC, 128, INTR, poses, frames)
+    O.integrate_ex(b, HV, TRUNC, TRUNC, 128, INTR, poses, frames)
+    assert a.tobytes() == b.tobytes()
