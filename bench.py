@@ -17,6 +17,7 @@ against 8 TB/s), and the CPU baseline (the oracle's C restatement of the
 same extraction, OpenMP over z planes, timed on the host cores).
 """
 import argparse
+import ctypes
 import json
 import os
 import sys
@@ -59,6 +60,10 @@ def parse():
     ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "extract_kinfu512_traffic.json"),
                     help="JSON with measured HBM bytes per launch (from rocprofv3 --pmc)")
     return ap.parse_args()
+
+
+def nvox_int(dim):
+    return dim * dim * dim
 
 
 def host_threads(requested=0):
@@ -123,18 +128,21 @@ def _event_ms(torch, fn, steps, warmup):
 
 def bench_diff(torch, dev, dim, steps, warmup):
     """Forward vs forward+backward on the 256^3 fp32 sum-of-64-bumps field:
-    plain extraction (float3 positions) against the differentiable forward
-    (which also keeps the per-cube records) + the backward kernels, for the
-    gradient of sum of squared distances of the vertices to a sphere of
-    radius 0.35.  Kernel time between HIP events; the loss gradient is
-    computed once outside the timed region."""
-    import ctypes
-    import numpy as np
+    the plain extraction (float3 positions, mcx_extract) and the
+    differentiable forward (mcx_extract_diff, which also keeps the per-cube
+    records the backward reads), both called directly through the C ABI, and
+    the backward (mcx_extract_backward) for the gradient of the sum of
+    squared distances of the vertices to a sphere of radius 0.35.  Five
+    copies of the 64 MiB field rotate between steps (320 MiB: more than the
+    256 MB MALL holds), each step's backward on its forward's copy.  Kernel
+    time between HIP events; the loss gradient is computed once outside the
+    timed region."""
     from marching_cube_amd import _capi, diff, mesh, synth
     from marching_cube_amd.mesh import _ptr, _stream_handle
     lib = _capi.load()
     f, h = synth.bumps_volume(dim)
-    vol = torch.from_numpy(f).to(dev)
+    copies = [torch.from_numpy(f).to(dev) for _ in range(5)]
+    vol = copies[0]
     Z, Y, X = vol.shape
     ex = diff.DiffExtractor(vol.shape, dev)
     pos = ex.forward(vol, h)
@@ -144,27 +152,33 @@ def bench_diff(torch, dev, dim, steps, warmup):
     nrm = d.norm(dim=1, keepdim=True)
     grad_pos = (2 * (nrm - 0.35) / nrm * d).contiguous()
     gvol = torch.empty_like(vol)
-    # plain forward into its own buffers
-    ws = mesh.Workspace(dev)
+    ws = mesh.Workspace(dev, lib.mcx_extract_workspace_size(X, Y, Z))
     pbuf = torch.empty((count, 3), dtype=torch.float32, device=dev)
     dc = torch.zeros(1, dtype=torch.int64, device=dev)
-    assert mesh.extract_into(vol, h, pos=pbuf, cap=count, workspace=ws) == count
-    assert pbuf.cpu().numpy().tobytes() == pos.cpu().numpy().tobytes()
     stream = _stream_handle(dev)
+    # same positions from both forwards ("the same vertex accuracy")
+    hc = ctypes.c_int64(0)
+    _capi.check(lib.mcx_extract(_ptr(vol), _capi.ELEM_F32, X, Y, Z, float(h), None, _ptr(pbuf),
+                                None, None, count, None, ctypes.byref(hc), ws.ptr, ws.nbytes,
+                                stream), "mcx_extract")
+    assert hc.value == count
+    assert pbuf.cpu().numpy().tobytes() == pos.cpu().numpy().tobytes()
 
     def plain(i):
-        mesh.extract_into(vol, h, pos=pbuf, cap=count, d_count=dc, workspace=ws, sync=False)
+        _capi.check(lib.mcx_extract(_ptr(copies[i % 5]), _capi.ELEM_F32, X, Y, Z, float(h), None,
+                                    _ptr(pbuf), None, None, count, _ptr(dc), None, ws.ptr,
+                                    ws.nbytes, stream), "mcx_extract")
 
     def fwd_diff(i):
-        _capi.check(lib.mcx_extract_diff(_ptr(vol), X, Y, Z, float(h), _ptr(ex.pos),
+        _capi.check(lib.mcx_extract_diff(_ptr(copies[i % 5]), X, Y, Z, float(h), _ptr(ex.pos),
                                          ex.pos.shape[0], _ptr(dc), None, _ptr(ex.ws),
                                          ex.ws.numel(), stream), "mcx_extract_diff")
 
     def bwd(i):
-        _capi.check(lib.mcx_extract_backward(_ptr(vol), X, Y, Z, float(h), _ptr(grad_pos),
-                                             _ptr(gvol), _ptr(ex.ws), ex.ws.numel(),
-                                             _ptr(ex.scratch), ex.scratch.numel(), stream),
-                    "mcx_extract_backward")
+        _capi.check(lib.mcx_extract_backward(_ptr(copies[i % 5]), X, Y, Z, float(h),
+                                             _ptr(grad_pos), _ptr(gvol), _ptr(ex.ws),
+                                             ex.ws.numel(), _ptr(ex.scratch), ex.scratch.numel(),
+                                             stream), "mcx_extract_backward")
 
     def fwd_bwd(i):
         fwd_diff(i)
@@ -173,10 +187,12 @@ def bench_diff(torch, dev, dim, steps, warmup):
     plain_ms, _ = _event_ms(torch, plain, steps, warmup)
     fdiff_ms, _ = _event_ms(torch, fwd_diff, steps, warmup)
     fb_ms, _ = _event_ms(torch, fwd_bwd, steps, warmup)
-    bwd_ms, _ = _event_ms(torch, bwd, steps, warmup)
     ws.check()
     _capi.check(lib.mcx_workspace_status(_ptr(ex.ws), ex.ws.numel(), stream), "diff workspace")
+    ex.forward(vol, h)
     ref = ex.backward(vol, grad_pos)
+    fwd_bwd(0)
+    torch.cuda.synchronize()
     assert gvol.cpu().numpy().tobytes() == ref.cpu().numpy().tobytes()  # deterministic
     # end-to-end autograd step (forward, loss, backward) with a reused extractor
     vg = vol.clone().requires_grad_(True)
@@ -190,53 +206,96 @@ def bench_diff(torch, dev, dim, steps, warmup):
     nvox = X * Y * Z
     return {"workload": f"bumps{dim}_fp32_fwd_bwd", "vertices": count,
             "fwd_ms": round(plain_ms, 4), "fwd_diff_ms": round(fdiff_ms, 4),
-            "bwd_ms": round(bwd_ms, 4), "fwd_bwd_ms": round(fb_ms, 4),
+            "bwd_ms": round(fb_ms - fdiff_ms, 4), "fwd_bwd_ms": round(fb_ms, 4),
             "fwd_bwd_over_fwd": round(fb_ms / plain_ms, 3),
+            "fwd_bwd_over_fwd_diff": round(fb_ms / fdiff_ms, 3),
+            "volume_copies": 5, "calls": "C ABI (mcx_extract, mcx_extract_diff, "
+                                          "mcx_extract_backward), HIP events",
             "fwd_gvoxel_per_s": round(nvox / (plain_ms * 1e-3) / 1e9, 1),
             "autograd_step_ms": round(ag_ms, 4)}
 
 
-def bench_batch(torch, dev, count, steps, warmup):
+def bench_batch(torch, dev, count, steps, warmup, threads_all, no_cpu, tune):
     """3,000 uneven short2 shape volumes (sides log-uniform in [16,128] per
     axis; sphere/box/torus at random poses): one batched launch sequence
-    against one extraction call per volume, same outputs (checked bit-exact)."""
-    import numpy as np
-    from marching_cube_amd import batch, mesh, synth
+    (planes per column and emit workgroups picked by the autotuner) against
+    one extraction call per volume.  Both outputs are checked byte for byte
+    after a call of each into buffers filled with -1.  Roofline: algorithmic
+    bytes 4 * voxels + 28 * vertices per batched call over its HIP-event
+    time.  CPU baseline: the oracle on the first 300 volumes."""
+    from marching_cube_amd import _capi, autotune, batch, mesh, synth
+    lib = _capi.load()
     vols, hs = synth.batch_shapes(count, device=dev)
     torch.cuda.synchronize()
-    plan = batch.BatchPlan(vols, hs)
-    n, vbase, out = plan.extract(want=("vv",))
-    vv = out["vv"]
-    vb = vbase.cpu().numpy()
-    dc = torch.zeros(1, dtype=torch.int64, device=dev)
+    saved = autotune.get()
+    tuned = None
+    if tune:
+        plan0 = batch.BatchPlan(vols, hs)
+        n0, vb0, out0 = plan0.extract(want=("vv",))
+        vv0 = out0["vv"]
+        del plan0
+        t = autotune.tune_batch(lambda: batch.BatchPlan(vols, hs),
+                                lambda pl: pl.extract_into(vv=vv0, cap=n0, vbase=vb0, sync=False))
+        tuned = {"batch_slab": t["batch_slab"], "emit_groups_per_cu": t["emit_groups_per_cu"],
+                 "candidates": len(t["table"])}
+        del vv0, out0
+    try:
+        plan = batch.BatchPlan(vols, hs)
+        n, vbase, out = plan.extract(want=("vv",))
+        vv = out["vv"]
+        vb = vbase.cpu().numpy()
+        dc = torch.zeros(1, dtype=torch.int64, device=dev)   # batched calls' total
+        dcp = torch.zeros(1, dtype=torch.int64, device=dev)  # per-volume calls' counts
 
-    def batched(i):
-        plan.extract_into(vv=vv, cap=n, vbase=vbase, d_count=dc, sync=False)
+        def batched(i):
+            plan.extract_into(vv=vv, cap=n, vbase=vbase, d_count=dc, sync=False)
 
-    vv2 = torch.empty_like(vv)
-    ws = mesh.Workspace(dev)
-    lib_sizes = [mesh.volume_kind(v)[1:] for v in vols]
-    from marching_cube_amd import _capi
-    lib = _capi.load()
-    ws.ensure(max(lib.mcx_extract_workspace_size(X, Y, Z) for X, Y, Z in lib_sizes))
-    segs = [vv2[vb[b]:vb[b + 1]] for b in range(count)]
+        vv2 = torch.empty_like(vv)
+        ws = mesh.Workspace(dev)
+        lib_sizes = [mesh.volume_kind(v)[1:] for v in vols]
+        ws.ensure(max(lib.mcx_extract_workspace_size(X, Y, Z) for X, Y, Z in lib_sizes))
+        segs = [vv2[vb[b]:vb[b + 1]] for b in range(count)]
 
-    def per_volume(i):
-        for b in range(count):
-            mesh.extract_into(vols[b], float(hs[b]), vv=segs[b], cap=int(vb[b + 1] - vb[b]),
-                              d_count=dc, workspace=ws, sync=False)
+        def per_volume(i):
+            for b in range(count):
+                mesh.extract_into(vols[b], float(hs[b]), vv=segs[b], cap=int(vb[b + 1] - vb[b]),
+                                  d_count=dcp, workspace=ws, sync=False)
 
-    b_ms, _ = _event_ms(torch, batched, steps, warmup)
-    p_ms, _ = _event_ms(torch, per_volume, max(2, steps // 4), 1)
-    plan.check()  # device-side status of every asynchronous call above
-    ws.check()
-    assert vv.cpu().numpy().tobytes() == vv2.cpu().numpy().tobytes()
+        b_ms, _ = _event_ms(torch, batched, steps, warmup)
+        p_ms, _ = _event_ms(torch, per_volume, max(2, steps // 4), 1)
+        # one call of each into outputs filled with -1: every vertex written
+        vv.fill_(-1)
+        vv2.fill_(-1)
+        batched(0)
+        per_volume(0)
+        torch.cuda.synchronize()
+        plan.check()  # device-side status of every asynchronous call above
+        ws.check()
+        assert int(dc.item()) == int(vb[count])
+        assert int(dcp.item()) == int(vb[count] - vb[count - 1])  # the last volume's count
+        assert vv.cpu().numpy().tobytes() == vv2.cpu().numpy().tobytes()
+    finally:
+        autotune.set(**saved)
     nvox = int(sum(X * Y * Z for X, Y, Z in lib_sizes))
-    return {"workload": f"batch{count}_shapes_16_128", "volumes": count, "voxels": nvox,
-            "vertices": int(n), "batched_ms": round(b_ms, 4), "per_volume_ms": round(p_ms, 4),
-            "speedup": round(p_ms / b_ms, 2),
-            "batched_gvoxel_per_s": round(nvox / (b_ms * 1e-3) / 1e9, 1),
-            "batched_mtri_per_s": round(n / 3 / (b_ms * 1e-3) / 1e6, 1)}
+    alg = 4 * nvox + 28 * int(n)
+    achieved = alg / (b_ms * 1e-3) / 1e9
+    res = {"workload": f"batch{count}_shapes_16_128", "volumes": count, "voxels": nvox,
+           "vertices": int(n), "batched_ms": round(b_ms, 4), "per_volume_ms": round(p_ms, 4),
+           "speedup": round(p_ms / b_ms, 2),
+           "batched_gvoxel_per_s": round(nvox / (b_ms * 1e-3) / 1e9, 1),
+           "batched_mtri_per_s": round(n / 3 / (b_ms * 1e-3) / 1e6, 1),
+           "algorithmic_bytes": alg,
+           "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                        "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4)},
+           "tuning": tuned, "cpu_baseline": None}
+    if not no_cpu:
+        m = min(300, count)
+        host = [v.cpu().numpy().view(np.uint32) for v in vols[:m]]
+        res["cpu_baseline"] = cpu_baseline_entry(
+            host, [float(x) for x in hs[:m]], threads_all, 3, 3,
+            f"the first {m} of the {count} volumes (same inputs); single thread and all cores "
+            "(OpenMP over each volume's planes): median of 3 passes after 1 warm-up pass")
+    return res
 
 
 def bench_c1(torch, dev, steps, warmup, threads_all, no_cpu):
@@ -423,11 +482,77 @@ def bench_fusion_graph(torch, dev, nvol, steps):
     g.capture()
     g_ms, _ = _event_ms(torch, lambda i: g.replay(), max(3, steps // 4), 1)
     g.check()
+    loss_g = g.loss.clone()
+    grad_g = g.grad_depth_all.clone()
+    vol_ms, loss_v, grad_v = per_volume_fusion(torch, dev, g)
+    # the same pass one volume at a time: the same losses, and gradients equal
+    # up to the fixed-point scale (per volume there, per batch here)
+    assert torch.allclose(loss_v, loss_g, rtol=1e-5, atol=0)
+    assert (grad_v - grad_g).abs().max().item() <= 1e-6 * grad_g.abs().max().item()
     nvox = int(sum(a * b * c for a, b, c in shapes))
     return {"workload": f"fusion{nvol}_64_192_diff_graph", "volumes": nvol, "voxels": nvox,
             "frames_per_volume": K, "frame_size": [W, H],
             "vertices": int(g.count.sum().item()), "graph_ms": round(g_ms, 3),
-            "eager_ms": round(e_ms, 3), "speedup": round(e_ms / g_ms, 2)}
+            "eager_ms": round(e_ms, 3), "per_volume_ms": round(vol_ms, 3),
+            "graph_vs_per_volume": round(vol_ms / g_ms, 2),
+            "gvoxel_frames_per_s": round(nvox * K / (g_ms * 1e-3) / 1e9, 1)}
+
+
+def per_volume_fusion(torch, dev, g):
+    """The pass of FusionGraph `g` one volume at a time through the
+    single-volume entry points (fp32 integration, differentiable extraction,
+    the loss with one volume, MC backward, integration backward), with the
+    same kernels: HIP-event time of one eager pass, its losses and dL/ddepth."""
+    import ctypes
+    from marching_cube_amd import _capi, fusion_diff
+    from marching_cube_amd.mesh import _ptr, _stream_handle
+    lib = _capi.load()
+    s = _stream_handle(dev)
+    tsdf = torch.empty_like(g.tsdf_all)
+    w = torch.empty_like(g.w_all)
+    gt = torch.empty_like(g.tsdf_all)
+    grad = torch.empty_like(g.grad_depth_all)
+    loss = torch.zeros(g.B, dtype=torch.float32, device=dev)
+    big = max(range(g.B), key=lambda b: g.tsdf[b].numel())
+    Zb, Yb, Xb = g.shapes[big]
+    ws = torch.empty(max(lib.mcx_extract_workspace_size_ex(Xb, Yb, Zb, 1 << 23), 4096),
+                     dtype=torch.uint8, device=dev)
+    _capi.check(lib.mcx_workspace_init(_ptr(ws), ws.numel(), s), "workspace_init")
+    # the same workspace leaves more record room to smaller volumes
+    scr = torch.empty(max(16, max(lib.mcx_backward_scratch_size(X, Y, Z, ws.numel())
+                                  for Z, Y, X in g.shapes)), dtype=torch.uint8, device=dev)
+    cap = int(g.count.max().item() * 2) + 4096
+    pos = torch.zeros((cap, 3), dtype=torch.float32, device=dev)
+    gpos = torch.zeros_like(pos)
+    vb1 = torch.zeros((g.B, 2), dtype=torch.int64, device=dev)  # per volume [0, count]
+    ws_l = torch.empty(max(4096, lib.mcx_chamfer_workspace_size(cap)), dtype=torch.uint8,
+                       device=dev)
+    integ = [fusion_diff._Integrator(g.shapes[b], g.W, g.H, g.K, g.hs[b], g.trunc, g.max_w,
+                                     list(g.intr), dev) for b in range(g.B)]
+
+    def one(i):
+        for b in range(g.B):
+            Z, Y, X = g.shapes[b]
+            n = Z * Y * X
+            tb = tsdf[g.vox_begin[b]:g.vox_begin[b] + n].view(Z, Y, X)
+            wb = w[g.vox_begin[b]:g.vox_begin[b] + n].view(Z, Y, X)
+            gb = gt[g.vox_begin[b]:g.vox_begin[b] + n].view(Z, Y, X)
+            _capi.check(lib.mcx_tsdf_init(_ptr(tb), _ptr(wb), n, g.init_value, s), "tsdf_init")
+            integ[b].forward(tb, wb, g.poses[b], g.depth[b])
+            _capi.check(lib.mcx_extract_diff(_ptr(tb), X, Y, Z, g.hs[b], _ptr(pos), cap,
+                                             _ptr(vb1[b, 1:]), None, _ptr(ws), ws.numel(), s),
+                        "mcx_extract_diff")
+            _capi.check(lib.mcx_chamfer_batched(_ptr(pos), _ptr(vb1[b]), 1, _ptr(g.targets[b]),
+                                                g.M, cap, _ptr(loss[b:]), _ptr(gpos),
+                                                _ptr(ws_l), ws_l.numel(), s), "chamfer")
+            _capi.check(lib.mcx_extract_backward(_ptr(tb), X, Y, Z, g.hs[b], _ptr(gpos),
+                                                 _ptr(gb), _ptr(ws), ws.numel(), _ptr(scr),
+                                                 scr.numel(), s), "mcx_extract_backward")
+            integ[b].backward(gb, None, g.poses[b], g.depth[b], grad[b])
+
+    ms, _ = _event_ms(torch, one, 1, 1)
+    _capi.check(lib.mcx_workspace_status(_ptr(ws), ws.numel(), s), "per-volume workspace")
+    return ms, loss, grad
 
 
 def main():
@@ -461,7 +586,43 @@ def main():
     e1.record()
     torch.cuda.synchronize()
     integrate_ms = e0.elapsed_time(e1)
-    del depth
+    # the reference's call pattern: one integration call per frame (K = 1)
+    from marching_cube_amd import _capi
+    from marching_cube_amd.mesh import _ptr, _stream_handle
+    lib = _capi.load()
+    vox1 = fusion.voxel_new((dim, dim, dim), cfg["trunc"], h, device=dev)
+    dposes = torch.as_tensor(poses, dtype=torch.float32).reshape(-1, 12).to(dev).contiguous()
+    iws = torch.empty(lib.mcx_integrate_workspace_size(cfg["W"], cfg["H"]), dtype=torch.uint8,
+                      device=dev)
+    intr_c = (ctypes.c_float * 4)(*intr)
+    st = _stream_handle(dev)
+    frame_px = cfg["W"] * cfg["H"]
+    torch.cuda.synchronize()
+    e0.record()
+    for k in range(args.frames):
+        _capi.check(lib.mcx_integrate(_ptr(vox1.data), dim, dim, dim, float(h),
+                                      float(cfg["trunc"]), int(cfg["max_w"]), intr_c,
+                                      ctypes.c_void_p(dposes.data_ptr() + 48 * k),
+                                      ctypes.c_void_p(depth.data_ptr() + 2 * frame_px * k),
+                                      0.001, cfg["W"], cfg["H"], 1, _ptr(iws), iws.numel(), st),
+                    "mcx_integrate")
+    e1.record()
+    torch.cuda.synchronize()
+    per_frame_ms = e0.elapsed_time(e1) / args.frames
+    assert torch.equal(vox1.data, vox.data)  # same updates in the same order
+    del vox1, depth
+    frame_bytes = 8 * nvox_int(dim) + 2 * cfg["W"] * cfg["H"]
+    per_frame = {"workload": f"kinfu{dim}_integrate_per_frame", "frames": args.frames,
+                 "ms_per_frame": round(per_frame_ms, 4),
+                 "algorithmic_bytes_per_frame": frame_bytes,
+                 "roofline": {"bound": "hbm",
+                              "achieved": round(frame_bytes / (per_frame_ms * 1e-3) / 1e9, 1),
+                              "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                              "frac": round(frame_bytes / (per_frame_ms * 1e-3) / 1e9
+                                            / HBM_PEAK_GBS, 4)},
+                 "note": "one mcx_integrate call per frame; a voxel is read and written only "
+                         "where the frame updates it"}
+    print(f"[bench] integrate per frame {per_frame}", file=sys.stderr, flush=True)
     vols = [vox.data, vox.data.clone()]
     print(f"[bench] volume {dim}^3 fused from {args.frames} frames in {integrate_ms:.1f} ms "
           f"(setup {time.perf_counter() - t0:.1f} s)", file=sys.stderr, flush=True)
@@ -539,7 +700,8 @@ def main():
     diff_leg = bench_diff(torch, dev, args.diff_dim, args.steps, args.warmup) if args.diff_dim else None
     if diff_leg:
         print(f"[bench] diff {diff_leg}", file=sys.stderr, flush=True)
-    batch_leg = (bench_batch(torch, dev, args.batch_count, args.steps, args.warmup)
+    batch_leg = (bench_batch(torch, dev, args.batch_count, args.steps, args.warmup,
+                             host_threads(args.cpu_threads), args.no_cpu, not args.no_tune)
                  if args.batch_count else None)
     if batch_leg:
         print(f"[bench] batch {batch_leg}", file=sys.stderr, flush=True)
@@ -612,6 +774,7 @@ def main():
         "full_api_ms": round(full_ms, 4),
         "integrate_ms": round(integrate_ms, 2),
         "integrate_gvoxel_frames_per_s": round(nvox * args.frames / (integrate_ms * 1e-3) / 1e9, 2),
+        "integrate_per_frame": per_frame,
         "fwd_bwd": diff_leg,
         "batched": batch_leg,
         "fusion_graph": graph_leg,

@@ -74,15 +74,18 @@ struct IntegrateParams {
 // Frames that no voxel of a workgroup can observe, as bits of s_skip (one per
 // frame).  The workgroup's voxels, linear indices [i0, i1] of one X x Y x Z
 // volume, lie in a box of voxel centres; frame k is skipped when the whole
-// box is behind its camera (every corner at camera depth below -1e-4 m, so
-// every voxel's fp32 q.z is below FLT_MIN), or in front of it (every corner
-// deeper than 1 cm) and projecting past one edge of the pixel window by more
-// than a pixel (plus 1e-5 of the coordinate): the image of a convex box in
-// front of the camera is the convex hull of its corners' images.  Corners are
-// projected in double precision from the voxels' own fp32 centres; a voxel's
-// fp32 projection differs from the exact one by far less than the margin, so
-// every voxel of a skipped frame would have taken its q.z or window skip.
-// Called by every thread of the workgroup (two barriers inside).
+// box is behind its camera, or in front of it (every corner deeper than 1 cm)
+// and projecting past one edge of the pixel window: the image of a convex box
+// in front of the camera is the convex hull of its corners' images.  Eight
+// lanes take a frame, one corner each, with the voxels' own fp32 operations
+// (a corner is a voxel centre).  A voxel's fp32 q differs from the exact one
+// by at most d = 4e-7 * (|dx| + |dy| + |dz|) per component (the subtraction
+// and the fma chain, |R| <= 1), so its u differs from the exact projection by
+// at most (fx d / qz_min) (1 + |u - cx|_max / fx), likewise v; the tests use
+// four times that plus one pixel as the margin ("behind": every corner below
+// -max(1e-4 m, 4 d)), so every voxel of a skipped frame would have taken its
+// q.z or window skip.  Called by every thread of the workgroup (barriers
+// inside).
 __device__ void frame_skip_flags(const float* s_pose, int K, const IntegrateParams& P,
                                  int64_t i0, int64_t i1, uint32_t* s_skip) {
   const int64_t XY = (int64_t)P.X * P.Y;
@@ -97,52 +100,101 @@ __device__ void frame_skip_flags(const float* s_pose, int K, const IntegratePara
       x1 = r1 - y1 * P.X;
     }
   }
-  const double cx[2] = {(double)(((float)x0 + 0.5f) * P.h), (double)(((float)x1 + 0.5f) * P.h)};
-  const double cy[2] = {(double)(((float)y0 + 0.5f) * P.h), (double)(((float)y1 + 0.5f) * P.h)};
-  const double cz[2] = {(double)(((float)z0 + 0.5f) * P.h), (double)(((float)z1 + 0.5f) * P.h)};
-  for (int k0 = 0; k0 < K; k0 += blockDim.x) {
-    const int k = k0 + (int)threadIdx.x;
-    bool skip = false;
+  for (int w = threadIdx.x; w < (K + 31) / 32; w += blockDim.x) s_skip[w] = 0;
+  __syncthreads();
+  const int lane = threadIdx.x & 63, c = lane & 7;
+  const float px = ((float)((c & 1) ? x1 : x0) + 0.5f) * P.h;
+  const float py = ((float)((c & 2) ? y1 : y0) + 0.5f) * P.h;
+  const float pz = ((float)((c & 4) ? z1 : z0) + 0.5f) * P.h;
+  const int nf = (int)(blockDim.x >> 3);  // frames per round
+  for (int kb = (int)(threadIdx.x >> 6) * 8; kb < K; kb += nf) {
+    const int k = kb + (lane >> 3);
+    float qz = 0.0f, u = 0.0f, v = 0.0f, dist = 0.0f;
     if (k < K) {
       const float* R = s_pose + 12 * k;
-      double qzmin = 1e300, qzmax = -1e300, umin = 1e300, umax = -1e300, vmin = 1e300,
-             vmax = -1e300;
+      const float dx = px - R[9], dy = py - R[10], dz = pz - R[11];
+      dist = fabsf(dx) + fabsf(dy) + fabsf(dz);
+      const float qx = __fmaf_rn(R[6], dz, __fmaf_rn(R[3], dy, R[0] * dx));
+      const float qy = __fmaf_rn(R[7], dz, __fmaf_rn(R[4], dy, R[1] * dx));
+      qz = __fmaf_rn(R[8], dz, __fmaf_rn(R[5], dy, R[2] * dx));
+      u = __fmaf_rn(P.fx, qx / qz, P.cx);
+      v = __fmaf_rn(P.fy, qy / qz, P.cy);
+    }
+    float qzmin = qz, qzmax = qz, umin = u, umax = u, vmin = v, vmax = v, dmax = dist;
 #pragma unroll
-      for (int c = 0; c < 8; ++c) {
-        const double dx = cx[c & 1] - R[9], dy = cy[(c >> 1) & 1] - R[10], dz = cz[c >> 2] - R[11];
-        const double qx = (double)R[0] * dx + (double)R[3] * dy + (double)R[6] * dz;
-        const double qy = (double)R[1] * dx + (double)R[4] * dy + (double)R[7] * dz;
-        const double qz = (double)R[2] * dx + (double)R[5] * dy + (double)R[8] * dz;
-        qzmin = fmin(qzmin, qz);
-        qzmax = fmax(qzmax, qz);
-        const double u = (double)P.fx * (qx / qz) + (double)P.cx;
-        const double v = (double)P.fy * (qy / qz) + (double)P.cy;
-        umin = fmin(umin, u);
-        umax = fmax(umax, u);
-        vmin = fmin(vmin, v);
-        vmax = fmax(vmax, v);
-      }
-      if (qzmax < -1e-4) {
+    for (int o = 1; o < 8; o <<= 1) {
+      dmax = fmaxf(dmax, __shfl_xor(dmax, o));
+      qzmin = fminf(qzmin, __shfl_xor(qzmin, o));
+      qzmax = fmaxf(qzmax, __shfl_xor(qzmax, o));
+      umin = fminf(umin, __shfl_xor(umin, o));
+      umax = fmaxf(umax, __shfl_xor(umax, o));
+      vmin = fminf(vmin, __shfl_xor(vmin, o));
+      vmax = fmaxf(vmax, __shfl_xor(vmax, o));
+    }
+    bool skip = false;
+    if (k < K) {
+      const float d = 4e-7f * dmax;  // bound on a voxel's fp32 error in q
+      if (qzmax < -fmaxf(1e-4f, 4.0f * d)) {
         skip = true;
-      } else if (qzmin > 0.01) {
-        const double tol = 1.0 + 1e-5 * fmax(fmax(fabs(umin), fabs(umax)), fmax(fabs(vmin), fabs(vmax)));
+      } else if (qzmin > 0.01f) {
+        const float au = fmaxf(fabsf(umin - P.cx), fabsf(umax - P.cx)) / P.fx;
+        const float av = fmaxf(fabsf(vmin - P.cy), fabsf(vmax - P.cy)) / P.fy;
+        const float tu = 1.0f + 4.0f * (P.fx * d / qzmin) * (1.0f + au);
+        const float tv = 1.0f + 4.0f * (P.fy * d / qzmin) * (1.0f + av);
         // a voxel is observed only at rint(u) in [0, W - 2), rint(v) in [0, H - 2)
-        skip = umax < -0.5 - tol || umin > (double)P.W - 2.5 + tol || vmax < -0.5 - tol ||
-               vmin > (double)P.H - 2.5 + tol;
+        skip = umax < -0.5f - tu || umin > (float)P.W - 2.5f + tu || vmax < -0.5f - tv ||
+               vmin > (float)P.H - 2.5f + tv;
       }
     }
-    const uint64_t m = __ballot(skip);
-    if ((threadIdx.x & 63) == 0 && k0 + (int)threadIdx.x < K) {
-      const int w = (k0 + (int)threadIdx.x) >> 5;
-      s_skip[w] = (uint32_t)m;
-      if (k0 + (int)threadIdx.x + 32 < K) s_skip[w + 1] = (uint32_t)(m >> 32);
+    const uint64_t m = __ballot(skip && c == 0);  // bit 8 j: frame kb + j
+    if (lane == 0 && m) {
+      uint32_t bits = 0;
+#pragma unroll
+      for (int j = 0; j < 8; ++j) bits |= (uint32_t)((m >> (8 * j)) & 1u) << j;
+      atomicOr(&s_skip[kb >> 5], bits << (kb & 31));
     }
   }
   __syncthreads();
 }
 
-__device__ __forceinline__ bool skipped(const uint32_t* s_skip, int k) {
-  return (s_skip[k >> 5] >> (k & 31)) & 1u;
+// f(k) for every frame k < K not marked in s_skip, in order (a uniform
+// 32-frame mask at a time: one LDS read per 32 frames).
+template <typename F>
+__device__ __forceinline__ void for_active_frames(const uint32_t* s_skip, int K, F&& f) {
+  for (int k0 = 0; k0 < K; k0 += 32) {
+    uint32_t act = ~s_skip[k0 >> 5];
+    if (K - k0 < 32) act &= (1u << (K - k0)) - 1u;
+    act = __builtin_amdgcn_readfirstlane(act);
+    while (act) {
+      const int k = k0 + __builtin_ctz(act);
+      act &= act - 1u;
+      f(k);
+    }
+  }
+}
+
+// Packed short2 update of voxel i by frames [0, K) (the oracle's
+// update_voxel).  The voxel is read at its first update and written only if
+// some frame updated it: outside every frustum and behind every surface it
+// costs no memory traffic.
+// One packed update (the oracle's update_voxel after its lookups): the new
+// (tsdf, weight) word from the current one and the observation t.
+__device__ __forceinline__ uint32_t packed_update(uint32_t cur, float tsdf, int max_w) {
+  const int16_t ti = (int16_t)(cur & 0xffffu);
+  const int wi = (int)(cur >> 16);
+  const float told = (float)ti / 32767.0f;
+  const float tnew = __fmaf_rn(told, (float)wi, tsdf) / (float)(wi + 1);
+  const int wnew = wi + 1 < max_w ? wi + 1 : max_w;
+  int fixedp = (int)(tnew * 32767.0f);
+  fixedp = fixedp > 32767 ? 32767 : (fixedp < -32767 ? -32767 : fixedp);
+  return (uint32_t)(uint16_t)(int16_t)fixedp | ((uint32_t)wnew << 16);
+}
+
+// The truncated observation of a signed distance: sdf / max_t in front of the
+// surface (at most 1), sdf / min_t behind it (at least -1 once sdf >= -min_t
+// was tested): one division by the selected distance.
+__device__ __forceinline__ float truncated(float sdf, float trunc_pos, float trunc_neg) {
+  return fminf(sdf / (sdf > 0.0f ? trunc_pos : trunc_neg), 1.0f);
 }
 
 // Packed short2 update of voxel i by frames [0, K) (the oracle's
@@ -156,48 +208,46 @@ __device__ __forceinline__ void integrate_voxel(const IntegrateParams& P, const 
   const int64_t frame = (int64_t)P.W * P.H;
   uint32_t cur = 0;
   bool have = false;
-  for (int k = 0; k < P.K; ++k) {
-    if (skipped(s_skip, k)) continue;
+  for_active_frames(s_skip, P.K, [&](int k) {
     const float* R = s_pose + 12 * k;
     const float dx = px - R[9], dy = py - R[10], dz = pz - R[11];
     const float qx = __fmaf_rn(R[6], dz, __fmaf_rn(R[3], dy, R[0] * dx));
     const float qy = __fmaf_rn(R[7], dz, __fmaf_rn(R[4], dy, R[1] * dx));
     const float qz = __fmaf_rn(R[8], dz, __fmaf_rn(R[5], dy, R[2] * dx));
-    if (qz < FLT_MIN) continue;
+    if (qz < FLT_MIN) return;
     const float u = __fmaf_rn(P.fx, qx / qz, P.cx);
     const float v = __fmaf_rn(P.fy, qy / qz, P.cy);
-    if (!(fabsf(u) < 1e9f) || !(fabsf(v) < 1e9f)) continue;
+    if (!(fabsf(u) < 1e9f) || !(fabsf(v) < 1e9f)) return;
     const int ui = (int)rintf(u), vi = (int)rintf(v);
-    if (ui < 0 || vi < 0 || ui >= P.W - 2 || vi >= P.H - 2) continue;
+    if (ui < 0 || vi < 0 || ui >= P.W - 2 || vi >= P.H - 2) return;
     const int pix = vi * P.W + ui;
     const DT dm = static_cast<const DT*>(P.depth)[(int64_t)k * frame + pix];
-    const float d = (float)dm * P.depth_scale;
-    const float zval = d * P.rayfac[pix];
-    if (!(zval > FLT_MIN)) continue;
+    const float zval = ((float)dm * P.depth_scale) * P.rayfac[pix];
+    if (!(zval > FLT_MIN)) return;
     const float qn = sqrtf(__fmaf_rn(qz, qz, __fmaf_rn(qy, qy, qx * qx)));
     const float sdf = zval - qn;
-    if (!(sdf >= -P.trunc_neg)) continue;
-    const float tsdf = sdf > 0.0f ? fminf(sdf / P.trunc, 1.0f) : sdf / P.trunc_neg;
+    if (!(sdf >= -P.trunc_neg)) return;
+    const float tsdf = truncated(sdf, P.trunc, P.trunc_neg);
     if (!have) {
       cur = P.vol[i];
       have = true;
     }
-    const int16_t ti = (int16_t)(cur & 0xffffu);
-    const int wi = (int)(cur >> 16);
-    const float told = (float)ti / 32767.0f;
-    const float tnew = __fmaf_rn(told, (float)wi, tsdf) / (float)(wi + 1);
-    const int wnew = wi + 1 < P.max_w ? wi + 1 : P.max_w;
-    int fixedp = (int)(tnew * 32767.0f);
-    fixedp = fixedp > 32767 ? 32767 : (fixedp < -32767 ? -32767 : fixedp);
-    cur = (uint32_t)(uint16_t)(int16_t)fixedp | ((uint32_t)wnew << 16);
-  }
+    cur = packed_update(cur, tsdf, P.max_w);
+  });
   if (have) P.vol[i] = cur;
+}
+
+// Launches size the shared memory to K frames: poses (12 floats each), then
+// one skip bit per frame.
+__host__ __device__ constexpr size_t frames_lds_bytes(int K) {
+  return (size_t)K * 12 * sizeof(float) + (size_t)((K + 31) / 32) * sizeof(uint32_t);
 }
 
 template <typename DT>
 __global__ __launch_bounds__(256) void integrate_kernel(IntegrateParams P) {
-  __shared__ float s_pose[kFramesPerPass * 12];
-  __shared__ uint32_t s_skip[kFramesPerPass / 32];
+  extern __shared__ float s_frames[];
+  float* const s_pose = s_frames;
+  uint32_t* const s_skip = reinterpret_cast<uint32_t*>(s_frames + 12 * P.K);
   for (int i = threadIdx.x; i < P.K * 12; i += blockDim.x) s_pose[i] = P.poses[i];
   __syncthreads();
   const int64_t n = (int64_t)P.X * P.Y * P.Z;
@@ -215,6 +265,77 @@ __global__ __launch_bounds__(256) void integrate_kernel(IntegrateParams P) {
   const float pz = ((float)z + 0.5f) * P.h;
   integrate_voxel<DT>(P, s_pose, s_skip, px, py, pz, i);
 }
+
+// One frame (the reference's per-frame call): four voxels a thread (voxel
+// ib + t + 256 j of 1024 per workgroup, coalesced).  The four projections
+// come first, then the depth, ray-factor and voxel loads of all four are
+// issued together (at clamped, valid addresses), then the updates: the
+// operations and results of integrate_voxel, with the memory latency of the
+// four overlapped.  Only voxels the frame updates are written.
+constexpr int kVoxelsPerThread1 = 4;
+template <typename DT>
+__global__ __launch_bounds__(256) void integrate1_kernel(IntegrateParams P) {
+  __shared__ float s_pose[12];
+  __shared__ uint32_t s_skip[1];
+  if (threadIdx.x < 12) s_pose[threadIdx.x] = P.poses[threadIdx.x];
+  __syncthreads();
+  const int64_t n = (int64_t)P.X * P.Y * P.Z;
+  const int64_t ib = (int64_t)blockIdx.x * 256 * kVoxelsPerThread1;
+  frame_skip_flags(s_pose, 1, P, ib, min(ib + 256 * kVoxelsPerThread1, n) - 1, s_skip);
+  if (s_skip[0] & 1u) return;  // no voxel of the workgroup can see the frame
+  const float* R = s_pose;
+  const uint32_t xy = (uint32_t)((int64_t)P.X * P.Y);
+  int pix[kVoxelsPerThread1];
+  float qn[kVoxelsPerThread1];
+  int64_t vi[kVoxelsPerThread1];
+#pragma unroll
+  for (int j = 0; j < kVoxelsPerThread1; ++j) {
+    const int64_t i = ib + threadIdx.x + 256 * j;
+    vi[j] = i < n ? i : n - 1;
+    pix[j] = -1;
+    qn[j] = 0.0f;
+    if (i >= n) continue;
+    const int z = (int)(i / xy);
+    const uint32_t rem = (uint32_t)(i - (int64_t)z * xy);
+    const uint32_t y = P.divX.div(rem);
+    const int x = (int)(rem - y * (uint32_t)P.X);
+    const float px = ((float)x + 0.5f) * P.h;
+    const float py = ((float)(int)y + 0.5f) * P.h;
+    const float pz = ((float)z + 0.5f) * P.h;
+    const float dx = px - R[9], dy = py - R[10], dz = pz - R[11];
+    const float qx = __fmaf_rn(R[6], dz, __fmaf_rn(R[3], dy, R[0] * dx));
+    const float qy = __fmaf_rn(R[7], dz, __fmaf_rn(R[4], dy, R[1] * dx));
+    const float qz = __fmaf_rn(R[8], dz, __fmaf_rn(R[5], dy, R[2] * dx));
+    if (qz < FLT_MIN) continue;
+    const float u = __fmaf_rn(P.fx, qx / qz, P.cx);
+    const float v = __fmaf_rn(P.fy, qy / qz, P.cy);
+    if (!(fabsf(u) < 1e9f) || !(fabsf(v) < 1e9f)) continue;
+    const int ui = (int)rintf(u), vv = (int)rintf(v);
+    if (ui < 0 || vv < 0 || ui >= P.W - 2 || vv >= P.H - 2) continue;
+    pix[j] = vv * P.W + ui;
+    qn[j] = sqrtf(__fmaf_rn(qz, qz, __fmaf_rn(qy, qy, qx * qx)));
+  }
+  DT dm[kVoxelsPerThread1];
+  float rf[kVoxelsPerThread1];
+  uint32_t cur[kVoxelsPerThread1];
+#pragma unroll
+  for (int j = 0; j < kVoxelsPerThread1; ++j) {
+    const int q = pix[j] >= 0 ? pix[j] : 0;
+    dm[j] = static_cast<const DT*>(P.depth)[q];
+    rf[j] = P.rayfac[q];
+    cur[j] = P.vol[vi[j]];
+  }
+#pragma unroll
+  for (int j = 0; j < kVoxelsPerThread1; ++j) {
+    if (pix[j] < 0) continue;
+    const float zval = ((float)dm[j] * P.depth_scale) * rf[j];
+    if (!(zval > FLT_MIN)) continue;
+    const float sdf = zval - qn[j];
+    if (!(sdf >= -P.trunc_neg)) continue;
+    P.vol[vi[j]] = packed_update(cur[j], truncated(sdf, P.trunc, P.trunc_neg), P.max_w);
+  }
+}
+
 
 // ---- fp32 TSDF volumes (differentiable form; no counterpart in the reference)
 // tsdf (fp32) and weight (int32) in separate [z][y][x] arrays; the same
@@ -261,11 +382,10 @@ __device__ __forceinline__ void integrate_tsdf_voxel(const IntegrateParams& P, c
                                                      const float* tsdf, const int32_t* wvol,
                                                      bool* changed) {
   bool have = tsdf == nullptr;
-  for (int k = 0; k < P.K; ++k) {
-    if (skipped(s_skip, k)) continue;
+  for_active_frames(s_skip, P.K, [&](int k) {
     int pix;
     float t, dtdd;
-    if (!observe<float>(P, s_pose + 12 * k, k, px, py, pz, &pix, &t, &dtdd)) continue;
+    if (!observe<float>(P, s_pose + 12 * k, k, px, py, pz, &pix, &t, &dtdd)) return;
     if (!have) {
       f = *tsdf;
       w = *wvol;
@@ -274,7 +394,7 @@ __device__ __forceinline__ void integrate_tsdf_voxel(const IntegrateParams& P, c
     *changed = true;
     f = __fmaf_rn(f, (float)w, t) / (float)(w + 1);
     w = w + 1 < P.max_w ? w + 1 : P.max_w;
-  }
+  });
 }
 
 __device__ __forceinline__ void voxel_centre(const IntegrateParams& P, int64_t i, float* px,
@@ -291,8 +411,9 @@ __device__ __forceinline__ void voxel_centre(const IntegrateParams& P, int64_t i
 
 __global__ __launch_bounds__(256) void integrate_tsdf_kernel(IntegrateParams P, float* tsdf,
                                                              int32_t* wvol) {
-  __shared__ float s_pose[kFramesPerPass * 12];
-  __shared__ uint32_t s_skip[kFramesPerPass / 32];
+  extern __shared__ float s_frames[];
+  float* const s_pose = s_frames;
+  uint32_t* const s_skip = reinterpret_cast<uint32_t*>(s_frames + 12 * P.K);
   for (int i = threadIdx.x; i < P.K * 12; i += blockDim.x) s_pose[i] = P.poses[i];
   __syncthreads();
   const int64_t n = (int64_t)P.X * P.Y * P.Z;
@@ -349,20 +470,18 @@ __device__ __forceinline__ void backward_voxel(const IntegrateParams& P, const f
                                                unsigned long long* acc) {
   const int M = P.max_w;
   int nobs = 0;
-  for (int k = 0; k < P.K; ++k) {
-    if (skipped(s_skip, k)) continue;
+  for_active_frames(s_skip, P.K, [&](int k) {
     int pix;
     float t, dtdd;
     if (observe<float>(P, s_pose + 12 * k, k, px, py, pz, &pix, &t, &dtdd)) ++nobs;
-  }
+  });
   const int jc = M - w0 > 0 ? M - w0 : 0;  // first observation that sees the cap
   const double cap_a = (double)M / (double)(M + 1);
   int o = 0;
-  for (int k = 0; k < P.K; ++k) {
-    if (skipped(s_skip, k)) continue;
+  for_active_frames(s_skip, P.K, [&](int k) {
     int pix;
     float t, dtdd;
-    if (!observe<float>(P, s_pose + 12 * k, k, px, py, pz, &pix, &t, &dtdd)) continue;
+    if (!observe<float>(P, s_pose + 12 * k, k, px, py, pz, &pix, &t, &dtdd)) return;
     const int wb = w0 + o < M ? w0 + o : M;
     double c = 1.0 / (double)(wb + 1);
     const int last_unc = (nobs - 1 < jc - 1) ? nobs - 1 : jc - 1;  // last uncapped index
@@ -376,14 +495,15 @@ __device__ __forceinline__ void backward_voxel(const IntegrateParams& P, const f
       atomicAdd(acc + (int64_t)k * P.W * P.H + pix, (unsigned long long)q);
     }
     ++o;
-  }
+  });
 }
 
 __global__ __launch_bounds__(256) void integrate_tsdf_backward_kernel(
     IntegrateParams P, const float* __restrict__ g, const int32_t* __restrict__ w0vol,
     const uint32_t* maxg, float max_dtdd, unsigned long long* acc) {
-  __shared__ float s_pose[kFramesPerPass * 12];
-  __shared__ uint32_t s_skip[kFramesPerPass / 32];
+  extern __shared__ float s_frames[];
+  float* const s_pose = s_frames;
+  uint32_t* const s_skip = reinterpret_cast<uint32_t*>(s_frames + 12 * P.K);
   for (int i = threadIdx.x; i < P.K * 12; i += blockDim.x) s_pose[i] = P.poses[i];
   __syncthreads();
   const int64_t n = (int64_t)P.X * P.Y * P.Z;
@@ -433,8 +553,9 @@ __device__ __forceinline__ int batch_setup(const BatchIntegrateParams& B, Integr
 
 __global__ __launch_bounds__(256) void integrate_tsdf_batched_kernel(BatchIntegrateParams B,
                                                                      float* tsdf, int32_t* wvol) {
-  __shared__ float s_pose[kFramesPerPass * 12];
-  __shared__ uint32_t s_skip[kFramesPerPass / 32];
+  extern __shared__ float s_frames[];
+  float* const s_pose = s_frames;
+  uint32_t* const s_skip = reinterpret_cast<uint32_t*>(s_frames + 12 * B.base.K);
   IntegrateParams P;
   int64_t vb, ib;
   batch_setup(B, P, s_pose, &vb, &ib);
@@ -465,8 +586,9 @@ __global__ __launch_bounds__(256) void integrate_tsdf_batched_kernel(BatchIntegr
 __global__ __launch_bounds__(256) void integrate_tsdf_backward_batched_kernel(
     BatchIntegrateParams B, const float* __restrict__ g, const int32_t* __restrict__ w0vol,
     const uint32_t* maxg, float max_dtdd, unsigned long long* acc) {
-  __shared__ float s_pose[kFramesPerPass * 12];
-  __shared__ uint32_t s_skip[kFramesPerPass / 32];
+  extern __shared__ float s_frames[];
+  float* const s_pose = s_frames;
+  uint32_t* const s_skip = reinterpret_cast<uint32_t*>(s_frames + 12 * B.base.K);
   IntegrateParams P;
   int64_t vb, ib;
   const int vi = batch_setup(B, P, s_pose, &vb, &ib);
@@ -661,7 +783,14 @@ int integrate_impl(void* d_vol, int X, int Y, int Z, float h, float trunc_neg, f
     P.K = K - k0 < kFramesPerPass ? K - k0 : kFramesPerPass;
     P.poses = d_poses + (int64_t)12 * k0;
     P.depth = d_depth + (int64_t)k0 * W * H;
-    hipLaunchKernelGGL(integrate_kernel<DT>, dim3(blocks_for(n)), dim3(256), 0, s, P);
+    if (P.K == 1) {
+      const int64_t per = 256 * kVoxelsPerThread1;
+      hipLaunchKernelGGL(integrate1_kernel<DT>, dim3((unsigned)((n + per - 1) / per)), dim3(256),
+                         0, s, P);
+    } else {
+      hipLaunchKernelGGL(integrate_kernel<DT>, dim3(blocks_for(n)), dim3(256),
+                         frames_lds_bytes(P.K), s, P);
+    }
     MCX_HIP(hipGetLastError());
   }
   return MCX_OK;
@@ -762,8 +891,8 @@ extern "C" int mcx_integrate_tsdf(float* d_tsdf, int32_t* d_weight, int X, int Y
     P.K = K - k0 < kFramesPerPass ? K - k0 : kFramesPerPass;
     P.poses = d_poses + (int64_t)12 * k0;
     P.depth = d_depth + (int64_t)k0 * W * H;
-    hipLaunchKernelGGL(integrate_tsdf_kernel, dim3(blocks_for(n)), dim3(256), 0, s, P, d_tsdf,
-                       d_weight);
+    hipLaunchKernelGGL(integrate_tsdf_kernel, dim3(blocks_for(n)), dim3(256),
+                       frames_lds_bytes(P.K), s, P, d_tsdf, d_weight);
     MCX_HIP(hipGetLastError());
   }
   return MCX_OK;
@@ -807,8 +936,8 @@ extern "C" int mcx_integrate_tsdf_backward(const float* d_grad_tsdf, const int32
   P.K = K;
   P.poses = d_poses;
   P.depth = d_depth;
-  hipLaunchKernelGGL(integrate_tsdf_backward_kernel, dim3(blocks_for(n)), dim3(256), 0, s, P,
-                     d_grad_tsdf, d_weight0, maxg, mdt, acc);
+  hipLaunchKernelGGL(integrate_tsdf_backward_kernel, dim3(blocks_for(n)), dim3(256),
+                     frames_lds_bytes(K), s, P, d_grad_tsdf, d_weight0, maxg, mdt, acc);
   MCX_HIP(hipGetLastError());
   hipLaunchKernelGGL(fixed_to_float_kernel, dim3(blocks_for(npix)), dim3(256), 0, s, acc, npix,
                      maxg, mdt, d_grad_depth);
@@ -848,8 +977,8 @@ extern "C" int mcx_integrate_tsdf_batched(const mcx_batch* plan, float* d_tsdf, 
   B.nvol = plan->nvol;
   B.from_init = from_init ? 1 : 0;
   B.init_value = init_value;
-  hipLaunchKernelGGL(integrate_tsdf_batched_kernel, dim3((unsigned)plan->blocks), dim3(256), 0, s,
-                     B, d_tsdf, d_weight);
+  hipLaunchKernelGGL(integrate_tsdf_batched_kernel, dim3((unsigned)plan->blocks), dim3(256),
+                     frames_lds_bytes(K), s, B, d_tsdf, d_weight);
   MCX_HIP(hipGetLastError());
   return MCX_OK;
 }
@@ -910,7 +1039,7 @@ extern "C" int mcx_integrate_tsdf_backward_batched(const mcx_batch* plan,
   B.from_init = 0;
   B.init_value = 0.0f;
   hipLaunchKernelGGL(integrate_tsdf_backward_batched_kernel, dim3((unsigned)plan->blocks),
-                     dim3(256), 0, s, B, d_grad_tsdf, d_weight0, mg, mdt, acc);
+                     dim3(256), frames_lds_bytes(K), s, B, d_grad_tsdf, d_weight0, mg, mdt, acc);
   MCX_HIP(hipGetLastError());
   hipLaunchKernelGGL(fixed_to_float_kernel, dim3(blocks_for(npix)), dim3(256), 0, s, acc, npix,
                      mg, mdt, d_grad_depth);
