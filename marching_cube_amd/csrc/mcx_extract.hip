@@ -1150,8 +1150,18 @@ __device__ __forceinline__ uint32_t meta_hi(uint64_t out, uint32_t z) {
 // axes the edge does not run on, the interpolated coordinate is exactly the
 // corner's node coordinate (fma(t, +0, p) = p), so only the edge axis is
 // interpolated, and with node_off only that axis divides.
-template <int ELEM>
-__device__ __forceinline__ int64_t record_vertex(const VolDesc& V, const uint8_t* s_slot,
+// Grid of a record's volume: dimensions and spacing (in registers; batched
+// launches fetch them from the record's lane).
+struct GridDims {
+  int X, Y, Z;
+  float h;
+};
+
+// vol_of() gives the volume's voxels (only called where the record's voxel
+// is not one of its cube's corners).
+template <int ELEM, typename VolOf>
+__device__ __forceinline__ int64_t record_vertex(const GridDims& V, VolOf vol_of,
+                                                 int16_t* ver_voxel, const uint8_t* s_slot,
                                                  const uint32_t* rc, int x, int y, int z, int q,
                                                  uint32_t* o, bool want_key,
                                                  const int8_t* node_off) {
@@ -1218,7 +1228,7 @@ __device__ __forceinline__ int64_t record_vertex(const VolDesc& V, const uint8_t
       if ((dx | dy | dz) <= 1u) {
         tw = rc[1 + corner_of((int)dx, (int)dy, (int)dz)];
       } else {
-        const uint32_t t = V.vol[lin()];
+        const uint32_t t = vol_of()[lin()];
         asm volatile("v_mov_b32 %0, %1" : "=v"(tw) : "v"(t));
       }
     }
@@ -1230,7 +1240,7 @@ __device__ __forceinline__ int64_t record_vertex(const VolDesc& V, const uint8_t
   o[4] = (uint32_t)vy;
   o[5] = (uint32_t)vz;
   o[6] = tw;
-  if (V.ver_voxel && inb) V.ver_voxel[lin()] = (int16_t)(tw & 0xffffu);
+  if (ver_voxel && inb) ver_voxel[lin()] = (int16_t)(tw & 0xffffu);
   return key;
 }
 
@@ -1242,11 +1252,14 @@ __device__ __forceinline__ int64_t record_vertex(const VolDesc& V, const uint8_t
 // produces one vertex of a round of 64 and stores its record fields at its
 // output index (the vertex's own slot; consecutive lanes mostly write
 // consecutive records).
+// Batched launches: lane r < nb holds the grid of record r's volume in `dl`
+// (each vertex lane fetches its record's with a lane shuffle, not from the
+// volume table in memory); s_vi[r] is the record's volume.
 template <int ELEM, bool BATCH>
 __device__ __forceinline__ void emit_block(const ExtractParams& P, const VolDesc& V,
                                            const uint8_t* s_slot, const uint8_t* s_nv,
                                            uint32_t* s_w, int nb, const int8_t* node_off,
-                                           const uint32_t* s_vi) {
+                                           const uint32_t* s_vi, const GridDims& dl) {
   const uint32_t* s_rec = s_w;
   uint8_t* s_map = reinterpret_cast<uint8_t*>(s_w + kPackedMap);
   uint32_t* s_ex = s_w + kPackedEx;
@@ -1263,16 +1276,28 @@ __device__ __forceinline__ void emit_block(const ExtractParams& P, const VolDesc
   uint32_t* vv = reinterpret_cast<uint32_t*>(P.vv);
   for (int vb = 0; vb < tot; vb += 64) {
     const int j = vb + lane;
+    const int r = j < tot ? (int)s_map[j] : 0;
+    GridDims g;
+    if constexpr (BATCH) {
+      // every lane takes part in the shuffles (lanes past the round read
+      // record 0's grid and do not use it)
+      g.X = __shfl(dl.X, r);
+      g.Y = __shfl(dl.Y, r);
+      g.Z = __shfl(dl.Z, r);
+      g.h = __shfl(dl.h, r);
+    } else {
+      g = GridDims{V.X, V.Y, V.Z, V.h};
+    }
     if (j < tot) {
-      const int r = s_map[j];
       const uint32_t* rc = s_rec + r * kRecWords;
       const uint32_t* mt = s_meta + 4 * r;
       const int slot = j - (int)s_ex[r];
       const int64_t out = (int64_t)(((uint64_t)(mt[1] & 0xffu) << 32) | mt[0]) + slot;
       if (out < P.cap) {
         uint32_t o[7];
-        const VolDesc& Vr = BATCH ? P.vols[s_vi[r]] : V;  // the record's volume
-        const int64_t key = record_vertex<ELEM>(Vr, s_slot, rc, (int)mt[2], (int)mt[3],
+        auto vol_of = [&]() { return BATCH ? P.vols[s_vi[r]].vol : V.vol; };
+        const int64_t key = record_vertex<ELEM>(g, vol_of, BATCH ? nullptr : V.ver_voxel, s_slot,
+                                                rc, (int)mt[2], (int)mt[3],
                                                 (int)(mt[1] >> 8), slot, o, P.ekey != nullptr,
                                                 node_off);
         if (vv) {
@@ -1347,7 +1372,9 @@ __device__ __forceinline__ void emit_staged_tile(const ExtractParams& P, const u
       mt[3] = y;
     }
     out += (int64_t)__builtin_amdgcn_readlane(vincl, 63);
-    if (n) emit_block<ELEM, BATCH>(P, V, s_slot, s_nv, s_w, n, node_off, s_vi);
+    if (n)
+      emit_block<ELEM, BATCH>(P, V, s_slot, s_nv, s_w, n, node_off, s_vi,
+                              GridDims{V.X, V.Y, V.Z, V.h});
   }
 }
 
@@ -1487,6 +1514,7 @@ void emit_units_kernel(ExtractParams P) {
     const uint32_t p0k = (uint32_t)__shfl((int)p0, kk);
     const uint32_t zk = (uint32_t)__shfl((int)z, kk);
     const uint32_t vik = BATCH ? (uint32_t)__shfl((int)vi, kk) : 0u;
+    GridDims dl{0, 0, 0, 0.0f};
     if (lane < R) {
 #pragma unroll
       for (int q = 0; q < kRecWords; ++q) s_w[lane * kRecWords + q] = rec[q];
@@ -1499,9 +1527,12 @@ void emit_units_kernel(ExtractParams P) {
       mt[1] = meta_hi(o, zk);
       mt[2] = p - y * (uint32_t)Vk.X;
       mt[3] = y;
-      if constexpr (BATCH) s_vi[lane] = vik;
+      if constexpr (BATCH) {
+        s_vi[lane] = vik;
+        dl = GridDims{Vk.X, Vk.Y, Vk.Z, Vk.h};
+      }
     }
-    emit_block<ELEM, BATCH>(P, V, s_slot, s_nv, s_w, R, node_off, s_vi);
+    emit_block<ELEM, BATCH>(P, V, s_slot, s_nv, s_w, R, node_off, s_vi, dl);
   }
 }
 
