@@ -1,12 +1,16 @@
 """Summaries of rocprofv3 output for the profiles/ folder.
 
     python -m marching_cube_amd.rocprof_summary stats  <kernel_stats.csv> [--match extract]
-    python -m marching_cube_amd.rocprof_summary window <kernel_trace.csv> --first F --count N
+    python -m marching_cube_amd.rocprof_summary window <kernel_trace.csv> --first F --count N [--match k ...]
+    python -m marching_cube_amd.rocprof_summary counters <pm_counter_collection.csv> [--match k ...]
     python -m marching_cube_amd.rocprof_summary traffic <fetch_dir> <write_dir> --out X.json
 
 `window` averages, per extraction kernel, the dispatches F .. F+N-1 in
 launch order (a bench's timed region: it skips the sizing, checking and
 warm-up calls before it and the calls after it).
+
+`counters` averages each counter of a ``rocprofv3 --pmc`` run per dispatch,
+per kernel (instruction counts: SQ_INSTS_VALU, SQ_INSTS_SALU, SQ_WAVES ...).
 
 `traffic` reads two `rocprofv3 --pmc FETCH_SIZE` / `--pmc WRITE_SIZE` runs of
 the same command (``pm_counter_collection.csv``), scales each counter by what
@@ -84,6 +88,22 @@ def _short(name):
     return n[:cut][:70]
 
 
+def counters(path, match=None):
+    """Per kernel: dispatches and the mean of every counter per dispatch."""
+    d = collections.defaultdict(lambda: collections.defaultdict(list))
+    with open(path) as f:
+        for r in csv.DictReader(f):
+            name = r["Kernel_Name"]
+            if match and not any(m in name for m in match):
+                continue
+            d[name][r["Counter_Name"]].append(float(r["Counter_Value"]))
+    out = []
+    for name, cs in d.items():
+        n = max(len(v) for v in cs.values())
+        out.append((name, n, {c: sum(v) / len(v) for c, v in cs.items()}))
+    return out
+
+
 def traffic(fetch_dir, write_dir):
     fetch = _per_kernel(os.path.join(fetch_dir, "pm_counter_collection.csv"), "FETCH_SIZE")
     write = _per_kernel(os.path.join(write_dir, "pm_counter_collection.csv"), "WRITE_SIZE")
@@ -125,6 +145,10 @@ def main(argv=None):
     w.add_argument("csv")
     w.add_argument("--first", type=int, required=True)
     w.add_argument("--count", type=int, required=True)
+    w.add_argument("--match", nargs="*", help="kernel name parts (default: the extraction kernels)")
+    c = sub.add_parser("counters")
+    c.add_argument("csv")
+    c.add_argument("--match", nargs="*")
     b = sub.add_parser("traffic")
     b.add_argument("fetch_dir")
     b.add_argument("write_dir")
@@ -138,7 +162,7 @@ def main(argv=None):
             print(f"| {_short(name)} | {calls} | {avg:.2f} | {tot:.1f} |")
         return 0
     if args.cmd == "window":
-        rows = window(args.csv, args.first, args.count)
+        rows = window(args.csv, args.first, args.count, tuple(args.match or EXTRACT_KERNELS))
         print("| kernel | dispatches in run | in window | avg us | median us | min us | max us |")
         print("|---|---|---|---|---|---|---|")
         tot = 0.0
@@ -146,6 +170,15 @@ def main(argv=None):
             print(f"| {_short(name)} | {n} | {k} | {avg:.2f} | {med:.2f} | {lo:.2f} | {hi:.2f} |")
             tot += avg
         print(f"\nsum of the window averages: {tot:.2f} us")
+        return 0
+    if args.cmd == "counters":
+        rows = counters(args.csv, args.match)
+        names = sorted({c for _, _, cs in rows for c in cs})
+        print("| kernel | dispatches | " + " | ".join(f"{c} per dispatch" for c in names) + " |")
+        print("|---|---|" + "---|" * len(names))
+        for name, n, cs in rows:
+            vals = " | ".join(f"{cs[c]:.0f}" if c in cs else "" for c in names)
+            print(f"| {_short(name)} | {n} | {vals} |")
         return 0
     res = traffic(args.fetch_dir, args.write_dir)
     res.update(json.loads(args.meta))

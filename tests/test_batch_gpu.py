@@ -113,3 +113,39 @@ def test_batch_staged_path(dev, records):
     out = vv.cpu().numpy()
     for b, ref in enumerate(refs):
         assert out[vb[b]:vb[b + 1]].tobytes() == ref.tobytes(), b
+
+
+@pytest.mark.parametrize("slab", [16, 64])
+def test_batch_at_shape_batch_sizes(dev, slab):
+    # the size range of the batched-shapes workload (sides 16..128 per axis),
+    # plus a full 128^3 sphere and a 33 x 127 x 125 noise volume (rows that
+    # end inside a lane, dense surface): vertex_voxel records and the int3
+    # map of every volume against the oracle, at two plane counts per column
+    from marching_cube_amd import autotune
+    print(f"[case] batch of 242 volumes, sides 16..128, slab {slab}", flush=True)
+    vols, hs = synth.batch_shapes(240, lo=16, hi=128, seed=41)
+    sph, hsph = synth.sphere_volume(128)
+    vols.append(sph)
+    hs = np.append(hs, np.float32(hsph))
+    vols.append(synth.noise_volume((33, 127, 125), seed=5, p_zero_w=0.02))
+    hs = np.append(hs, np.float32(1.0 / 128))
+    assert max(max(v.shape) for v in vols) == 128
+    tens = [torch.from_numpy(v.view(np.int32)).to(dev) for v in vols]
+    saved = autotune.get()
+    try:
+        autotune.set(batch_slab=slab)
+        plan = batch.BatchPlan(tens, hs)
+    finally:
+        autotune.set(saved["columns_per_cu"], saved["emit_groups_per_cu"], saved["batch_slab"])
+    n, vbase, out = plan.extract(want=("vv", "v2vox"))
+    plan.check()
+    vb = vbase.cpu().numpy()
+    vv = out["vv"].cpu().numpy()
+    v2 = out["v2vox"].cpu().numpy()
+    for b, (v, h) in enumerate(zip(vols, hs)):
+        ref = O.extract(v, float(h), want=("vv", "v2vox"))
+        seg = slice(vb[b], vb[b + 1])
+        assert vb[b + 1] - vb[b] == ref["n"], b
+        assert vv[seg].tobytes() == ref["vv"].view(np.int32).reshape(-1, 7).tobytes(), b
+        assert v2[seg].tobytes() == ref["v2vox"].tobytes(), b
+    assert n == vb[-1] > 1_000_000
