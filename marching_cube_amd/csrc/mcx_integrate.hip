@@ -179,12 +179,16 @@ __device__ __forceinline__ void for_active_frames(const uint32_t* s_skip, int K,
 // costs no memory traffic.
 // One packed update (the oracle's update_voxel after its lookups): the new
 // (tsdf, weight) word from the current one and the observation t.
+// A voxel at +1 observed at +1 (free space seen again) stays at +1 with no
+// division: told = 32767 / 32767 = 1, fma(1, wi, 1) = wi + 1 and
+// (wi + 1) / (wi + 1) = 1 exactly (integers below 2^24), 1 * 32767 = 32767.
 __device__ __forceinline__ uint32_t packed_update(uint32_t cur, float tsdf, int max_w) {
   const int16_t ti = (int16_t)(cur & 0xffffu);
   const int wi = (int)(cur >> 16);
-  const float told = (float)ti / 32767.0f;
-  const float tnew = __fmaf_rn(told, (float)wi, tsdf) / (float)(wi + 1);
   const int wnew = wi + 1 < max_w ? wi + 1 : max_w;
+  if (ti == 32767 && tsdf == 1.0f) return 0x7fffu | ((uint32_t)wnew << 16);
+  const float told = tsdf_fraction(ti);  // ti / 32767, correctly rounded
+  const float tnew = __fmaf_rn(told, (float)wi, tsdf) / (float)(wi + 1);
   int fixedp = (int)(tnew * 32767.0f);
   fixedp = fixedp > 32767 ? 32767 : (fixedp < -32767 ? -32767 : fixedp);
   return (uint32_t)(uint16_t)(int16_t)fixedp | ((uint32_t)wnew << 16);
@@ -192,8 +196,11 @@ __device__ __forceinline__ uint32_t packed_update(uint32_t cur, float tsdf, int 
 
 // The truncated observation of a signed distance: sdf / max_t in front of the
 // surface (at most 1), sdf / min_t behind it (at least -1 once sdf >= -min_t
-// was tested): one division by the selected distance.
+// was tested): one division by the selected distance.  At sdf >= max_t (> 0,
+// checked by every entry point) the quotient is at least 1 before rounding,
+// so at least 1 after it (rounding is monotonic): 1 without the division.
 __device__ __forceinline__ float truncated(float sdf, float trunc_pos, float trunc_neg) {
+  if (sdf >= trunc_pos) return 1.0f;
   return fminf(sdf / (sdf > 0.0f ? trunc_pos : trunc_neg), 1.0f);
 }
 
@@ -366,8 +373,14 @@ __device__ __forceinline__ bool observe(const IntegrateParams& P, const float* R
   const float qn = sqrtf(__fmaf_rn(qz, qz, __fmaf_rn(qy, qy, qx * qx)));
   const float sdf = zval - qn;
   if (!(sdf >= -P.trunc)) return false;
-  const float ts = sdf / P.trunc;
   *pix_out = pix;
+  if (sdf >= P.trunc) {
+    // sdf / trunc >= 1 before rounding, so after it: clamped to 1, no gradient
+    *t_out = 1.0f;
+    *dtdd_out = 0.0f;
+    return true;
+  }
+  const float ts = sdf / P.trunc;
   *t_out = fminf(fmaxf(ts, -1.0f), 1.0f);
   *dtdd_out = (ts > -1.0f && ts < 1.0f) ? P.depth_scale * rf / P.trunc : 0.0f;
   return true;
@@ -392,7 +405,11 @@ __device__ __forceinline__ void integrate_tsdf_voxel(const IntegrateParams& P, c
       have = true;
     }
     *changed = true;
-    f = __fmaf_rn(f, (float)w, t) / (float)(w + 1);
+    // f = 1 observed at 1 stays 1: fma(1, w, 1) = w + 1 = (float)(w + 1)
+    // exactly for 0 <= w < 2^24 (a weight read from the caller's volume may
+    // lie outside), and the quotient of equal values is 1
+    if (!(f == 1.0f && t == 1.0f && (uint32_t)w < (1u << 24)))
+      f = __fmaf_rn(f, (float)w, t) / (float)(w + 1);
     w = w + 1 < P.max_w ? w + 1 : P.max_w;
   });
 }

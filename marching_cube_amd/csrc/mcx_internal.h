@@ -24,6 +24,16 @@ __device__ __forceinline__ float tsdf_unit(int s) {
   return __fmaf_rn(__fmaf_rn(-q, 32766.0f, x), r, q);
 }
 
+// s / 32767 correctly rounded for every 16-bit s (the stored tsdf of the
+// packed integration as a fraction), in the same form as tsdf_unit (equal to
+// IEEE division on all 65536 values; checked by the same device test).
+__device__ __forceinline__ float tsdf_fraction(int s) {
+  constexpr float r = 1.0f / 32767.0f;
+  const float x = (float)s;
+  const float q = x * r;
+  return __fmaf_rn(__fmaf_rn(-q, 32767.0f, x), r, q);
+}
+
 // num / den correctly rounded, for the marching-cubes interpolation weight
 // of two short2 corners of opposite classification (num = 0 - fa, den = fb -
 // fa + 1e-15f, fa = tsdf_unit(ka), fb = tsdf_unit(kb), ka < 0 <= kb or

@@ -2,7 +2,7 @@
 memory, calling libmcx through the C ABI.
 
   cu_voxel_new(v, size, min_t, max_t, grid_s)   cu_kinect_fusion_v1.cu:381-398
-      -> voxel_new(size, trunc, grid_s)   allocates and fills (-32766, 0)
+      -> voxel_new(size, min_t, grid_s, max_t=...)   allocates and fills (-32766, 0)
   _cu_voxel_volumetric_integration(v, ci, ce, size, depth, ...)   :534-549
       -> integrate(volume, intr, poses, depth_frames)  K frames in one sweep
   _cu_voxel_extract_depth_and_normal_map(v, ci, ce, size, d, n, c)   :972-986
@@ -34,36 +34,54 @@ def voxel_init(vol):
     return vol
 
 
-def voxel_new(size, trunc, grid_s, device="cuda"):
-    """cu_voxel_new: a (Z, Y, X) packed short2 volume initialised to (-32766, 0)."""
+def voxel_new(size, trunc, grid_s, device="cuda", max_t=None):
+    """cu_voxel_new: a (Z, Y, X) packed short2 volume initialised to (-32766, 0),
+    truncated at min_t = trunc behind the surface and max_t (default trunc)
+    in front of it."""
     X, Y, Z = size
     vol = torch.empty((Z, Y, X), dtype=torch.int32, device=device)
     voxel_init(vol)
-    return CuVoxel(data=vol, grid_s=grid_s, min_t=trunc, max_t=trunc)
+    return CuVoxel(data=vol, grid_s=grid_s, min_t=trunc, max_t=trunc if max_t is None else max_t)
 
 
-def integrate(volume, intr, poses, depth, *, trunc, max_w=128, depth_scale=0.001, h=None):
-    """Fuse K depth frames (uint16 (K, H, W) device tensor) into `volume`
-    (CuVoxel or packed int32 tensor) in one sweep.  poses: (K, 12) float32."""
+def integrate(volume, intr, poses, depth, *, trunc=None, min_t=None, max_t=None, max_w=128,
+              depth_scale=0.001, h=None):
+    """Fuse K depth frames ((K, H, W) device tensor: uint16, or float32 in
+    units of depth_scale metres) into `volume` (CuVoxel or packed int32
+    tensor) in one sweep.  poses: (K, 12) float32, camera-to-world.
+    Truncation: min_t behind the surface and max_t in front of it, as the
+    reference's cu_voxel fields (cu_kinect_fusion_v1.cu:455-463); `trunc`
+    sets both, else they come from the arguments or from the CuVoxel."""
     if isinstance(volume, CuVoxel):
         vol, h = volume.data, volume.grid_s if h is None else h
+        min_t = volume.min_t if min_t is None else min_t
+        max_t = volume.max_t if max_t is None else max_t
     else:
         vol = volume
+    if trunc is not None:
+        min_t = max_t = trunc
     if h is None:
         raise ValueError("grid size h is required")
+    if min_t is None or max_t is None:
+        raise ValueError("truncation (trunc, or min_t and max_t) is required")
     Z, Y, X = vol.shape
-    if depth.dtype != torch.uint16 and depth.dtype != torch.int16:
-        raise ValueError("depth frames must be uint16 (K, H, W)")
+    if depth.dtype in (torch.uint16, torch.int16):
+        kind = _capi.DEPTH_U16
+    elif depth.dtype == torch.float32:
+        kind = _capi.DEPTH_F32
+    else:
+        raise ValueError("depth frames must be uint16 or float32 (K, H, W)")
     K, H, W = depth.shape
     poses = torch.as_tensor(poses, dtype=torch.float32).reshape(K, 12).to(vol.device).contiguous()
     lib = _lib()
     ws = torch.empty(max(1, lib.mcx_integrate_workspace_size(W, H)), dtype=torch.uint8,
                      device=vol.device)
     hi, hp = _host_f32(intr)
-    _capi.check(lib.mcx_integrate(_ptr(vol), X, Y, Z, float(h), float(trunc), int(max_w), hp,
-                                  _ptr(poses), _ptr(depth.contiguous()), float(depth_scale), W, H,
-                                  K, _ptr(ws), ws.numel(), _stream_handle(vol.device)),
-                "mcx_integrate")
+    _capi.check(lib.mcx_integrate_ex(_ptr(vol), X, Y, Z, float(h), float(min_t), float(max_t),
+                                     int(max_w), hp, _ptr(poses), _ptr(depth.contiguous()), kind,
+                                     float(depth_scale), W, H, K, _ptr(ws), ws.numel(),
+                                     _stream_handle(vol.device)),
+                "mcx_integrate_ex")
     del hi
     return volume
 

@@ -1,13 +1,16 @@
 // Device check of the exact fp32 forms in mcx_internal.h (built by
 // marching_cube_amd/build.py, run by tests/test_exact_div_gpu.py):
 //  * tsdf_unit(k) == (float)k / 32766.0f (IEEE division) for all 65536 k;
+//  * tsdf_fraction(k) == (float)k / 32767.0f for all 65536 k (the packed
+//    integration's stored tsdf);
 //  * div_weight_short2(0 - fa, fb - fa + 1e-15f) == the IEEE quotient for all
 //    2^31 pairs of opposite classification (ka < 0 <= kb and kb < 0 <= ka),
 //    the operands of the marching-cubes interpolation weight
 //    (cu_marching_cubes_v1.cu:259) for short2 volumes.
 // Compiled with the library's flags (-fhip-fp32-correctly-rounded-divide-sqrt,
 // -ffp-contract=off), so `/` is the IEEE division the reference computes.
-// Prints "unit_mismatch U weight_mismatch W pairs P"; exits 0 when run.
+// Prints "unit_mismatch U weight_mismatch W pairs P fraction_mismatch F";
+// exits 0 when run.
 #include <hip/hip_runtime.h>
 
 #include <cstdio>
@@ -21,6 +24,8 @@ __global__ void check_unit(unsigned long long* cnt) {
   if (k > 32767) return;
   const float ieee = (float)k / 32766.0f;
   if (__float_as_uint(mcx::tsdf_unit(k)) != __float_as_uint(ieee)) atomicAdd(&cnt[0], 1ull);
+  const float ieee7 = (float)k / 32767.0f;
+  if (__float_as_uint(mcx::tsdf_fraction(k)) != __float_as_uint(ieee7)) atomicAdd(&cnt[3], 1ull);
 }
 
 __global__ void check_weight(unsigned long long* cnt) {
@@ -45,13 +50,14 @@ __global__ void check_weight(unsigned long long* cnt) {
 
 int main() {
   unsigned long long* d = nullptr;
-  if (hipMalloc(&d, 3 * sizeof(unsigned long long)) != hipSuccess) return 2;
-  if (hipMemset(d, 0, 3 * sizeof(unsigned long long)) != hipSuccess) return 2;
+  if (hipMalloc(&d, 4 * sizeof(unsigned long long)) != hipSuccess) return 2;
+  if (hipMemset(d, 0, 4 * sizeof(unsigned long long)) != hipSuccess) return 2;
   hipLaunchKernelGGL(check_unit, dim3(256), dim3(256), 0, 0, d);
   hipLaunchKernelGGL(check_weight, dim3(8192), dim3(256), 0, 0, d);
   if (hipDeviceSynchronize() != hipSuccess) return 3;
-  unsigned long long h[3] = {};
+  unsigned long long h[4] = {};
   if (hipMemcpy(h, d, sizeof(h), hipMemcpyDeviceToHost) != hipSuccess) return 3;
-  std::printf("unit_mismatch %llu weight_mismatch %llu pairs %llu\n", h[0], h[1], h[2]);
+  std::printf("unit_mismatch %llu weight_mismatch %llu pairs %llu fraction_mismatch %llu\n", h[0],
+              h[1], h[2], h[3]);
   return hipFree(d) == hipSuccess ? 0 : 4;
 }

@@ -1,7 +1,8 @@
-"""The exact fp32 forms the extraction kernels use for short2 volumes
-(mcx_internal.h: tsdf_unit, div_weight_short2) equal IEEE division on their
-whole input domain, checked on the device by tests/cpp/exact_div_check
-(65536 unpacks, 2^31 interpolation-weight pairs)."""
+"""The exact fp32 forms the extraction and integration kernels use for short2
+volumes (mcx_internal.h: tsdf_unit, div_weight_short2, tsdf_fraction) equal
+IEEE division on their whole input domain, checked on the device by
+tests/cpp/exact_div_check (65536 unpacks by 32766, 2^31 interpolation-weight
+pairs, 65536 stored-tsdf fractions by 32767)."""
 import os
 import subprocess
 
@@ -23,3 +24,4 @@ def test_exact_forms_equal_ieee_division():
     assert vals["pairs"] == 1 << 31
     assert vals["unit_mismatch"] == 0
     assert vals["weight_mismatch"] == 0
+    assert vals["fraction_mismatch"] == 0

@@ -44,6 +44,38 @@ def test_integrate_bit_exact(dev):
     assert got.tobytes() == ref.tobytes()
 
 
+def test_integrate_split_truncation_float_frames_per_frame_calls(dev):
+    # min_t 0.08 behind the surface, max_t 0.03 in front (the volume's own
+    # fields); uint16 frames in one sweep, and float frames in metres one call
+    # per frame (the reference's call pattern): both equal the oracle's
+    # two-distance integration and each other
+    print("[case] integrate 40^3 x 8 frames, min_t 0.08, max_t 0.03, u16 fused and f32 per frame",
+          flush=True)
+    intr, objs, poses, depth, h, dim = small_scene(dev, seed=13)
+    K = depth.shape[0]
+    intr_np = np.array(intr, np.float32)
+    vox = fusion.voxel_new((dim, dim, dim), 0.08, h, device=dev, max_t=0.03)
+    assert (vox.min_t, vox.max_t) == (0.08, 0.03)
+    fusion.integrate(vox, intr, poses, depth, max_w=128)
+    ref = O.volume_init((dim, dim, dim))
+    O.integrate_ex(ref, h, 0.08, 0.03, 128, intr_np, poses, depth.cpu().numpy())
+    got = vox.data.cpu().numpy().view(np.uint32)
+    assert got.tobytes() == ref.tobytes()
+    sym = O.volume_init((dim, dim, dim))
+    O.integrate_ex(sym, h, 0.03, 0.03, 128, intr_np, poses, depth.cpu().numpy())
+    assert (sym != ref).mean() > 0.001  # the two distances change the result
+    metres = (depth.to(torch.float32) * 0.001).contiguous()
+    vox2 = fusion.voxel_new((dim, dim, dim), 0.08, h, device=dev, max_t=0.03)
+    for k in range(K):
+        fusion.integrate(vox2, intr, poses[k:k + 1], metres[k:k + 1], depth_scale=1.0, max_w=128)
+    ref2 = O.volume_init((dim, dim, dim))
+    O.integrate_ex(ref2, h, 0.08, 0.03, 128, intr_np, poses, metres.cpu().numpy(),
+                   depth_scale=1.0)
+    got2 = vox2.data.cpu().numpy().view(np.uint32)
+    assert got2.tobytes() == ref2.tobytes()
+    assert got2.tobytes() == got.tobytes()
+
+
 def test_integrate_then_extract_matches_oracle(dev):
     print("[case] integrate + extract 40^3", flush=True)
     intr, objs, poses, depth, h, dim = small_scene(dev, K=12, seed=9)
