@@ -559,7 +559,8 @@ def main():
     args = parse()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
-              "(the multi-GPU sharding of the path is described in DESIGN.md)", file=sys.stderr)
+              "(the path would shard by z-slabs and by volume, which is not built)",
+              file=sys.stderr)
         return 2
     import torch
     from marching_cube_amd import fusion, mesh

@@ -913,16 +913,21 @@ void count4_kernel(ExtractParams P) {
 __global__ __launch_bounds__(256) void scan_kernel(ExtractParams P) {
   __shared__ int64_t s_wave[3][4];
   __shared__ int64_t s_excl[3];
-  __shared__ uint32_t s_ticket;
+  __shared__ uint32_t s_ticket, s_nz;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   if (tid == 0) s_ticket = atomicAdd(&P.hdr->scan_ticket, 1u);
   __syncthreads();
   const int64_t blk = s_ticket;
+  // The list length.  The last ticket resets this counter for the next call
+  // once every other block has published, so the read must be complete
+  // before the block's first publication: one thread reads it beside its
+  // count loads and writes it to LDS before the barrier that precedes the
+  // publication (the write waits for the read; that barrier waits for the
+  // count loads anyway), and the block uses the LDS copy.
+  uint32_t nz_read = 0;
+  if (tid == 0) nz_read = P.hdr->nz_count;
   const bool last_blk = blk == (int64_t)gridDim.x - 1;
   const int64_t i0 = blk * kScanItems + (int64_t)tid * kScanPer;
-  // read before this block publishes anything: the last block resets the
-  // counter once every block has published
-  const uint32_t nz = P.hdr->nz_count;
 
   // ---- tiles: one 128-bit load of the thread's 4 counts (i0 is a multiple
   // of 4, the array is 256-byte aligned), element loads at the end
@@ -943,7 +948,9 @@ __global__ __launch_bounds__(256) void scan_kernel(ExtractParams P) {
   for (int k = 0; k < kScanPer; ++k) st += v[k];
   const uint64_t incl0 = wave_incl_scan((uint64_t)st);
   if (lane == 63) s_wave[0][wave] = (int64_t)incl0;
+  if (tid == 0) s_nz = nz_read;
   __syncthreads();
+  const uint32_t nz = s_nz;
   int64_t texcl0 = (int64_t)incl0 - st, total0 = 0;
 #pragma unroll
   for (int w = 0; w < 4; ++w) {

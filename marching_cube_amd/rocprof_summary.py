@@ -130,7 +130,7 @@ def traffic(fetch_dir, write_dir):
         wb = sorted(wv)[len(wv) // 2] * 1024.0 * w_scale if wv else 0.0
         out["kernels"][short] = {"dispatches": len(fv), "fetch_bytes": round(fb),
                                  "write_bytes": round(wb)}
-        total += fb + wb
+        total += round(fb) + round(wb)  # the sum of the bytes written out
     out["hbm_bytes_per_launch"] = round(total)
     return out
 
