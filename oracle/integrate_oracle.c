@@ -15,8 +15,8 @@
  * with round-half-even and the [0, W-2) x [0, H-2) pixel window (:447-449),
  * the q.z < FLT_MIN and depth <= FLT_MIN skips (:446, :455), the two
  * truncation distances (:455-463) and init to (-32766, 0) (:375-376).
- * Changed (documented in DESIGN.md): voxel centres at (i + 0.5) h so that
- * integration and extraction (getNodeCoo) agree; integer weights.
+ * Changed on purpose: voxel centres at (i + 0.5) h so that integration and
+ * extraction (getNodeCoo) agree; integer weights.
  *
  * Every float operation is written out (fmaf where a fused multiply-add is
  * meant) so the GPU kernel can reproduce the sequence bit for bit.
