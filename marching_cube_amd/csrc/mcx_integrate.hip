@@ -204,6 +204,28 @@ __device__ __forceinline__ float truncated(float sdf, float trunc_pos, float tru
   return fminf(sdf / (sdf > 0.0f ? trunc_pos : trunc_neg), 1.0f);
 }
 
+// Coordinates of voxel i of an X x Y x Z volume (x fastest).  Below 2^32
+// voxels: two 32-bit magic-number divisions (i / X, then the row / Y), exact
+// for every 32-bit numerator; larger volumes divide in 64 bits.
+__device__ __forceinline__ void voxel_coords(const IntegrateParams& P, int64_t i, int* x, int* y,
+                                             int* z) {
+  if (i < ((int64_t)1 << 32)) {
+    const uint32_t row = P.divX.div((uint32_t)i);
+    const uint32_t zz = P.divY.div(row);
+    *x = (int)((uint32_t)i - row * (uint32_t)P.X);
+    *y = (int)(row - zz * (uint32_t)P.Y);
+    *z = (int)zz;
+  } else {
+    const uint32_t xy = (uint32_t)((int64_t)P.X * P.Y);
+    const int zz = (int)(i / xy);
+    const uint32_t rem = (uint32_t)(i - (int64_t)zz * xy);
+    const uint32_t yy = P.divX.div(rem);
+    *x = (int)(rem - yy * (uint32_t)P.X);
+    *y = (int)yy;
+    *z = zz;
+  }
+}
+
 // Packed short2 update of voxel i by frames [0, K) (the oracle's
 // update_voxel).  The voxel is read at its first update and written only if
 // some frame updated it: outside every frustum and behind every surface it
@@ -262,13 +284,10 @@ __global__ __launch_bounds__(256) void integrate_kernel(IntegrateParams P) {
   frame_skip_flags(s_pose, P.K, P, ib, min(ib + (int64_t)blockDim.x, n) - 1, s_skip);
   const int64_t i = ib + threadIdx.x;
   if (i >= n) return;
-  const uint32_t xy = (uint32_t)((int64_t)P.X * P.Y);
-  const int z = (int)(i / xy);
-  const uint32_t rem = (uint32_t)(i - (int64_t)z * xy);
-  const uint32_t y = P.divX.div(rem);
-  const int x = (int)(rem - y * (uint32_t)P.X);
+  int x, y, z;
+  voxel_coords(P, i, &x, &y, &z);
   const float px = ((float)x + 0.5f) * P.h;
-  const float py = ((float)(int)y + 0.5f) * P.h;
+  const float py = ((float)y + 0.5f) * P.h;
   const float pz = ((float)z + 0.5f) * P.h;
   integrate_voxel<DT>(P, s_pose, s_skip, px, py, pz, i);
 }
@@ -291,7 +310,6 @@ __global__ __launch_bounds__(256) void integrate1_kernel(IntegrateParams P) {
   frame_skip_flags(s_pose, 1, P, ib, min(ib + 256 * kVoxelsPerThread1, n) - 1, s_skip);
   if (s_skip[0] & 1u) return;  // no voxel of the workgroup can see the frame
   const float* R = s_pose;
-  const uint32_t xy = (uint32_t)((int64_t)P.X * P.Y);
   int pix[kVoxelsPerThread1];
   float qn[kVoxelsPerThread1];
   int64_t vi[kVoxelsPerThread1];
@@ -302,12 +320,10 @@ __global__ __launch_bounds__(256) void integrate1_kernel(IntegrateParams P) {
     pix[j] = -1;
     qn[j] = 0.0f;
     if (i >= n) continue;
-    const int z = (int)(i / xy);
-    const uint32_t rem = (uint32_t)(i - (int64_t)z * xy);
-    const uint32_t y = P.divX.div(rem);
-    const int x = (int)(rem - y * (uint32_t)P.X);
+    int x, y, z;
+    voxel_coords(P, i, &x, &y, &z);
     const float px = ((float)x + 0.5f) * P.h;
-    const float py = ((float)(int)y + 0.5f) * P.h;
+    const float py = ((float)y + 0.5f) * P.h;
     const float pz = ((float)z + 0.5f) * P.h;
     const float dx = px - R[9], dy = py - R[10], dz = pz - R[11];
     const float qx = __fmaf_rn(R[6], dz, __fmaf_rn(R[3], dy, R[0] * dx));
@@ -416,13 +432,10 @@ __device__ __forceinline__ void integrate_tsdf_voxel(const IntegrateParams& P, c
 
 __device__ __forceinline__ void voxel_centre(const IntegrateParams& P, int64_t i, float* px,
                                              float* py, float* pz) {
-  const uint32_t xy = (uint32_t)((int64_t)P.X * P.Y);
-  const int z = (int)(i / xy);
-  const uint32_t rem = (uint32_t)(i - (int64_t)z * xy);
-  const uint32_t y = P.divX.div(rem);
-  const int x = (int)(rem - y * (uint32_t)P.X);
+  int x, y, z;
+  voxel_coords(P, i, &x, &y, &z);
   *px = ((float)x + 0.5f) * P.h;
-  *py = ((float)(int)y + 0.5f) * P.h;
+  *py = ((float)y + 0.5f) * P.h;
   *pz = ((float)z + 0.5f) * P.h;
 }
 
@@ -558,6 +571,7 @@ __device__ __forceinline__ int batch_setup(const BatchIntegrateParams& B, Integr
   P.Z = V.Z;
   P.h = V.h;
   P.divX = V.divX;
+  P.divY = FastDiv((uint32_t)V.Y);
   const int64_t frames = (int64_t)P.K * P.W * P.H;
   P.depth = static_cast<const float*>(B.base.depth) + (int64_t)vi * frames;
   const float* poses = B.base.poses + (int64_t)vi * P.K * 12;
