@@ -398,7 +398,9 @@ int mcx_synth_depth(uint16_t* d_depth, int W, int H, int K, const float* intr,
  * set them before launching, from one thread).  Outputs do not depend on
  * them; only the decomposition of the work does.  marching_cube_amd.autotune
  * times candidates on the caller's volume and sets the fastest. */
-#define MCX_TUNE_COLUMNS_PER_CU 0     /* count kernel columns (waves) per CU, single volume; 56 */
+#define MCX_TUNE_COLUMNS_PER_CU 0     /* count kernel columns (waves) per CU, single volume, at
+                                         most (smaller volumes: about 8192 cubes a column, at
+                                         least 4 per CU); 56 */
 #define MCX_TUNE_EMIT_GROUPS_PER_CU 1 /* emit kernel persistent workgroups per CU; 16 */
 #define MCX_TUNE_BATCH_SLAB 2         /* planes per column of batched plans (at creation); 64 */
 int mcx_set_tuning(int key, int value);

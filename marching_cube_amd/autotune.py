@@ -4,8 +4,10 @@ The outputs never depend on the launch shape; only the time does.  The
 tunables (mcx_set_tuning):
 
 * columns_per_cu  -- count-kernel columns (one wave each) per CU for a single
-  volume: more columns give several rounds of waves per SIMD (balance where
-  the surface is uneven) at the price of more slab-boundary planes read twice.
+  volume, at most: more columns give several rounds of waves per SIMD
+  (balance where the surface is uneven) at the price of more slab-boundary
+  planes read twice.  Smaller volumes take fewer (about 8192 cubes a column,
+  at least 4 per CU), so at 256^3 and below the larger candidates coincide.
 * emit_groups_per_cu -- persistent emit workgroups per CU.
 * batch_slab -- planes per column of batched plans.
 

@@ -1966,7 +1966,16 @@ int prepare_single(ExtractParams& P, const void* d_vol, int X, int Y, int Z, flo
   P.nvol = 1;
   P.n_tiles = n_tiles;
   const int64_t R = P.vol0.R;
-  const int64_t target = (int64_t)cu_count() * g_tuning.columns_per_cu;
+  int64_t target = (int64_t)cu_count() * g_tuning.columns_per_cu;
+  // Smaller volumes take fewer, longer columns: about 8192 cubes a column,
+  // at least 4 columns per CU.  A column reads one plane more than its slab
+  // and fills its plane pipeline before its first step, costs that slabs of
+  // 1-3 planes (what the per-CU target alone gives at 128^3 and 256^3) pay
+  // every few planes: at 256^3 the fp32 forward took 0.115 ms with 3-plane
+  // slabs and 0.079 ms with 16; 128^3 0.039 ms with 1 and 0.032 with 4.
+  const int64_t by_size =
+      std::max((int64_t)cu_count() * 4, cubes_of(X, Y, Z) / 8192);
+  if (by_size < target) target = by_size;
   int64_t S = (target + R - 1) / R;
   if (S > Z - 1) S = Z - 1;
   P.slab = (int)((Z - 1 + S - 1) / S);
