@@ -2265,7 +2265,8 @@ extern "C" int mcx_batch_create(mcx_batch** out, int nvol, const int* dims_xyz, 
     if (!size_ok(X, Y, Z)) return MCX_ERR_UNSUPPORTED;
     vols[b] = make_vol(d_vols[b], X, Y, Z, h[b], nullptr, tb);
     const int64_t nv = (int64_t)X * Y * Z;
-    bvol[b] = BatchVol{X, Y, Z, h[b], voxels, blocks, gblocks, FastDiv((uint32_t)X)};
+    bvol[b] = BatchVol{X, Y, Z, h[b], voxels, blocks, gblocks, FastDiv((uint32_t)X),
+                       FastDiv((uint32_t)Y)};
     voxels += nv;
     blocks += (nv + 255) / 256;
     gblocks += ((int64_t)X * Y + kGatherSpan - 1) / kGatherSpan * Z;

@@ -571,7 +571,7 @@ __device__ __forceinline__ int batch_setup(const BatchIntegrateParams& B, Integr
   P.Z = V.Z;
   P.h = V.h;
   P.divX = V.divX;
-  P.divY = FastDiv((uint32_t)V.Y);
+  P.divY = V.divY;
   const int64_t frames = (int64_t)P.K * P.W * P.H;
   P.depth = static_cast<const float*>(B.base.depth) + (int64_t)vi * frames;
   const float* poses = B.base.poses + (int64_t)vi * P.K * 12;

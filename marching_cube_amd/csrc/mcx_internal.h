@@ -279,7 +279,7 @@ struct BatchVol {
   int64_t vox_begin;
   int64_t blk_begin;
   int64_t gblk_begin;
-  FastDiv divX;
+  FastDiv divX, divY;
 };
 
 struct VolDesc;  // extraction descriptors (mcx_extract.hip)
