@@ -12,8 +12,9 @@ cannot be served from the 256 MB MALL.
 
 Prints one JSON line: value = Gvoxel/s (voxels of the volume per second of
 step time), roofline of the extraction kernel (algorithmic bytes
-4*X*Y*Z + 28*V per launch over its HIP-event duration on the launch stream,
-against 8 TB/s), and the CPU baseline (the oracle's C restatement of the
+4*X*Y*Z + 28*V per launch over the average launch duration: the span of one
+HIP event pair on the launch stream around the K timed calls, over K; against
+8 TB/s), and the CPU baseline (the oracle's C restatement of the
 same extraction, OpenMP over z planes, timed on the host cores).
 """
 import argparse
@@ -110,11 +111,24 @@ def cpu_baseline_entry(vols, hs, threads_all, reps_single, reps_all, sample):
             "nproc": os.cpu_count(), "sample": sample}
 
 
-def _event_ms(torch, fn, steps, warmup):
-    """Average and median ms of fn() between HIP events on the current stream."""
+def _event_ms(torch, fn, steps, warmup, spread=False):
+    """Average ms of fn() over `steps` calls issued back to back: the span of
+    one HIP event pair on the current stream around them, over `steps` (an
+    event pair around every call would add idle time of its own).  With
+    spread, also the median of `steps` further calls, each between its own
+    event pair; else None."""
     for i in range(warmup):
         fn(i)
     torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for i in range(steps):
+        fn(i)
+    e1.record()
+    torch.cuda.synchronize()
+    avg = e0.elapsed_time(e1) / steps
+    if not spread:
+        return avg, None
     evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
            for _ in range(steps)]
     for i in range(steps):
@@ -123,7 +137,7 @@ def _event_ms(torch, fn, steps, warmup):
         evs[i][1].record()
     torch.cuda.synchronize()
     ms = sorted(a.elapsed_time(b) for a, b in evs)
-    return sum(ms) / len(ms), ms[len(ms) // 2]
+    return avg, ms[len(ms) // 2]
 
 
 def bench_diff(torch, dev, dim, steps, warmup):
@@ -300,9 +314,9 @@ def bench_batch(torch, dev, count, steps, warmup, threads_all, no_cpu, tune):
 
 def bench_c1(torch, dev, steps, warmup, threads_all, no_cpu):
     """The reference's CPU-path case: 128^3 short2 sphere (r 0.3 m in a 1 m
-    cube, trunc 3 voxels), vertex_voxel output.  GPU: one extraction call per
-    step between HIP events (the 8 MiB volume stays in cache, so this is not
-    an HBM figure); CPU: the oracle, single thread and all cores."""
+    cube, trunc 3 voxels), vertex_voxel output.  GPU: extraction calls back to
+    back between one HIP event pair (the 8 MiB volume stays in cache, so this
+    is not an HBM figure); CPU: the oracle, single thread and all cores."""
     from marching_cube_amd import mesh, synth
     vol_np, h = synth.sphere_volume(128)
     vol = torch.from_numpy(vol_np.view(np.int32)).to(dev)
@@ -311,10 +325,22 @@ def bench_c1(torch, dev, steps, warmup, threads_all, no_cpu):
     vv = torch.empty((n, mesh.VV_WORDS), dtype=torch.int32, device=dev)
     dc = torch.zeros(1, dtype=torch.int64, device=dev)
 
-    def step(i):
-        mesh.extract_into(vol, h, vv=vv, cap=n, d_count=dc, workspace=ws, sync=False)
+    # straight through the C ABI, arguments converted once (back-to-back calls
+    # of this short extraction must not wait on the host)
+    from marching_cube_amd import _capi
+    lib = _capi.load()
+    c_args = [ctypes.c_void_p(vol.data_ptr()), ctypes.c_int(_capi.ELEM_SHORT2), ctypes.c_int(128),
+              ctypes.c_int(128), ctypes.c_int(128), ctypes.c_float(h),
+              ctypes.c_void_p(vv.data_ptr()), None, None, None, ctypes.c_int64(n),
+              ctypes.c_void_p(dc.data_ptr()), None, ws.ptr, ctypes.c_size_t(ws.nbytes),
+              mesh._stream_handle(dev)]
 
-    avg, med = _event_ms(torch, step, max(steps, 20), warmup)
+    def step(i):
+        st = lib.mcx_extract(*c_args)
+        if st:
+            _capi.check(st, "mcx_extract")
+
+    avg, med = _event_ms(torch, step, max(steps, 20), warmup, spread=True)
     ws.check()
     assert int(dc.item()) == n
     nvox = 128 ** 3
@@ -344,7 +370,7 @@ def bench_raycast(torch, dev, vol, h, cfg, views, steps, warmup, cpu_threads, no
     def cast(i):
         fusion.raycast(vol, intr, poses[i % views], W, H, h=h, depth=depth, normals=normals)
 
-    avg, med = _event_ms(torch, cast, max(steps, views), warmup)
+    avg, med = _event_ms(torch, cast, max(steps, views), warmup, spread=True)
     hits = 0
     for k in range(views):
         cast(k)
@@ -649,29 +675,58 @@ def main():
     d_count = torch.zeros(1, dtype=torch.int64, device=dev)
     assert mesh.extract_into(vols[1], h, vv=vv, cap=cap, workspace=ws) == count
 
+    # the timed calls go straight through the C ABI (mcx_extract, the drop-in
+    # boundary), their arguments converted once
+    lib = _capi.load()
+    ws.ensure(lib.mcx_extract_workspace_size(dim, dim, dim))
+    c_args = [[ctypes.c_void_p(v.data_ptr()), ctypes.c_int(_capi.ELEM_SHORT2), ctypes.c_int(dim),
+               ctypes.c_int(dim), ctypes.c_int(dim), ctypes.c_float(h),
+               ctypes.c_void_p(vv.data_ptr()), None, None, None, ctypes.c_int64(cap),
+               ctypes.c_void_p(d_count.data_ptr()), None, ws.ptr, ctypes.c_size_t(ws.nbytes),
+               mesh._stream_handle(dev)]
+              for v in vols]
+    extract_c = lib.mcx_extract
+
     def step(i):
-        mesh.extract_into(vols[i & 1], h, vv=vv, cap=cap, d_count=d_count, workspace=ws,
-                          sync=False)
+        st = extract_c(*c_args[i & 1])
+        if st:
+            _capi.check(st, "mcx_extract")
 
     for i in range(args.warmup):
         step(i)
     torch.cuda.synchronize()
-    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
-           for _ in range(args.steps)]
+    # The timed region: the K calls back to back, one HIP event pair on the
+    # launch stream around all of them (an event pair around every call adds
+    # ~7 us of idle time per call on the stream), so the average extraction
+    # time is the span over K, gaps between the calls included.
+    e_start = torch.cuda.Event(enable_timing=True)
+    e_end = torch.cuda.Event(enable_timing=True)
     torch.cuda.synchronize()
     t_start = time.perf_counter()
+    e_start.record()
+    for i in range(args.steps):
+        step(i)
+    e_end.record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t_start
+    ws.check()  # device-side status of the asynchronous calls
+    assert int(d_count.item()) == count
+    avg_kernel_ms = e_start.elapsed_time(e_end) / args.steps
+    ms_per_step = wall * 1e3 / args.steps
+    # Per-call spread (not the timed region): an event pair around each of K
+    # further calls.
+    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+           for _ in range(args.steps)]
     for i in range(args.steps):
         evs[i][0].record()
         step(i)
         evs[i][1].record()
     torch.cuda.synchronize()
-    wall = time.perf_counter() - t_start
-    ws.check()  # device-side status of the asynchronous calls
-    assert int(d_count.item()) == count
-    kernel_ms = sorted(a.elapsed_time(b) for a, b in evs)
-    avg_kernel_ms = sum(kernel_ms) / len(kernel_ms)
-    med_kernel_ms = kernel_ms[len(kernel_ms) // 2]
-    ms_per_step = wall * 1e3 / args.steps
+    per_call = sorted(a.elapsed_time(b) for a, b in evs)
+    per_call_ms = {"avg": round(sum(per_call) / len(per_call), 4),
+                   "median": round(per_call[len(per_call) // 2], 4),
+                   "min": round(per_call[0], 4), "max": round(per_call[-1], 4),
+                   "note": "an event pair around each call (adds idle time of its own)"}
 
     alg_bytes = 4 * nvox + 28 * count
     achieved = alg_bytes / (avg_kernel_ms * 1e-3) / 1e9
@@ -770,7 +825,7 @@ def main():
         "triangles": count // 3,
         "mtri_per_s": round(count / 3 / (ms_per_step * 1e-3) / 1e6, 1),
         "kernel_ms_avg": round(avg_kernel_ms, 4),
-        "kernel_ms_median": round(med_kernel_ms, 4),
+        "per_call_event_ms": per_call_ms,
         "algorithmic_bytes": alg_bytes,
         "full_api_ms": round(full_ms, 4),
         "integrate_ms": round(integrate_ms, 2),
