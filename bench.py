@@ -584,7 +584,7 @@ def per_volume_fusion(torch, dev, g):
 def main():
     args = parse()
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+        print(f"bench.py: the extraction runs on one GPU; --gpus {args.gpus} is not supported "
               "(the path would shard by z-slabs and by volume, which is not built)",
               file=sys.stderr)
         return 2
