@@ -1801,6 +1801,8 @@ bool size_ok(int X, int Y, int Z) { return plane_ok(X, Y) && Z <= (1 << 24); }
 // The 128-bit count kernel addresses a column's planes (slab + 1, plus the
 // prefetched ones past it) through one buffer resource with 32-bit offsets.
 bool span_ok(int64_t XY, int slab) { return ((int64_t)slab + 4) * XY * 4 <= ((int64_t)1 << 31); }
+// The most planes a column of that kernel may take (< 1: planes too large).
+int64_t span_slab(int64_t XY) { return ((int64_t)1 << 31) / (XY * 4) - 4; }
 
 int64_t tiles_per_plane(int X, int Y) { return ((int64_t)X * Y + kTile - 1) / kTile; }
 
@@ -1977,6 +1979,10 @@ int prepare_single(ExtractParams& P, const void* d_vol, int X, int Y, int Z, flo
       std::max((int64_t)cu_count() * 4, cubes_of(X, Y, Z) / 8192);
   if (by_size < target) target = by_size;
   int64_t S = (target + R - 1) / R;
+  // large planes take more, shorter slabs, so that the 128-bit count kernel
+  // still reaches a column's planes through one buffer resource
+  const int64_t smax = span_slab((int64_t)X * Y);
+  if (smax >= 1) S = std::max(S, (Z - 1 + smax - 1) / smax);
   if (S > Z - 1) S = Z - 1;
   P.slab = (int)((Z - 1 + S - 1) / S);
   S = (Z - 1 + P.slab - 1) / P.slab;
@@ -2272,9 +2278,12 @@ extern "C" int mcx_batch_create(mcx_batch** out, int nvol, const int* dims_xyz, 
     gblocks += ((int64_t)X * Y + kGatherSpan - 1) / kGatherSpan * Z;
     const int64_t nt = tiles_of(X, Y, Z);
     if (nt > 0) {
-      // one column per in-plane range, slabs of at most 64 planes
+      // one column per in-plane range, slabs of the plan's planes per column
+      // (fewer where the planes are large)
       const int R = vols[b].R;
-      const int slab = g_tuning.batch_slab;
+      const int64_t smax = span_slab((int64_t)X * Y);
+      const int slab = (int)(smax >= 1 ? std::min((int64_t)g_tuning.batch_slab, smax)
+                                       : (int64_t)g_tuning.batch_slab);
       for (int z0 = 0; z0 < Z - 1; z0 += slab)
         for (int r = 0; r < R; ++r) cols.push_back(Column{b, r, z0, std::min(z0 + slab, Z - 1)});
     }
@@ -2282,7 +2291,7 @@ extern "C" int mcx_batch_create(mcx_batch** out, int nvol, const int* dims_xyz, 
     cubes += cubes_of(X, Y, Z);
     maxX = std::max(maxX, X);
     wide = wide && (reinterpret_cast<uintptr_t>(d_vols[b]) & 3u) == 0 &&
-           span_ok((int64_t)X * Y, g_tuning.batch_slab);
+           span_slab((int64_t)X * Y) >= 1;
   }
   mcx_batch* p = new mcx_batch();
   p->wide = wide;
