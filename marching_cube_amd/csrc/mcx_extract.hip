@@ -172,6 +172,7 @@ struct ExtractParams {
   uint32_t* nz_vol;      // batched launches: volume of each list entry (nz_list order)
   uint32_t* clist_vol;   // batched launches: volume of each clist entry
   bool write_masks;
+  int64_t first_block;   // gather: workgroup index of this launch's blockIdx.x 0
   uint32_t* rec;
   int64_t rec_cap;
 };
@@ -1560,7 +1561,7 @@ __global__ __launch_bounds__(kEmitThreads) void contrib_kernel(ExtractParams P, 
   __shared__ int s_tmp[8];
   __shared__ TileInfo s_tile[kTilesPerContribGroup];
   const int tid = threadIdx.x;
-  const int64_t t0 = (int64_t)blockIdx.x * kTilesPerContribGroup;
+  const int64_t t0 = (P.first_block + blockIdx.x) * kTilesPerContribGroup;
   s_case[tid] = dCase[tid];
   s_nv[tid] = dNumVerts[tid];
   if (tid < kTilesPerContribGroup) {
@@ -1678,10 +1679,11 @@ __global__ __launch_bounds__(256) void gather_kernel(ExtractParams P, const floa
   __shared__ int64_t s_rec[2][2][4];          // first record of the tile
   __shared__ uint32_t s_max[4];
   int vi = 0;
-  uint32_t blk = blockIdx.x;
+  const int64_t gblk = P.first_block + blockIdx.x;
+  uint32_t blk = (uint32_t)gblk;  // single volumes: below 2^31
   if constexpr (BATCH) {
-    vi = batch_volume_of<&BatchVol::gblk_begin>(P.bvol, P.nvol, blockIdx.x);
-    blk = (uint32_t)(blockIdx.x - P.bvol[vi].gblk_begin);
+    vi = batch_volume_of<&BatchVol::gblk_begin>(P.bvol, P.nvol, gblk);
+    blk = (uint32_t)(gblk - P.bvol[vi].gblk_begin);
     gvol += P.bvol[vi].vox_begin;
   }
   const VolDesc& V = vol_at<BATCH>(P, vi);
@@ -1791,6 +1793,32 @@ __global__ __launch_bounds__(256) void classify_kernel(const uint32_t* __restric
 
 // ================================================================ host side
 size_t al256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+// The backward's two launches over nb workgroups, in launches of at most
+// kMaxLaunchBlocks workgroups (each given its first workgroup).
+template <bool BATCH>
+int launch_contrib(ExtractParams P, int64_t nb, const float* gpos, float* contrib,
+                   hipStream_t s) {
+  for (int64_t b0 = 0; b0 < nb; b0 += kMaxLaunchBlocks) {
+    P.first_block = b0;
+    hipLaunchKernelGGL(contrib_kernel<BATCH>, dim3((unsigned)std::min(kMaxLaunchBlocks, nb - b0)),
+                       dim3(kEmitThreads), 0, s, P, gpos, contrib);
+    MCX_HIP(hipGetLastError());
+  }
+  return MCX_OK;
+}
+
+template <bool BATCH>
+int launch_gather(ExtractParams P, int64_t nb, const float* contrib, float* gvol,
+                  uint32_t* maxg, hipStream_t s) {
+  for (int64_t b0 = 0; b0 < nb; b0 += kMaxLaunchBlocks) {
+    P.first_block = b0;
+    hipLaunchKernelGGL(gather_kernel<BATCH>, dim3((unsigned)std::min(kMaxLaunchBlocks, nb - b0)),
+                       dim3(256), 0, s, P, contrib, gvol, maxg);
+    MCX_HIP(hipGetLastError());
+  }
+  return MCX_OK;
+}
 
 // Byte offsets of the count kernel's plane loads (in-plane word + X + 1024)
 // stay below 2^32.
@@ -2222,16 +2250,12 @@ int backward_impl(const float* d_vol, int X, int Y, int Z, float h, const float*
   }
   if (!d_scratch || scratch_bytes < (size_t)P.rec_cap * 8 * sizeof(float)) return MCX_ERR_WORKSPACE;
   float* contrib = static_cast<float*>(d_scratch);
-  const unsigned g1 = (unsigned)((n_tiles + kTilesPerContribGroup - 1) / kTilesPerContribGroup);
-  hipLaunchKernelGGL(contrib_kernel<false>, dim3(g1), dim3(kEmitThreads), 0, s, P, d_grad_pos,
-                     contrib);
-  MCX_HIP(hipGetLastError());
+  const int64_t g1 = (n_tiles + kTilesPerContribGroup - 1) / kTilesPerContribGroup;
+  rc = launch_contrib<false>(P, g1, d_grad_pos, contrib, s);
+  if (rc) return rc;
   const int64_t g2 = ((int64_t)X * Y + kGatherSpan - 1) / kGatherSpan * Z;
   if (g2 >= ((int64_t)1 << 31)) return MCX_ERR_UNSUPPORTED;
-  hipLaunchKernelGGL(gather_kernel<false>, dim3((unsigned)g2), dim3(256), 0, s, P, contrib,
-                     d_grad_vol, nullptr);
-  MCX_HIP(hipGetLastError());
-  return MCX_OK;
+  return launch_gather<false>(P, g2, contrib, d_grad_vol, nullptr, s);
 }
 }  // namespace
 
@@ -2470,15 +2494,10 @@ extern "C" int mcx_extract_backward_batched(const mcx_batch* plan, const float* 
   if (!d_scratch || scratch_bytes < (size_t)P.rec_cap * 8 * sizeof(float)) return MCX_ERR_WORKSPACE;
   if (plan->gather_blocks >= ((int64_t)1 << 31)) return MCX_ERR_UNSUPPORTED;
   float* contrib = static_cast<float*>(d_scratch);
-  const unsigned g1 =
-      (unsigned)((plan->n_tiles + kTilesPerContribGroup - 1) / kTilesPerContribGroup);
-  hipLaunchKernelGGL(contrib_kernel<true>, dim3(g1), dim3(kEmitThreads), 0, s, P, d_grad_pos,
-                     contrib);
-  MCX_HIP(hipGetLastError());
-  hipLaunchKernelGGL(gather_kernel<true>, dim3((unsigned)plan->gather_blocks), dim3(256), 0, s, P,
-                     contrib, d_grad_vol, d_maxabs);
-  MCX_HIP(hipGetLastError());
-  return MCX_OK;
+  const int64_t g1 = (plan->n_tiles + kTilesPerContribGroup - 1) / kTilesPerContribGroup;
+  rc = launch_contrib<true>(P, g1, d_grad_pos, contrib, s);
+  if (rc) return rc;
+  return launch_gather<true>(P, plan->gather_blocks, contrib, d_grad_vol, d_maxabs, s);
 }
 
 // ---- launch-shape tunables (no counterpart in the reference) ------------

@@ -69,6 +69,7 @@ struct IntegrateParams {
   float depth_scale;
   int W, H, K;
   FastDiv divX, divY;
+  int64_t first_block;   // workgroup index of this launch's blockIdx.x 0
 };
 
 // Frames that no voxel of a workgroup can observe, as bits of s_skip (one per
@@ -280,7 +281,7 @@ __global__ __launch_bounds__(256) void integrate_kernel(IntegrateParams P) {
   for (int i = threadIdx.x; i < P.K * 12; i += blockDim.x) s_pose[i] = P.poses[i];
   __syncthreads();
   const int64_t n = (int64_t)P.X * P.Y * P.Z;
-  const int64_t ib = (int64_t)blockIdx.x * blockDim.x;
+  const int64_t ib = (P.first_block + blockIdx.x) * (int64_t)blockDim.x;
   frame_skip_flags(s_pose, P.K, P, ib, min(ib + (int64_t)blockDim.x, n) - 1, s_skip);
   const int64_t i = ib + threadIdx.x;
   if (i >= n) return;
@@ -306,7 +307,7 @@ __global__ __launch_bounds__(256) void integrate1_kernel(IntegrateParams P) {
   if (threadIdx.x < 12) s_pose[threadIdx.x] = P.poses[threadIdx.x];
   __syncthreads();
   const int64_t n = (int64_t)P.X * P.Y * P.Z;
-  const int64_t ib = (int64_t)blockIdx.x * 256 * kVoxelsPerThread1;
+  const int64_t ib = (P.first_block + blockIdx.x) * 256 * kVoxelsPerThread1;
   frame_skip_flags(s_pose, 1, P, ib, min(ib + 256 * kVoxelsPerThread1, n) - 1, s_skip);
   if (s_skip[0] & 1u) return;  // no voxel of the workgroup can see the frame
   const float* R = s_pose;
@@ -447,7 +448,7 @@ __global__ __launch_bounds__(256) void integrate_tsdf_kernel(IntegrateParams P, 
   for (int i = threadIdx.x; i < P.K * 12; i += blockDim.x) s_pose[i] = P.poses[i];
   __syncthreads();
   const int64_t n = (int64_t)P.X * P.Y * P.Z;
-  const int64_t ib = (int64_t)blockIdx.x * blockDim.x;
+  const int64_t ib = (P.first_block + blockIdx.x) * (int64_t)blockDim.x;
   frame_skip_flags(s_pose, P.K, P, ib, min(ib + (int64_t)blockDim.x, n) - 1, s_skip);
   const int64_t i = ib + threadIdx.x;
   if (i >= n) return;
@@ -537,7 +538,7 @@ __global__ __launch_bounds__(256) void integrate_tsdf_backward_kernel(
   for (int i = threadIdx.x; i < P.K * 12; i += blockDim.x) s_pose[i] = P.poses[i];
   __syncthreads();
   const int64_t n = (int64_t)P.X * P.Y * P.Z;
-  const int64_t ib = (int64_t)blockIdx.x * blockDim.x;
+  const int64_t ib = (P.first_block + blockIdx.x) * (int64_t)blockDim.x;
   frame_skip_flags(s_pose, P.K, P, ib, min(ib + (int64_t)blockDim.x, n) - 1, s_skip);
   const int64_t i = ib + threadIdx.x;
   if (i >= n) return;
@@ -558,12 +559,14 @@ struct BatchIntegrateParams {
   int nvol;
   int from_init;         // start every voxel from (init_value, 0) without reading it
   float init_value;
+  int64_t first_block;   // workgroup index of this launch's blockIdx.x 0
 };
 
 // This workgroup's volume, its launch parameters and its poses in LDS.
 __device__ __forceinline__ int batch_setup(const BatchIntegrateParams& B, IntegrateParams& P,
                                            float* s_pose, int64_t* vox_begin, int64_t* i0) {
-  const int vi = batch_volume_of<&BatchVol::blk_begin>(B.bvol, B.nvol, blockIdx.x);
+  const int64_t blk = B.first_block + blockIdx.x;
+  const int vi = batch_volume_of<&BatchVol::blk_begin>(B.bvol, B.nvol, blk);
   const BatchVol& V = B.bvol[vi];
   P = B.base;
   P.X = V.X;
@@ -577,7 +580,7 @@ __device__ __forceinline__ int batch_setup(const BatchIntegrateParams& B, Integr
   const float* poses = B.base.poses + (int64_t)vi * P.K * 12;
   for (int i = threadIdx.x; i < P.K * 12; i += blockDim.x) s_pose[i] = poses[i];
   *vox_begin = V.vox_begin;
-  *i0 = (blockIdx.x - V.blk_begin) * 256;
+  *i0 = (blk - V.blk_begin) * 256;
   __syncthreads();
   return vi;
 }
@@ -814,15 +817,17 @@ int integrate_impl(void* d_vol, int X, int Y, int Z, float h, float trunc_neg, f
     P.K = K - k0 < kFramesPerPass ? K - k0 : kFramesPerPass;
     P.poses = d_poses + (int64_t)12 * k0;
     P.depth = d_depth + (int64_t)k0 * W * H;
-    if (P.K == 1) {
-      const int64_t per = 256 * kVoxelsPerThread1;
-      hipLaunchKernelGGL(integrate1_kernel<DT>, dim3((unsigned)((n + per - 1) / per)), dim3(256),
-                         0, s, P);
-    } else {
-      hipLaunchKernelGGL(integrate_kernel<DT>, dim3(blocks_for(n)), dim3(256),
-                         frames_lds_bytes(P.K), s, P);
+    const int64_t per = P.K == 1 ? 256 * kVoxelsPerThread1 : 256;
+    const int64_t nb = (n + per - 1) / per;
+    for (int64_t b0 = 0; b0 < nb; b0 += kMaxLaunchBlocks) {
+      P.first_block = b0;
+      const dim3 grid((unsigned)std::min(kMaxLaunchBlocks, nb - b0));
+      if (P.K == 1)
+        hipLaunchKernelGGL(integrate1_kernel<DT>, grid, dim3(256), 0, s, P);
+      else
+        hipLaunchKernelGGL(integrate_kernel<DT>, grid, dim3(256), frames_lds_bytes(P.K), s, P);
+      MCX_HIP(hipGetLastError());
     }
-    MCX_HIP(hipGetLastError());
   }
   return MCX_OK;
 }
@@ -922,9 +927,13 @@ extern "C" int mcx_integrate_tsdf(float* d_tsdf, int32_t* d_weight, int X, int Y
     P.K = K - k0 < kFramesPerPass ? K - k0 : kFramesPerPass;
     P.poses = d_poses + (int64_t)12 * k0;
     P.depth = d_depth + (int64_t)k0 * W * H;
-    hipLaunchKernelGGL(integrate_tsdf_kernel, dim3(blocks_for(n)), dim3(256),
-                       frames_lds_bytes(P.K), s, P, d_tsdf, d_weight);
-    MCX_HIP(hipGetLastError());
+    const int64_t nb = (n + 255) / 256;
+    for (int64_t b0 = 0; b0 < nb; b0 += kMaxLaunchBlocks) {
+      P.first_block = b0;
+      hipLaunchKernelGGL(integrate_tsdf_kernel, dim3((unsigned)std::min(kMaxLaunchBlocks, nb - b0)),
+                         dim3(256), frames_lds_bytes(P.K), s, P, d_tsdf, d_weight);
+      MCX_HIP(hipGetLastError());
+    }
   }
   return MCX_OK;
 }
@@ -967,9 +976,14 @@ extern "C" int mcx_integrate_tsdf_backward(const float* d_grad_tsdf, const int32
   P.K = K;
   P.poses = d_poses;
   P.depth = d_depth;
-  hipLaunchKernelGGL(integrate_tsdf_backward_kernel, dim3(blocks_for(n)), dim3(256),
-                     frames_lds_bytes(K), s, P, d_grad_tsdf, d_weight0, maxg, mdt, acc);
-  MCX_HIP(hipGetLastError());
+  const int64_t nb = (n + 255) / 256;
+  for (int64_t b0 = 0; b0 < nb; b0 += kMaxLaunchBlocks) {
+    P.first_block = b0;
+    hipLaunchKernelGGL(integrate_tsdf_backward_kernel,
+                       dim3((unsigned)std::min(kMaxLaunchBlocks, nb - b0)), dim3(256),
+                       frames_lds_bytes(K), s, P, d_grad_tsdf, d_weight0, maxg, mdt, acc);
+    MCX_HIP(hipGetLastError());
+  }
   hipLaunchKernelGGL(fixed_to_float_kernel, dim3(blocks_for(npix)), dim3(256), 0, s, acc, npix,
                      maxg, mdt, d_grad_depth);
   MCX_HIP(hipGetLastError());
@@ -1008,9 +1022,13 @@ extern "C" int mcx_integrate_tsdf_batched(const mcx_batch* plan, float* d_tsdf, 
   B.nvol = plan->nvol;
   B.from_init = from_init ? 1 : 0;
   B.init_value = init_value;
-  hipLaunchKernelGGL(integrate_tsdf_batched_kernel, dim3((unsigned)plan->blocks), dim3(256),
-                     frames_lds_bytes(K), s, B, d_tsdf, d_weight);
-  MCX_HIP(hipGetLastError());
+  for (int64_t b0 = 0; b0 < plan->blocks; b0 += kMaxLaunchBlocks) {
+    B.first_block = b0;
+    hipLaunchKernelGGL(integrate_tsdf_batched_kernel,
+                       dim3((unsigned)std::min(kMaxLaunchBlocks, plan->blocks - b0)), dim3(256),
+                       frames_lds_bytes(K), s, B, d_tsdf, d_weight);
+    MCX_HIP(hipGetLastError());
+  }
   return MCX_OK;
 }
 
@@ -1069,9 +1087,13 @@ extern "C" int mcx_integrate_tsdf_backward_batched(const mcx_batch* plan,
   B.nvol = plan->nvol;
   B.from_init = 0;
   B.init_value = 0.0f;
-  hipLaunchKernelGGL(integrate_tsdf_backward_batched_kernel, dim3((unsigned)plan->blocks),
-                     dim3(256), frames_lds_bytes(K), s, B, d_grad_tsdf, d_weight0, mg, mdt, acc);
-  MCX_HIP(hipGetLastError());
+  for (int64_t b0 = 0; b0 < plan->blocks; b0 += kMaxLaunchBlocks) {
+    B.first_block = b0;
+    hipLaunchKernelGGL(integrate_tsdf_backward_batched_kernel,
+                       dim3((unsigned)std::min(kMaxLaunchBlocks, plan->blocks - b0)), dim3(256),
+                       frames_lds_bytes(K), s, B, d_grad_tsdf, d_weight0, mg, mdt, acc);
+    MCX_HIP(hipGetLastError());
+  }
   hipLaunchKernelGGL(fixed_to_float_kernel, dim3(blocks_for(npix)), dim3(256), 0, s, acc, npix,
                      mg, mdt, d_grad_depth);
   MCX_HIP(hipGetLastError());

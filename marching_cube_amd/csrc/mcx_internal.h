@@ -286,6 +286,12 @@ struct VolDesc;  // extraction descriptors (mcx_extract.hip)
 struct Column;
 
 // Volume of workgroup `blk` (the last b with begin[b] <= blk), by binary
+// A dispatch carries its grid size in work-items as a 32-bit number, so a
+// grid of one workgroup per 256 or more items (volumes of 2^32 voxels and
+// more) is issued as several launches of at most this many workgroups, each
+// told the index of its first workgroup.
+constexpr int64_t kMaxLaunchBlocks = (int64_t)1 << 23;
+
 // search over a BatchVol table; the member selects which begin.
 template <int64_t BatchVol::*BEGIN>
 __device__ __forceinline__ int batch_volume_of(const BatchVol* bv, int nvol, int64_t blk) {

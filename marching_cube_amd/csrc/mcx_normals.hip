@@ -10,6 +10,7 @@
 // leaves the volume (the reference reads out of bounds) the normal is zero.
 // One thread per vertex; positions are read with a stride so vertex_voxel
 // records (7 words) and packed float3 arrays (3 words) both work.
+#include <algorithm>
 #include <float.h>
 #include <hip/hip_runtime.h>
 
@@ -83,9 +84,14 @@ extern "C" int mcx_normals(const void* d_vol, int X, int Y, int Z, float h, cons
   if (!d_vol || X < 1 || Y < 1 || Z < 1 || n < 0 || stride_floats < 3) return MCX_ERR_ARG;
   if (n == 0) return MCX_OK;
   if (!d_pos || !d_normals) return MCX_ERR_ARG;
-  hipLaunchKernelGGL(normals_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
-                     as_stream(stream), static_cast<const uint32_t*>(d_vol), X, Y, Z, h, d_pos,
-                     stride_floats, n, d_normals);
-  MCX_HIP(hipGetLastError());
+  // a thread per vertex, in launches of at most kMaxLaunchBlocks workgroups
+  const int64_t per = kMaxLaunchBlocks * 256;
+  for (int64_t v0 = 0; v0 < n; v0 += per) {
+    const int64_t nn = std::min(per, n - v0);
+    hipLaunchKernelGGL(normals_kernel, dim3((unsigned)((nn + 255) / 256)), dim3(256), 0,
+                       as_stream(stream), static_cast<const uint32_t*>(d_vol), X, Y, Z, h,
+                       d_pos + v0 * stride_floats, stride_floats, nn, d_normals + v0 * 3);
+    MCX_HIP(hipGetLastError());
+  }
   return MCX_OK;
 }

@@ -545,6 +545,8 @@ extern "C" int mcx_rle_encode(const void* d_vol, int X, int Y, int Z, int z0, in
   MCX_HIP(hipMemcpyAsync(&M, cnt + nblk, sizeof(M), hipMemcpyDeviceToHost, s));
   MCX_HIP(hipStreamSynchronize(s));
   if (M < nz || M > N) return MCX_ERR_INTERNAL;
+  // one thread per run below: the grid (in work-items) must fit 32 bits
+  if (M >= ((int64_t)1 << 31)) return MCX_ERR_UNSUPPORTED;
   hipLaunchKernelGGL(enc_compact_kernel, g2, dim3(256), 0, s, vol, XY, cnt, cp, slice_first);
   MCX_HIP(hipGetLastError());
   hipLaunchKernelGGL(enc_size_kernel, dim3(grid_for(M, 256)), dim3(256), 0, s, vol, cp, M, N,

@@ -1,39 +1,43 @@
-"""Extraction past the 32-bit limits, byte for byte against the oracle.
+"""Extraction and integration past the 32-bit limits, byte for byte against
+the oracle.
 
-2048 x 1024 x 1040 short2 (2.18 G voxels, 8.7 GB): voxel indices past 2^31,
-byte offsets past 2^33, 8 MiB planes.  At this plane size a column of the
-128-bit count kernel may take at most 252 planes (its buffer offsets stay
-below 2^31), so the volume is cut into five 208-plane slabs instead of the
-four that 56 columns per CU on 256 CUs would give.  The volume is built on the device
-(two spheres and a box, fp32 signed distances) and copied to the host for
-the oracle, so both see the same bytes.
+2048 x 2048 x 1040 short2 volumes (4.36 G voxels, 17.4 GB): voxel indices
+past 2^32, 16 MiB planes.  At this plane size a column of the 128-bit count
+kernel may take at most 124 planes (its buffer offsets stay below 2^31), so
+the volume is cut into nine 116-plane slabs instead of the two that 56
+columns per CU on 256 CUs would give; the integration kernels find voxel
+coordinates past 2^32 with 64-bit divisions.  Volumes and frames are built on
+the device and copied to the host for the oracle, so both see the same bytes.
 """
 import numpy as np
 import pytest
 import torch
 
-from marching_cube_amd import mesh
+from marching_cube_amd import fusion, mesh, synth
 from oracle import oracle as O
 
 pytestmark = pytest.mark.gpu
 
+X, Y, Z = 2048, 2048, 1040
+H_GRID = 1.0 / 1024
 TSDF_SCALE = 32767
 
 
-def _shapes_volume(dev, X, Y, Z, h, trunc):
+def _shapes_volume(dev, trunc):
     """Packed short2 (Z, Y, X) volume of the union of two spheres and a box:
     in the band |sdf| < trunc, tsdf = rint(sdf / trunc * 32767) (fp32 on the
     device) and weight 1 + (x + 2y + 3z) % 7; elsewhere tsdf +32767, weight
-    0.  Built 32 planes at a time."""
+    0.  Built 16 planes at a time."""
+    h = H_GRID
     vol = torch.empty((Z, Y, X), dtype=torch.int32, device=dev)
     xs = ((torch.arange(X, device=dev, dtype=torch.float32) + 0.5) * h).view(1, 1, X)
     ys = ((torch.arange(Y, device=dev, dtype=torch.float32) + 0.5) * h).view(1, Y, 1)
     xi = torch.arange(X, device=dev, dtype=torch.int32).view(1, 1, X)
     yi = torch.arange(Y, device=dev, dtype=torch.int32).view(1, Y, 1)
-    spheres = (((0.55, 0.5, 0.5), 0.3), ((1.45, 0.45, 0.75), 0.2))
+    spheres = (((0.55, 0.5, 0.5), 0.3), ((1.45, 1.45, 0.75), 0.2))
     bc, bh = (1.35, 0.55, 0.3), (0.25, 0.2, 0.12)
-    for z0 in range(0, Z, 32):
-        z1 = min(z0 + 32, Z)
+    for z0 in range(0, Z, 16):
+        z1 = min(z0 + 16, Z)
         zs = ((torch.arange(z0, z1, device=dev, dtype=torch.float32) + 0.5) * h).view(-1, 1, 1)
         zi = torch.arange(z0, z1, device=dev, dtype=torch.int32).view(-1, 1, 1)
         sdf = None
@@ -55,15 +59,15 @@ def _shapes_volume(dev, X, Y, Z, h, trunc):
     return vol
 
 
-def test_volume_past_32bit_limits_bit_exact(dev):
-    X, Y, Z = 2048, 1024, 1040
-    h = 1.0 / 1024
-    print(f"[case] {X}x{Y}x{Z} short2 shapes volume, vertex_voxel + int3 map + ver_voxel",
-          flush=True)
-    assert X * Y * Z > 2 ** 31
-    vol = _shapes_volume(dev, X, Y, Z, h, 3 * h)
+def test_extraction_past_32bit_limits_bit_exact(dev):
+    h = H_GRID
+    print(f"[case] extraction, {X}x{Y}x{Z} short2 shapes volume: vertex_voxel, int3 map, "
+          "ver_voxel", flush=True)
+    assert X * Y * Z > 2 ** 32
+    vol = _shapes_volume(dev, 3 * h)
     host = vol.cpu().numpy().view(np.uint32)
     ref = O.extract(host, h, want=("vv", "v2vox", "ver_voxel"), nthreads=16)
+    del host
     n = ref["n"]
     assert n > 1_000_000
     ws = mesh.Workspace(dev)
@@ -72,6 +76,39 @@ def test_volume_past_32bit_limits_bit_exact(dev):
     v2 = torch.empty((n, 3), dtype=torch.int32, device=dev)
     verv = torch.empty((Z, Y, X), dtype=torch.int16, device=dev)
     assert mesh.extract_into(vol, h, vv=vv, v2vox=v2, ver_voxel=verv, cap=n, workspace=ws) == n
+    del vol, ws
     assert vv.cpu().numpy().tobytes() == ref["vv"].view(np.int32).reshape(-1, 7).tobytes()
     assert np.array_equal(v2.cpu().numpy(), ref["v2vox"])
     assert np.array_equal(verv.cpu().numpy(), ref["ver_voxel"])
+    del vv, v2, verv, ref
+    torch.cuda.empty_cache()
+
+
+def test_integration_past_32bit_limits_bit_exact(dev):
+    """Two synthetic uint16 640x480 frames of a 12-object scene, fused one
+    call per frame (the reference's pattern) and in one two-frame sweep;
+    both volumes equal the oracle's."""
+    h, trunc, max_w = H_GRID, 0.03, 128
+    print(f"[case] integration, {X}x{Y}x{Z} short2, 2 frames 640x480 (per-frame calls and one "
+          "sweep)", flush=True)
+    intr = [525.0, 525.0, 319.5, 239.5]
+    objs = synth.kinfu_scene(12, 1.0, seed=5)
+    poses = synth.orbit_poses(2, 1.0, radius=1.3, seed=5)
+    depth = fusion.synth_depth(intr, poses, objs, 640, 480, seed=5, device=dev)
+    per_frame = fusion.voxel_new((X, Y, Z), trunc, h, device=dev)
+    for k in range(2):
+        fusion.integrate(per_frame, intr, poses[k:k + 1], depth[k:k + 1], trunc=trunc,
+                         max_w=max_w)
+    got = per_frame.data.cpu().numpy().view(np.uint32)
+    del per_frame
+    torch.cuda.empty_cache()
+    ref = O.volume_init((Z, Y, X))
+    O.integrate(ref, h, trunc, max_w, intr, poses, depth.cpu().numpy())
+    assert (ref != O.volume_init((1, 1, 1))[0, 0, 0]).sum() > 10_000_000  # updated voxels
+    assert np.array_equal(got, ref)
+    del got
+    sweep = fusion.voxel_new((X, Y, Z), trunc, h, device=dev)
+    fusion.integrate(sweep, intr, poses, depth, trunc=trunc, max_w=max_w)
+    assert np.array_equal(sweep.data.cpu().numpy().view(np.uint32), ref)
+    del sweep, ref
+    torch.cuda.empty_cache()
