@@ -109,7 +109,7 @@ extern "C" int mcx_chamfer_batched(const float* d_pos, const int64_t* d_vbase, i
   hipStream_t s = as_stream(stream);
   float* vloss = static_cast<float*>(d_ws);
   const int64_t blocks = (cap + 255) / 256;
-  if (blocks >= ((int64_t)1 << 31)) return MCX_ERR_UNSUPPORTED;
+  if (blocks > kMaxLaunchBlocks) return MCX_ERR_UNSUPPORTED;  // grid in work-items: 32 bits
   hipLaunchKernelGGL(chamfer_vertex_kernel, dim3((unsigned)blocks), dim3(256), 0, s, d_pos,
                      d_vbase, nvol, d_targets, M, cap, d_grad_pos, vloss);
   MCX_HIP(hipGetLastError());
