@@ -400,9 +400,11 @@ int mcx_synth_depth(uint16_t* d_depth, int W, int H, int K, const float* intr,
  * times candidates on the caller's volume and sets the fastest. */
 #define MCX_TUNE_COLUMNS_PER_CU 0     /* count kernel columns (waves) per CU, single volume, at
                                          most (smaller volumes: about 8192 cubes a column, at
-                                         least 4 per CU); 56 */
+                                         least 4 per CU; a column's planes + 4 stay within 2^31
+                                         bytes, so large planes take more, shorter ones); 56 */
 #define MCX_TUNE_EMIT_GROUPS_PER_CU 1 /* emit kernel persistent workgroups per CU; 16 */
-#define MCX_TUNE_BATCH_SLAB 2         /* planes per column of batched plans (at creation); 64 */
+#define MCX_TUNE_BATCH_SLAB 2         /* planes per column of batched plans (at creation; fewer
+                                         where the planes + 4 would pass 2^31 bytes); 64 */
 int mcx_set_tuning(int key, int value);
 int mcx_get_tuning(int key); /* -1 for an unknown key */
 
