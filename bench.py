@@ -218,14 +218,24 @@ def bench_diff(torch, dev, dim, steps, warmup):
         loss.backward()
     ag_ms, _ = _event_ms(torch, autograd_step, max(3, steps // 2), 2)
     nvox = X * Y * Z
+
+    def roof(alg, ms):
+        gbs = alg / (ms * 1e-3) / 1e9
+        return {"bound": "hbm", "algorithmic_bytes": alg, "achieved": round(gbs, 1),
+                "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4)}
+    bwd_ms = fb_ms - fdiff_ms
     return {"workload": f"bumps{dim}_fp32_fwd_bwd", "vertices": count,
             "fwd_ms": round(plain_ms, 4), "fwd_diff_ms": round(fdiff_ms, 4),
-            "bwd_ms": round(fb_ms - fdiff_ms, 4), "fwd_bwd_ms": round(fb_ms, 4),
+            "bwd_ms": round(bwd_ms, 4), "fwd_bwd_ms": round(fb_ms, 4),
             "fwd_bwd_over_fwd": round(fb_ms / plain_ms, 3),
             "fwd_bwd_over_fwd_diff": round(fb_ms / fdiff_ms, 3),
             "volume_copies": 5, "calls": "C ABI (mcx_extract, mcx_extract_diff, "
                                           "mcx_extract_backward), HIP events",
             "fwd_gvoxel_per_s": round(nvox / (plain_ms * 1e-3) / 1e9, 1),
+            # algorithmic bytes: forward 4 X Y Z + 12 V (float3 positions);
+            # backward 12 V (dL/dv) + 4 X Y Z (dL/dtsdf) + 4 X Y Z (volume re-read)
+            "roofline_fwd": roof(4 * nvox + 12 * count, plain_ms),
+            "roofline_bwd": roof(12 * count + 8 * nvox, bwd_ms),
             "autograd_step_ms": round(ag_ms, 4)}
 
 
@@ -830,6 +840,17 @@ def main():
         "full_api_ms": round(full_ms, 4),
         "integrate_ms": round(integrate_ms, 2),
         "integrate_gvoxel_frames_per_s": round(nvox * args.frames / (integrate_ms * 1e-3) / 1e9, 2),
+        # the fused sweep's algorithmic bytes: 8 X Y Z (read + write) + 2 W H K
+        # (depth); its time goes to VALU issue (the frames' projections and
+        # divisions per voxel), not to memory
+        "integrate_roofline": {
+            "bound": "hbm", "algorithmic_bytes": 8 * nvox + 2 * cfg["W"] * cfg["H"] * args.frames,
+            "achieved": round((8 * nvox + 2 * cfg["W"] * cfg["H"] * args.frames)
+                              / (integrate_ms * 1e-3) / 1e9, 2),
+            "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round((8 * nvox + 2 * cfg["W"] * cfg["H"] * args.frames)
+                          / (integrate_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 5),
+            "note": "VALU-issue bound"},
         "integrate_per_frame": per_frame,
         "fwd_bwd": diff_leg,
         "batched": batch_leg,
