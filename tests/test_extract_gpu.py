@@ -77,6 +77,49 @@ def test_empty_volume(dev):
     assert not r.ver_voxel.cpu().numpy().any()
 
 
+@pytest.mark.parametrize("shape", [(1, 5, 6), (5, 1, 6), (5, 6, 1), (1, 1, 1)])
+def test_degenerate_shapes_have_no_cubes(dev, shape):
+    # a side of one voxel: no cube, no vertex; ver_voxel is still cleared
+    vol = synth.noise_volume(shape, seed=7)
+    assert O.extract(vol, 0.5, want=("vv",))["n"] == 0
+    r = gpu_extract(vol, 0.5, dev)
+    assert r.count == 0
+    assert not r.ver_voxel.cpu().numpy().any()
+
+
+def test_device_error_word_reaches_every_caller(dev):
+    # The look-back's error word, set here from the host (a wait never hits
+    # its limit in a correct run): an asynchronous call's device count reads
+    # -1, the status query reports the error once and clears it, a
+    # synchronous call reports it as well, and later calls are unaffected.
+    print("[case] error word set from the host, async and sync calls", flush=True)
+    vol = synth.sphere_volume(64)[0]
+    ref = O.extract(vol, 1.0 / 64, want=("vv",))
+    n = ref["n"]
+    v = to_dev(vol, dev)
+    ws = mesh.Workspace(dev)
+    assert mesh.extract_into(v, 1.0 / 64, cap=0, workspace=ws) == n
+    err_word = ws.buf[12:16]  # WsHeader.error (after three 32-bit counters)
+    dc = torch.zeros(1, dtype=torch.int64, device=dev)
+    vv = torch.empty((n, 7), dtype=torch.int32, device=dev)
+    err_word.copy_(torch.tensor([1, 0, 0, 0], dtype=torch.uint8, device=dev))
+    mesh.extract_into(v, 1.0 / 64, vv=vv, cap=n, d_count=dc, workspace=ws, sync=False)
+    torch.cuda.synchronize()
+    assert int(dc.item()) == -1
+    with pytest.raises(_capi.McxError) as e:
+        ws.check()
+    assert e.value.status == _capi.ERR_INTERNAL
+    ws.check()  # reported once
+    err_word.copy_(torch.tensor([1, 0, 0, 0], dtype=torch.uint8, device=dev))
+    with pytest.raises(_capi.McxError) as e:
+        mesh.extract_into(v, 1.0 / 64, vv=vv, cap=n, workspace=ws)
+    assert e.value.status == _capi.ERR_INTERNAL
+    assert mesh.extract_into(v, 1.0 / 64, vv=vv, cap=n, d_count=dc, workspace=ws) == n
+    assert int(dc.item()) == n
+    assert vv.cpu().numpy().tobytes() == ref["vv"].view(np.int32).reshape(-1, 7).tobytes()
+    ws.check()
+
+
 def test_capacity_overflow_reports_required_count(dev):
     vol = synth.noise_volume((12, 13, 14), seed=4)
     ref = O.extract(vol, 0.5, want=("vv",))
