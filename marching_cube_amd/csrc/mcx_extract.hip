@@ -1665,7 +1665,13 @@ __global__ __launch_bounds__(kEmitThreads) void contrib_kernel(ExtractParams P, 
 // tiles' active-cube mask words, the popcount prefix of each word and the
 // tiles' first records in LDS, so a voxel's lookups read LDS and only active
 // cubes read their contribution (record index = tile's first record + rank
-// of the cube's bit).
+// of the cube's bit).  A wave takes 64 consecutive voxels at a time: each
+// lane finds which of its (at most 8) cubes are active (the cube (x - dx,
+// y - dy) has in-plane index q - dx - dy X), the wave packs the active
+// (voxel, corner) pairs onto consecutive lanes (about 1.6 % of the cubes are
+// active, so one round of 64 lanes usually takes them all) to rank them and
+// read their contributions, and each voxel lane then adds its values in
+// corner order, the order of the per-corner sum, so the bits are the same.
 constexpr int kGatherSpan = 1024;  // voxels per workgroup (4 per thread)
 // Batched launches: workgroup blk belongs to the volume whose gather range
 // [gblk_begin, next) holds it, and writes that volume's part of the
@@ -1678,6 +1684,8 @@ __global__ __launch_bounds__(256) void gather_kernel(ExtractParams P, const floa
   __shared__ uint32_t s_pre[2][2][4][kCPW];   // set bits of the tile's earlier words
   __shared__ int64_t s_rec[2][2][4];          // first record of the tile
   __shared__ uint32_t s_max[4];
+  __shared__ uint16_t s_pair[4][64 * 8];  // per wave: lane << 3 | corner of each active pair
+  __shared__ float s_val[4][64 * 8];      // the pair's contribution
   int vi = 0;
   const int64_t gblk = P.first_block + blockIdx.x;
   uint32_t blk = (uint32_t)gblk;  // single volumes: below 2^31
@@ -1727,29 +1735,76 @@ __global__ __launch_bounds__(256) void gather_kernel(ExtractParams P, const floa
   __syncthreads();
   float* const gz = gvol + (int64_t)z * XY;
   uint32_t gmax = 0;  // largest |g| bits of this thread
-#pragma unroll
+  const int lane = tid & 63, wv = tid >> 6;
+  uint16_t* const pairs = s_pair[wv];
+  float* const vals = s_val[wv];
+#pragma unroll 1
   for (int j = 0; j < kGatherSpan / 256; ++j) {
-    const uint32_t q = q0 + (uint32_t)(j * 256 + tid);
-    if (q >= XY) break;
-    const uint32_t y = V.divX.div(q);
-    const int x = (int)(q - y * (uint32_t)X);
-    float g = 0.0f;
+    // which of the (at most 8) cubes having this voxel as a corner are active
+    const uint32_t qb = q0 + (uint32_t)(j * 256 + wv * 64);
+    const uint32_t q = qb + (uint32_t)lane;
+    const bool inq = q < XY;
+    uint32_t act = 0;
+    if (inq) {
+      const uint32_t y = V.divX.div(q);
+      const int x = (int)(q - y * (uint32_t)X);
+      // cube (x - dx, y - dy) has in-plane index q - dx - dy X
+      const bool okx[2] = {x <= X - 2, x >= 1}, oky[2] = {(int)y <= Y - 2, y >= 1};
+      const bool okz[2] = {z <= Z - 2, z >= 1};
 #pragma unroll
-    for (int k = 0; k < 8; ++k) {
-      // the cube whose corner k is this voxel
+      for (int dy = 0; dy < 2; ++dy) {
+#pragma unroll
+        for (int dx = 0; dx < 2; ++dx) {
+          if (!(okx[dx] && oky[dy])) continue;
+          const uint32_t p = q - (uint32_t)dx - (dy ? (uint32_t)X : 0u);
+          const int slot = (int)(p >> 9) - base[dy];
+          const uint32_t word = (p >> 6) & 7u, bit = p & 63u;
+          const int c = dy ? (dx ? 2 : 3) : dx;  // corner index in the plane
+#pragma unroll
+          for (int dz = 0; dz < 2; ++dz)
+            if (okz[dz]) act |= (uint32_t)((s_mask[dz][dy][slot][word] >> bit) & 1ull) << (4 * dz + c);
+        }
+      }
+    }
+    // the wave's active (voxel, corner) pairs, packed in corner order per voxel
+    const uint32_t cnt = (uint32_t)__popc(act);
+    const uint32_t incl = wave_incl_scan32(cnt);
+    const uint32_t excl = incl - cnt;
+    const uint32_t total = (uint32_t)__builtin_amdgcn_readlane(incl, 63);
+    {
+      uint32_t a = act, e = excl;
+      while (a) {
+        const int k = __builtin_ctz(a);
+        a &= a - 1u;
+        pairs[e++] = (uint16_t)((lane << 3) | k);
+      }
+    }
+    wave_sync();
+    for (uint32_t pr = (uint32_t)lane; pr < total; pr += 64) {
+      const uint32_t d = pairs[pr];
+      const int k = (int)(d & 7u);
+      const uint32_t ql = qb + (d >> 3);
       const int dx = corner_dx(k), dy = corner_dy(k), dz = corner_dz(k);
-      const int cx = x - dx, cy = (int)y - dy, cz = z - dz;
-      if (cx < 0 || cy < 0 || cz < 0 || cx > X - 2 || cy > Y - 2 || cz > Z - 2) continue;
-      const uint32_t p = (uint32_t)cy * (uint32_t)X + (uint32_t)cx;
+      const uint32_t p = ql - (uint32_t)dx - (dy ? (uint32_t)X : 0u);
       const int slot = (int)(p >> 9) - base[dy];
       const int word = (int)((p >> 6) & 7u), bit = (int)(p & 63u);
       const uint64_t m = s_mask[dz][dy][slot][word];
-      if (!((m >> bit) & 1ull)) continue;
       const uint32_t rank = s_pre[dz][dy][slot][word] + (uint32_t)__popcll(m & ((1ull << bit) - 1ull));
-      g += contrib[(s_rec[dz][dy][slot] + rank) * 8 + k];
+      vals[pr] = contrib[(s_rec[dz][dy][slot] + rank) * 8 + k];
     }
-    gz[q] = g;
-    gmax = max(gmax, __float_as_uint(g) & 0x7fffffffu);
+    wave_sync();
+    if (inq) {
+      // the voxel's sum in corner order, as the per-corner loop adds it
+      float g = 0.0f;
+      uint32_t a = act, e = excl;
+      while (a) {
+        a &= a - 1u;
+        g += vals[e++];
+      }
+      gz[q] = g;
+      gmax = max(gmax, __float_as_uint(g) & 0x7fffffffu);
+    }
+    wave_sync();
   }
   if (maxg) {
     // one atomic per workgroup, only where it can raise the maximum (the
