@@ -23,19 +23,21 @@ H_GRID = 1.0 / 1024
 TSDF_SCALE = 32767
 
 
-def _shapes_volume(dev, trunc):
+def _shapes_volume(dev, trunc, dims=(X, Y, Z),
+                   spheres=(((0.55, 0.5, 0.5), 0.3), ((1.45, 1.45, 0.75), 0.2)),
+                   box=((1.35, 0.55, 0.3), (0.25, 0.2, 0.12))):
     """Packed short2 (Z, Y, X) volume of the union of two spheres and a box:
     in the band |sdf| < trunc, tsdf = rint(sdf / trunc * 32767) (fp32 on the
     device) and weight 1 + (x + 2y + 3z) % 7; elsewhere tsdf +32767, weight
     0.  Built 16 planes at a time."""
     h = H_GRID
+    X, Y, Z = dims
     vol = torch.empty((Z, Y, X), dtype=torch.int32, device=dev)
     xs = ((torch.arange(X, device=dev, dtype=torch.float32) + 0.5) * h).view(1, 1, X)
     ys = ((torch.arange(Y, device=dev, dtype=torch.float32) + 0.5) * h).view(1, Y, 1)
     xi = torch.arange(X, device=dev, dtype=torch.int32).view(1, 1, X)
     yi = torch.arange(Y, device=dev, dtype=torch.int32).view(1, Y, 1)
-    spheres = (((0.55, 0.5, 0.5), 0.3), ((1.45, 1.45, 0.75), 0.2))
-    bc, bh = (1.35, 0.55, 0.3), (0.25, 0.2, 0.12)
+    bc, bh = box
     for z0 in range(0, Z, 16):
         z1 = min(z0 + 16, Z)
         zs = ((torch.arange(z0, z1, device=dev, dtype=torch.float32) + 0.5) * h).view(-1, 1, 1)
@@ -112,3 +114,30 @@ def test_integration_past_32bit_limits_bit_exact(dev):
     assert np.array_equal(sweep.data.cpu().numpy().view(np.uint32), ref)
     del sweep, ref
     torch.cuda.empty_cache()
+
+
+def test_batch_with_large_planes_bit_exact(dev):
+    """A batch whose first volume has 4100 x 2048 planes (8.4 M voxels): its
+    columns take 59 planes instead of the plan's 64 (so that the 128-bit
+    count kernel still reaches them through one buffer resource), the small
+    volumes after it 64; every segment equals the oracle's extraction."""
+    from marching_cube_amd import batch
+    print("[case] batch: 4100x2048x70 volume + 5 small shape volumes", flush=True)
+    big = _shapes_volume(dev, 3 * H_GRID, dims=(4100, 2048, 70),
+                         spheres=(((0.6, 0.5, 0.035), 0.03), ((2.5, 1.2, 0.03), 0.05)),
+                         box=((3.2, 0.8, 0.04), (0.4, 0.3, 0.02)))
+    smalls, hs = synth.batch_shapes(5, lo=20, hi=64, seed=21)
+    vols = [big] + [torch.from_numpy(v.view(np.int32)).to(dev) for v in smalls]
+    hs = [H_GRID] + [float(x) for x in hs]
+    plan = batch.BatchPlan(vols, hs)
+    n, vbase, out = plan.extract(want=("vv",))
+    plan.check()
+    vb = vbase.cpu().numpy()
+    vv = out["vv"].cpu().numpy()
+    hosts = [big.cpu().numpy().view(np.uint32)] + smalls
+    for b, (v, h) in enumerate(zip(hosts, hs)):
+        ref = O.extract(v, float(h), want=("vv",), nthreads=16)
+        assert ref["n"] > (1_000_000 if b == 0 else 0), b
+        assert vb[b + 1] - vb[b] == ref["n"], b
+        assert vv[vb[b]:vb[b + 1]].tobytes() == ref["vv"].view(np.int32).reshape(-1, 7).tobytes(), b
+    assert n == vb[-1]
