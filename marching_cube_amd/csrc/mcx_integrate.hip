@@ -519,7 +519,16 @@ __device__ __forceinline__ void backward_voxel(const IntegrateParams& P, const f
     if (o + 1 <= last_unc) c *= (double)(w0 + o + 1) / (double)(w0 + last_unc + 1);
     const int first_cap = o + 1 > jc ? o + 1 : jc;
     const int ncap = nobs - first_cap;
-    if (ncap > 0) c *= pow(cap_a, (double)ncap);
+    if (ncap > 0) {
+      // cap_a^ncap by squaring (a library pow inlined here costs the kernel
+      // registers on every path, capped or not)
+      double pw = 1.0, base = cap_a;
+      for (int e = ncap; e; e >>= 1) {
+        if (e & 1) pw *= base;
+        base *= base;
+      }
+      c *= pw;
+    }
     const double contrib = (double)gi * c * (double)dtdd;
     if (contrib != 0.0) {
       const long long q = llrint(contrib * scale);
