@@ -75,21 +75,21 @@ def test_integrate_tsdf_bit_exact(dev):
     assert (rw > 0).mean() > 0.6  # the scene is observed
 
 
-def gpu_loss_and_grad(dev, h, trunc, poses, depth, targets, n):
+def gpu_loss_and_grad(dev, h, trunc, poses, depth, targets, n, max_w=128):
     d = torch.from_numpy(depth).to(dev).requires_grad_(True)
     t0, w0 = fusion_diff.tsdf_init((n, n, n), dev, -1.0)
     tsdf, _ = fusion_diff.integrate_tsdf(d, t0, w0, torch.from_numpy(poses).to(dev), h, trunc,
-                                         128, INTR)
+                                         max_w, INTR)
     pos = diff.extract_positions(tsdf, h)
     loss = fusion_diff.chamfer_loss(pos, torch.from_numpy(targets).to(dev))
     loss.backward()
     return float(loss.detach()), d.grad.cpu().numpy().astype(np.float64)
 
 
-def f64_chain(h, trunc, poses, depth64, targets, n):
+def f64_chain(h, trunc, poses, depth64, targets, n, max_w=128):
     t = np.full((n, n, n), -1.0, np.float64)
     w = np.zeros((n, n, n), np.int32)
-    O.integrate_tsdf_f64(t, w, h, trunc, 128, INTR, poses, depth64)
+    O.integrate_tsdf_f64(t, w, h, trunc, max_w, INTR, poses, depth64)
     pos, cube, edge = O.extract_f64(t, h)
     d2 = ((pos[:, None, :] - targets[None, :, :].astype(np.float64)) ** 2).sum(-1)
     j = d2.argmin(1)
@@ -98,11 +98,25 @@ def f64_chain(h, trunc, poses, depth64, targets, n):
 
 def test_depth_gradient_matches_fp64_fd(dev):
     print("[case] dL/ddepth 40^3 x 4 frames vs FP64 central differences", flush=True)
+    check_depth_gradient_fd(dev, max_w=128)
+
+
+def test_depth_gradient_with_weight_cap_matches_fp64_fd(dev):
+    # max weight 1 over 4 frames: every observation after a voxel's first is
+    # capped (a moving average with factor 1/2; most voxels of this scene are
+    # seen three or four times), so the backward's factor (M / (M + 1))^ncap
+    # is exercised
+    print("[case] dL/ddepth 40^3 x 4 frames, max weight 1, vs FP64 central differences",
+          flush=True)
+    check_depth_gradient_fd(dev, max_w=1)
+
+
+def check_depth_gradient_fd(dev, max_w):
     n = 40
     h, trunc, poses, depth, targets = scene(n=n)
-    loss, grad = gpu_loss_and_grad(dev, h, trunc, poses, depth, targets, n)
+    loss, grad = gpu_loss_and_grad(dev, h, trunc, poses, depth, targets, n, max_w)
     d64 = depth.astype(np.float64)
-    L0, topo0 = f64_chain(h, trunc, poses, d64, targets, n)
+    L0, topo0 = f64_chain(h, trunc, poses, d64, targets, n, max_w)
     assert abs(loss - L0) <= 1e-5 * abs(L0)
     gmax = np.abs(grad).max()
     assert gmax > 0
@@ -115,8 +129,8 @@ def test_depth_gradient_matches_fp64_fd(dev):
         dp, dm = d64.copy(), d64.copy()
         dp[k, v, u] += delta
         dm[k, v, u] -= delta
-        Lp, tp = f64_chain(h, trunc, poses, dp, targets, n)
-        Lm, tm = f64_chain(h, trunc, poses, dm, targets, n)
+        Lp, tp = f64_chain(h, trunc, poses, dp, targets, n, max_w)
+        Lm, tm = f64_chain(h, trunc, poses, dm, targets, n, max_w)
         stable = all(a.shape == b.shape and (a == b).all() for t in (tp, tm)
                      for a, b in zip(t, topo0))
         if not stable:
