@@ -1139,7 +1139,6 @@ constexpr int kWaveMeta = 64 * 4;
 constexpr int kPackedMap = kWaveRec, kPackedEx = kPackedMap + kWaveMap;
 constexpr int kPackedMeta = kPackedEx + kWaveEx;
 constexpr int kPackedWords = kPackedMeta + kWaveMeta;
-constexpr int kNodeTable = 2048;  // node offsets for grids up to 2048 per axis (else divisions)
 
 // Word 1 of a record's metadata in the emit kernels: bits 32..39 of its
 // output index (saturated: an index of 2^40 or more lies past any capacity)
@@ -1157,7 +1156,7 @@ __device__ __forceinline__ uint32_t meta_hi(uint64_t out, uint32_t z) {
 // come from the slot table and their words are read directly; along the two
 // axes the edge does not run on, the interpolated coordinate is exactly the
 // corner's node coordinate (fma(t, +0, p) = p), so only the edge axis is
-// interpolated, and with node_off only that axis divides.
+// interpolated.
 // Grid of a record's volume: dimensions and spacing (in registers; batched
 // launches fetch them from the record's lane).
 struct GridDims {
@@ -1171,8 +1170,7 @@ template <int ELEM, typename VolOf>
 __device__ __forceinline__ int64_t record_vertex(const GridDims& V, VolOf vol_of,
                                                  int16_t* ver_voxel, const uint8_t* s_slot,
                                                  const uint32_t* rc, int x, int y, int z, int q,
-                                                 uint32_t* o, bool want_key,
-                                                 const int8_t* node_off) {
+                                                 uint32_t* o, bool want_key) {
   const uint32_t sl = s_slot[(rc[0] & 0xffu) * 16 + q];
   const int ca = (int)(sl & 7u), cb = (int)((sl >> 3) & 7u), ax = (int)(sl >> 6);
   const uint32_t wa = rc[1 + ca], wb = rc[1 + cb];
@@ -1206,15 +1204,7 @@ __device__ __forceinline__ int64_t record_vertex(const GridDims& V, VolOf vol_of
   const float pz = ax == 2 ? pe : paz;
   // record fields (store_point :336-367)
   int vx, vy, vz;
-  if (node_off) {
-    const int nx = node_off[ix], ny = node_off[iy], nz = node_off[iz];
-    float ce = pe / h;
-    ce = ce - 0.5f;
-    const int ve = (int)floorf(ce);
-    vx = ax == 0 ? ve : ix + nx;
-    vy = ax == 1 ? ve : iy + ny;
-    vz = ax == 2 ? ve : iz + nz;
-  } else {
+  {
     float cx = px / h, cy = py / h, cz = pz / h;
     cx = cx - 0.5f;
     cy = cy - 0.5f;
@@ -1266,8 +1256,8 @@ __device__ __forceinline__ int64_t record_vertex(const GridDims& V, VolOf vol_of
 template <int ELEM, bool BATCH>
 __device__ __forceinline__ void emit_block(const ExtractParams& P, const VolDesc& V,
                                            const uint8_t* s_slot, const uint8_t* s_nv,
-                                           uint32_t* s_w, int nb, const int8_t* node_off,
-                                           const uint32_t* s_vi, const GridDims& dl) {
+                                           uint32_t* s_w, int nb, const uint32_t* s_vi,
+                                           const GridDims& dl) {
   const uint32_t* s_rec = s_w;
   uint8_t* s_map = reinterpret_cast<uint8_t*>(s_w + kPackedMap);
   uint32_t* s_ex = s_w + kPackedEx;
@@ -1306,8 +1296,7 @@ __device__ __forceinline__ void emit_block(const ExtractParams& P, const VolDesc
         auto vol_of = [&]() { return BATCH ? P.vols[s_vi[r]].vol : V.vol; };
         const int64_t key = record_vertex<ELEM>(g, vol_of, BATCH ? nullptr : V.ver_voxel, s_slot,
                                                 rc, (int)mt[2], (int)mt[3],
-                                                (int)(mt[1] >> 8), slot, o, P.ekey != nullptr,
-                                                node_off);
+                                                (int)(mt[1] >> 8), slot, o, P.ekey != nullptr);
         if (vv) {
 #pragma unroll
           for (int f = 0; f < 7; ++f) vv[7 * out + f] = o[f];
@@ -1332,8 +1321,7 @@ __device__ __forceinline__ void emit_block(const ExtractParams& P, const VolDesc
 template <int ELEM, bool BATCH>
 __device__ __forceinline__ void emit_staged_tile(const ExtractParams& P, const uint8_t* s_slot,
                                                  const uint8_t* s_nv, uint32_t* s_w,
-                                                 uint32_t t, const int8_t* node_off,
-                                                 uint32_t* s_vi) {
+                                                 uint32_t t, uint32_t* s_vi) {
   const int lane = threadIdx.x & 63;
   const int vi = BATCH ? volume_of_tile(P, t) : 0;
   const VolDesc& V = vol_at<BATCH>(P, vi);
@@ -1381,7 +1369,7 @@ __device__ __forceinline__ void emit_staged_tile(const ExtractParams& P, const u
     }
     out += (int64_t)__builtin_amdgcn_readlane(vincl, 63);
     if (n)
-      emit_block<ELEM, BATCH>(P, V, s_slot, s_nv, s_w, n, node_off, s_vi,
+      emit_block<ELEM, BATCH>(P, V, s_slot, s_nv, s_w, n, s_vi,
                               GridDims{V.X, V.Y, V.Z, V.h});
   }
 }
@@ -1397,20 +1385,17 @@ __device__ __forceinline__ void emit_staged_tile(const ExtractParams& P, const u
 // the tile's records before it: those in the unit come from a scan of the
 // unit's vertex counts, those before the unit (first entry only) are summed
 // from their case bytes.
-// Six waves per SIMD (at most 80 VGPRs, 25.6 KB of LDS per workgroup, 26.6 KB
-// batched).  Batched launches: every clist entry carries its volume, each
-// record's volume index is kept beside its metadata, and there is no
-// node-offset table (the grid spacing differs between volumes).
+// Seven workgroups per CU for single volumes (22.8 KB of LDS each), six
+// batched (23.8 KB).  Batched launches: every clist entry carries its
+// volume and each record's volume index is kept beside its metadata.
 template <int ELEM, bool BATCH>
 __global__ __launch_bounds__(kEmitThreads) __attribute__((amdgpu_waves_per_eu(6)))
 void emit_units_kernel(ExtractParams P) {
   __shared__ __attribute__((aligned(16))) uint8_t s_slot[256 * 16];
   __shared__ uint8_t s_nv[256];
   __shared__ uint32_t s_wave[kEmitThreads / 64][kPackedWords];
-  __shared__ uint32_t s_mark[kEmitThreads / 64][64];
-  __shared__ int8_t s_node[BATCH ? 1 : kNodeTable];
+  __shared__ uint8_t s_mark[kEmitThreads / 64][64];  // lane + 1 of the entry starting here
   __shared__ uint32_t s_vol[BATCH ? kEmitThreads / 64 : 1][64];  // volume of each record
-  __shared__ int s_node_ok;
   static_assert(kEmitThreads * 16 == sizeof(SlotTable), "16 table bytes per thread");
   reinterpret_cast<u32x4*>(s_slot)[threadIdx.x] =
       reinterpret_cast<const u32x4*>(dSlot.v)[threadIdx.x];
@@ -1425,25 +1410,9 @@ void emit_units_kernel(ExtractParams P) {
     if (P.d_count) *P.d_count = -1;
   }
   const VolDesc& V = P.vol0;  // single volumes
-  const int mdim = BATCH ? kNodeTable + 1 : max(V.X, max(V.Y, V.Z));
-  if (threadIdx.x == 0) s_node_ok = mdim <= kNodeTable;
-  __syncthreads();
-  if (!BATCH && mdim <= kNodeTable) {
-    for (int i = threadIdx.x; i < mdim; i += kEmitThreads) {
-      const float nd = ((float)i + 0.5f) * V.h;
-      float c = nd / V.h;
-      c = c - 0.5f;
-      const float d = floorf(c) - (float)i;  // 0 or -1 for any usable grid size
-      if (d >= -2.0f && d <= 2.0f)
-        s_node[i] = (int8_t)(int)d;
-      else
-        s_node_ok = 0;
-    }
-  }
   __syncthreads();  // the last workgroup barrier
-  const int8_t* node_off = s_node_ok ? s_node : nullptr;
   uint32_t* const s_w = s_wave[wv];
-  uint32_t* const mark = s_mark[wv];
+  uint8_t* const mark = s_mark[wv];
   uint32_t* const s_meta = s_w + kPackedMeta;
   uint32_t* const s_vi = s_vol[BATCH ? wv : 0];
   const uint32_t nl = P.hdr->nz_total ? P.hdr->list_n : 0u;  // clist entries
@@ -1453,8 +1422,7 @@ void emit_units_kernel(ExtractParams P) {
   const int64_t nwaves = (int64_t)gridDim.x * (kEmitThreads / 64);
   for (int64_t k = (int64_t)blockIdx.x * (kEmitThreads / 64) + wv; k < n_work; k += nwaves) {
     if (k >= n_units) {
-      emit_staged_tile<ELEM, BATCH>(P, s_slot, s_nv, s_w, P.dense_list[k - n_units], node_off,
-                                    s_vi);
+      emit_staged_tile<ELEM, BATCH>(P, s_slot, s_nv, s_w, P.dense_list[k - n_units], s_vi);
       continue;
     }
     const int64_t g0 = k << 6;
@@ -1478,7 +1446,7 @@ void emit_units_kernel(ExtractParams P) {
       const uint32_t local = (uint32_t)(t - Vl.tile_begin);
       z = Vl.divR.div(local);
       p0 = (local - z * (uint32_t)Vl.R) * (uint32_t)kTile;
-      mark[first] = (uint32_t)lane + 1u;
+      mark[first] = (uint8_t)(lane + 1);
     }
     // records of the first entry's tile before the unit: their vertices
     const int64_t head = g0 - __shfl(pre, 0);
@@ -1540,7 +1508,7 @@ void emit_units_kernel(ExtractParams P) {
         dl = GridDims{Vk.X, Vk.Y, Vk.Z, Vk.h};
       }
     }
-    emit_block<ELEM, BATCH>(P, V, s_slot, s_nv, s_w, R, node_off, s_vi, dl);
+    emit_block<ELEM, BATCH>(P, V, s_slot, s_nv, s_w, R, s_vi, dl);
   }
 }
 

@@ -222,9 +222,9 @@ def test_large_tiles_and_mixed_staged_path(dev, records):
 
 
 @pytest.mark.parametrize("shape", [(4, 6, 2600), (2050, 3, 5)])
-def test_long_axis_without_node_table(dev, shape):
-    # an axis above 2048 voxels: the emit kernel divides on every axis
-    # instead of reading its per-workgroup node-offset table
+def test_long_axis(dev, shape):
+    # one axis above 2048 voxels, the others a few voxels (rows of one
+    # voxel's tiles spanning several planes, or tiles of many rows)
     vol = synth.noise_volume(shape, seed=13, p_zero_w=0.05)
     check_against_oracle(vol, 0.0037, dev)
 
