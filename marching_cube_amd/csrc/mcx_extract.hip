@@ -1157,11 +1157,13 @@ __device__ __forceinline__ uint32_t meta_hi(uint64_t out, uint32_t z) {
 // axes the edge does not run on, the interpolated coordinate is exactly the
 // corner's node coordinate (fma(t, +0, p) = p), so only the edge axis is
 // interpolated.
-// Grid of a record's volume: dimensions and spacing (in registers; batched
-// launches fetch them from the record's lane).
+// Grid of a record's volume: dimensions, spacing and (batched launches) its
+// index in the volume table; in registers (batched launches fetch them from
+// the record's lane).
 struct GridDims {
   int X, Y, Z;
   float h;
+  uint32_t vi;
 };
 
 // vol_of() gives the volume's voxels (only called where the record's voxel
@@ -1250,14 +1252,13 @@ __device__ __forceinline__ int64_t record_vertex(const GridDims& V, VolOf vol_of
 // produces one vertex of a round of 64 and stores its record fields at its
 // output index (the vertex's own slot; consecutive lanes mostly write
 // consecutive records).
-// Batched launches: lane r < nb holds the grid of record r's volume in `dl`
-// (each vertex lane fetches its record's with a lane shuffle, not from the
-// volume table in memory); s_vi[r] is the record's volume.
+// Batched launches: lane r < nb holds the grid and index of record r's
+// volume in `dl` (each vertex lane fetches its record's with lane shuffles,
+// not from the volume table in memory).
 template <int ELEM, bool BATCH>
 __device__ __forceinline__ void emit_block(const ExtractParams& P, const VolDesc& V,
                                            const uint8_t* s_slot, const uint8_t* s_nv,
-                                           uint32_t* s_w, int nb, const uint32_t* s_vi,
-                                           const GridDims& dl) {
+                                           uint32_t* s_w, int nb, const GridDims& dl) {
   const uint32_t* s_rec = s_w;
   uint8_t* s_map = reinterpret_cast<uint8_t*>(s_w + kPackedMap);
   uint32_t* s_ex = s_w + kPackedEx;
@@ -1283,8 +1284,9 @@ __device__ __forceinline__ void emit_block(const ExtractParams& P, const VolDesc
       g.Y = __shfl(dl.Y, r);
       g.Z = __shfl(dl.Z, r);
       g.h = __shfl(dl.h, r);
+      g.vi = (uint32_t)__shfl((int)dl.vi, r);
     } else {
-      g = GridDims{V.X, V.Y, V.Z, V.h};
+      g = GridDims{V.X, V.Y, V.Z, V.h, 0u};
     }
     if (j < tot) {
       const uint32_t* rc = s_rec + r * kRecWords;
@@ -1293,7 +1295,7 @@ __device__ __forceinline__ void emit_block(const ExtractParams& P, const VolDesc
       const int64_t out = (int64_t)(((uint64_t)(mt[1] & 0xffu) << 32) | mt[0]) + slot;
       if (out < P.cap) {
         uint32_t o[7];
-        auto vol_of = [&]() { return BATCH ? P.vols[s_vi[r]].vol : V.vol; };
+        auto vol_of = [&]() { return BATCH ? P.vols[g.vi].vol : V.vol; };
         const int64_t key = record_vertex<ELEM>(g, vol_of, BATCH ? nullptr : V.ver_voxel, s_slot,
                                                 rc, (int)mt[2], (int)mt[3],
                                                 (int)(mt[1] >> 8), slot, o, P.ekey != nullptr);
@@ -1321,11 +1323,10 @@ __device__ __forceinline__ void emit_block(const ExtractParams& P, const VolDesc
 template <int ELEM, bool BATCH>
 __device__ __forceinline__ void emit_staged_tile(const ExtractParams& P, const uint8_t* s_slot,
                                                  const uint8_t* s_nv, uint32_t* s_w,
-                                                 uint32_t t, uint32_t* s_vi) {
+                                                 uint32_t t) {
   const int lane = threadIdx.x & 63;
   const int vi = BATCH ? volume_of_tile(P, t) : 0;
   const VolDesc& V = vol_at<BATCH>(P, vi);
-  if constexpr (BATCH) s_vi[lane] = (uint32_t)vi;
   uint32_t* const s_meta = s_w + kPackedMeta;
   const uint32_t local = (uint32_t)(t - V.tile_begin);
   const int z = (int)V.divR.div(local);
@@ -1369,8 +1370,8 @@ __device__ __forceinline__ void emit_staged_tile(const ExtractParams& P, const u
     }
     out += (int64_t)__builtin_amdgcn_readlane(vincl, 63);
     if (n)
-      emit_block<ELEM, BATCH>(P, V, s_slot, s_nv, s_w, n, s_vi,
-                              GridDims{V.X, V.Y, V.Z, V.h});
+      emit_block<ELEM, BATCH>(P, V, s_slot, s_nv, s_w, n,
+                              GridDims{V.X, V.Y, V.Z, V.h, (uint32_t)vi});
   }
 }
 
@@ -1385,9 +1386,9 @@ __device__ __forceinline__ void emit_staged_tile(const ExtractParams& P, const u
 // the tile's records before it: those in the unit come from a scan of the
 // unit's vertex counts, those before the unit (first entry only) are summed
 // from their case bytes.
-// Seven workgroups per CU for single volumes (22.8 KB of LDS each), six
-// batched (23.8 KB).  Batched launches: every clist entry carries its
-// volume and each record's volume index is kept beside its metadata.
+// Seven workgroups per CU (22.8 KB of LDS each).  Batched launches: every
+// clist entry carries its volume, and each record lane holds its volume's
+// grid and index for the vertex lanes.
 template <int ELEM, bool BATCH>
 __global__ __launch_bounds__(kEmitThreads) __attribute__((amdgpu_waves_per_eu(6)))
 void emit_units_kernel(ExtractParams P) {
@@ -1395,7 +1396,6 @@ void emit_units_kernel(ExtractParams P) {
   __shared__ uint8_t s_nv[256];
   __shared__ uint32_t s_wave[kEmitThreads / 64][kPackedWords];
   __shared__ uint8_t s_mark[kEmitThreads / 64][64];  // lane + 1 of the entry starting here
-  __shared__ uint32_t s_vol[BATCH ? kEmitThreads / 64 : 1][64];  // volume of each record
   static_assert(kEmitThreads * 16 == sizeof(SlotTable), "16 table bytes per thread");
   reinterpret_cast<u32x4*>(s_slot)[threadIdx.x] =
       reinterpret_cast<const u32x4*>(dSlot.v)[threadIdx.x];
@@ -1414,7 +1414,6 @@ void emit_units_kernel(ExtractParams P) {
   uint32_t* const s_w = s_wave[wv];
   uint8_t* const mark = s_mark[wv];
   uint32_t* const s_meta = s_w + kPackedMeta;
-  uint32_t* const s_vi = s_vol[BATCH ? wv : 0];
   const uint32_t nl = P.hdr->nz_total ? P.hdr->list_n : 0u;  // clist entries
   const int64_t total = nl ? P.lpre[nl] : 0;  // records of the list
   const int64_t n_units = (total + 63) >> 6;
@@ -1422,7 +1421,7 @@ void emit_units_kernel(ExtractParams P) {
   const int64_t nwaves = (int64_t)gridDim.x * (kEmitThreads / 64);
   for (int64_t k = (int64_t)blockIdx.x * (kEmitThreads / 64) + wv; k < n_work; k += nwaves) {
     if (k >= n_units) {
-      emit_staged_tile<ELEM, BATCH>(P, s_slot, s_nv, s_w, P.dense_list[k - n_units], s_vi);
+      emit_staged_tile<ELEM, BATCH>(P, s_slot, s_nv, s_w, P.dense_list[k - n_units]);
       continue;
     }
     const int64_t g0 = k << 6;
@@ -1490,7 +1489,7 @@ void emit_units_kernel(ExtractParams P) {
     const uint32_t p0k = (uint32_t)__shfl((int)p0, kk);
     const uint32_t zk = (uint32_t)__shfl((int)z, kk);
     const uint32_t vik = BATCH ? (uint32_t)__shfl((int)vi, kk) : 0u;
-    GridDims dl{0, 0, 0, 0.0f};
+    GridDims dl{0, 0, 0, 0.0f, 0u};
     if (lane < R) {
 #pragma unroll
       for (int q = 0; q < kRecWords; ++q) s_w[lane * kRecWords + q] = rec[q];
@@ -1503,12 +1502,9 @@ void emit_units_kernel(ExtractParams P) {
       mt[1] = meta_hi(o, zk);
       mt[2] = p - y * (uint32_t)Vk.X;
       mt[3] = y;
-      if constexpr (BATCH) {
-        s_vi[lane] = vik;
-        dl = GridDims{Vk.X, Vk.Y, Vk.Z, Vk.h};
-      }
+      if constexpr (BATCH) dl = GridDims{Vk.X, Vk.Y, Vk.Z, Vk.h, vik};
     }
-    emit_block<ELEM, BATCH>(P, V, s_slot, s_nv, s_w, R, s_vi, dl);
+    emit_block<ELEM, BATCH>(P, V, s_slot, s_nv, s_w, R, dl);
   }
 }
 
