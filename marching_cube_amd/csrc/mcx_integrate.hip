@@ -69,6 +69,7 @@ struct IntegrateParams {
   float depth_scale;
   int W, H, K;
   FastDiv divX, divY;
+  int step_x, step_y, step_z;  // 256 voxels as x, y, z steps (step_x < X, step_y < Y)
   int64_t first_block;   // workgroup index of this launch's blockIdx.x 0
 };
 
@@ -293,36 +294,56 @@ __global__ __launch_bounds__(256) void integrate_kernel(IntegrateParams P) {
   integrate_voxel<DT>(P, s_pose, s_skip, px, py, pz, i);
 }
 
-// One frame (the reference's per-frame call): four voxels a thread (voxel
-// ib + t + 256 j of 1024 per workgroup, coalesced).  The four projections
-// come first, then the depth, ray-factor and voxel loads of all four are
+// One frame (the reference's per-frame call): eight voxels a thread (voxel
+// ib + t + 256 j of the workgroup's 2048, coalesced).  The thread's first
+// voxel has its coordinates from two divisions; each next one lies 256
+// voxels further, (step_x, step_y, step_z) with carries.  The projections
+// come first, then the depth, ray-factor and voxel loads of all eight are
 // issued together (at clamped, valid addresses), then the updates: the
-// operations and results of integrate_voxel, with the memory latency of the
-// four overlapped.  Only voxels the frame updates are written.
-constexpr int kVoxelsPerThread1 = 4;
+// operations and results of integrate_voxel, with the memory latency
+// overlapped.  The voxels are addressed from the workgroup's first one
+// (32-bit offsets).  Only voxels the frame updates are written.
+// 0.422 ms per 640x480 frame into 512^3 against 0.435 ms with four voxels a
+// thread, each by two divisions (four with steps: 0.445; eight by divisions:
+// 0.434; sixteen: 0.422 with steps, 0.444 by divisions).
+constexpr int kVoxelsPerThread1 = 8;
 template <typename DT>
 __global__ __launch_bounds__(256) void integrate1_kernel(IntegrateParams P) {
+  constexpr int NV = kVoxelsPerThread1;
   __shared__ float s_pose[12];
   __shared__ uint32_t s_skip[1];
   if (threadIdx.x < 12) s_pose[threadIdx.x] = P.poses[threadIdx.x];
   __syncthreads();
   const int64_t n = (int64_t)P.X * P.Y * P.Z;
-  const int64_t ib = (P.first_block + blockIdx.x) * 256 * kVoxelsPerThread1;
-  frame_skip_flags(s_pose, 1, P, ib, min(ib + 256 * kVoxelsPerThread1, n) - 1, s_skip);
+  const int64_t ib = (P.first_block + blockIdx.x) * 256 * NV;
+  const int nloc = (int)min((int64_t)256 * NV, n - ib);  // voxels of the workgroup
+  frame_skip_flags(s_pose, 1, P, ib, ib + nloc - 1, s_skip);
   if (s_skip[0] & 1u) return;  // no voxel of the workgroup can see the frame
   const float* R = s_pose;
-  int pix[kVoxelsPerThread1];
-  float qn[kVoxelsPerThread1];
-  int64_t vi[kVoxelsPerThread1];
+  uint32_t* const vb = P.vol + ib;  // voxel ib + l is vb[l]
+  const int t = (int)threadIdx.x;
+  int x, y, z;
+  voxel_coords(P, ib + min(t, nloc - 1), &x, &y, &z);
+  int pix[NV];
+  float qn[NV];
 #pragma unroll
-  for (int j = 0; j < kVoxelsPerThread1; ++j) {
-    const int64_t i = ib + threadIdx.x + 256 * j;
-    vi[j] = i < n ? i : n - 1;
+  for (int j = 0; j < NV; ++j) {
+    if (j > 0) {  // x + step_x < 2X, y + step_y + 1 < 2Y
+      x += P.step_x;
+      y += P.step_y;
+      z += P.step_z;
+      if (x >= P.X) {
+        x -= P.X;
+        ++y;
+      }
+      if (y >= P.Y) {
+        y -= P.Y;
+        ++z;
+      }
+    }
     pix[j] = -1;
     qn[j] = 0.0f;
-    if (i >= n) continue;
-    int x, y, z;
-    voxel_coords(P, i, &x, &y, &z);
+    if (t + 256 * j >= nloc) continue;
     const float px = ((float)x + 0.5f) * P.h;
     const float py = ((float)y + 0.5f) * P.h;
     const float pz = ((float)z + 0.5f) * P.h;
@@ -339,24 +360,24 @@ __global__ __launch_bounds__(256) void integrate1_kernel(IntegrateParams P) {
     pix[j] = vv * P.W + ui;
     qn[j] = sqrtf(__fmaf_rn(qz, qz, __fmaf_rn(qy, qy, qx * qx)));
   }
-  DT dm[kVoxelsPerThread1];
-  float rf[kVoxelsPerThread1];
-  uint32_t cur[kVoxelsPerThread1];
+  DT dm[NV];
+  float rf[NV];
+  uint32_t cur[NV];
 #pragma unroll
-  for (int j = 0; j < kVoxelsPerThread1; ++j) {
+  for (int j = 0; j < NV; ++j) {
     const int q = pix[j] >= 0 ? pix[j] : 0;
     dm[j] = static_cast<const DT*>(P.depth)[q];
     rf[j] = P.rayfac[q];
-    cur[j] = P.vol[vi[j]];
+    cur[j] = vb[min(t + 256 * j, nloc - 1)];
   }
 #pragma unroll
-  for (int j = 0; j < kVoxelsPerThread1; ++j) {
+  for (int j = 0; j < NV; ++j) {
     if (pix[j] < 0) continue;
     const float zval = ((float)dm[j] * P.depth_scale) * rf[j];
     if (!(zval > FLT_MIN)) continue;
     const float sdf = zval - qn[j];
     if (!(sdf >= -P.trunc_neg)) continue;
-    P.vol[vi[j]] = packed_update(cur[j], truncated(sdf, P.trunc, P.trunc_neg), P.max_w);
+    vb[t + 256 * j] = packed_update(cur[j], truncated(sdf, P.trunc, P.trunc_neg), P.max_w);
   }
 }
 
@@ -799,6 +820,12 @@ int prepare_integrate(IntegrateParams& P, int X, int Y, int Z, float h, float tr
   P.H = H;
   P.divX = FastDiv((uint32_t)X);
   P.divY = FastDiv((uint32_t)Y);
+  {
+    const int64_t XY = (int64_t)X * Y, r = 256 % XY;
+    P.step_z = (int)(256 / XY);
+    P.step_y = (int)(r / X);
+    P.step_x = (int)(r % X);
+  }
   P.K = 0;
   P.poses = nullptr;
   P.depth = nullptr;

@@ -129,3 +129,25 @@ def test_synth_depth_matches_numpy_raycast(dev):
         assert ((d[k] > 0) == hit).mean() > 0.995  # silhouettes agree up to edge pixels
         both = hit & (d[k] > 0)
         assert np.abs(d[k][both] - best[both]).max() <= 0.0006  # rounding to 1 mm
+
+
+@pytest.mark.parametrize("dims", [(40, 40, 40), (7, 5, 300), (300, 3, 20), (256, 2, 9),
+                                  (1, 1, 700), (513, 1, 7)])
+def test_per_frame_calls_on_odd_shapes(dev, dims):
+    """One call per frame (the reference's pattern) into volumes whose
+    256-voxel step carries into y, into z, or not at all (X below, at and
+    above 256, X * Y below 256): each equals the oracle's integration."""
+    X, Y, Z = dims
+    print(f"[case] per-frame integration {X}x{Y}x{Z}, 8 frames", flush=True)
+    intr, objs, poses, depth, h, _ = small_scene(dev, seed=21)
+    # the cameras moved by the offset of the volume's centre from the scene's
+    # (the same frames), so that the thin volumes cut through the scene
+    poses = poses.copy()
+    poses[:, 9:] -= (np.array([0.5, 0.5, 0.5]) - 0.5 * h * np.array(dims)).astype(np.float32)
+    vox = fusion.voxel_new((X, Y, Z), 0.05, h, device=dev)
+    for k in range(depth.shape[0]):
+        fusion.integrate(vox, intr, poses[k:k + 1], depth[k:k + 1], trunc=0.05, max_w=128)
+    ref = O.volume_init((Z, Y, X))
+    O.integrate(ref, h, 0.05, 128, np.array(intr, np.float32), poses, depth.cpu().numpy())
+    assert (ref != O.volume_init((1, 1, 1))[0, 0, 0]).sum() >= 30  # voxels the frames update
+    assert vox.data.cpu().numpy().view(np.uint32).tobytes() == ref.tobytes()
