@@ -565,15 +565,17 @@ __global__ __launch_bounds__(256) void integrate_tsdf_backward_kernel(
   extern __shared__ float s_frames[];
   float* const s_pose = s_frames;
   uint32_t* const s_skip = reinterpret_cast<uint32_t*>(s_frames + 12 * P.K);
-  for (int i = threadIdx.x; i < P.K * 12; i += blockDim.x) s_pose[i] = P.poses[i];
-  __syncthreads();
   const int64_t n = (int64_t)P.X * P.Y * P.Z;
   const int64_t ib = (P.first_block + blockIdx.x) * (int64_t)blockDim.x;
-  frame_skip_flags(s_pose, P.K, P, ib, min(ib + (int64_t)blockDim.x, n) - 1, s_skip);
   const int64_t i = ib + threadIdx.x;
-  if (i >= n) return;
-  const float gi = g[i];
-  if (gi == 0.0f) return;
+  // most blocks carry no gradient: they leave after reading it, before the
+  // poses and the frame tests
+  const float gi = i < n ? g[i] : 0.0f;
+  if (!__syncthreads_or(gi != 0.0f)) return;
+  for (int k = threadIdx.x; k < P.K * 12; k += blockDim.x) s_pose[k] = P.poses[k];
+  __syncthreads();
+  frame_skip_flags(s_pose, P.K, P, ib, min(ib + (int64_t)blockDim.x, n) - 1, s_skip);
+  if (i >= n || gi == 0.0f) return;
   float px, py, pz;
   voxel_centre(P, i, &px, &py, &pz);
   backward_voxel(P, s_pose, s_skip, px, py, pz, gi, w0vol ? w0vol[i] : 0,
@@ -592,11 +594,15 @@ struct BatchIntegrateParams {
   int64_t first_block;   // workgroup index of this launch's blockIdx.x 0
 };
 
-// This workgroup's volume, its launch parameters and its poses in LDS.
-__device__ __forceinline__ int batch_setup(const BatchIntegrateParams& B, IntegrateParams& P,
+// The volume of this workgroup's block (binary search over the block ranges).
+__device__ __forceinline__ int batch_block_volume(const BatchIntegrateParams& B) {
+  return batch_volume_of<&BatchVol::blk_begin>(B.bvol, B.nvol, B.first_block + blockIdx.x);
+}
+
+// This workgroup's volume vi: its launch parameters and its poses in LDS.
+__device__ __forceinline__ int batch_setup(const BatchIntegrateParams& B, int vi, IntegrateParams& P,
                                            float* s_pose, int64_t* vox_begin, int64_t* i0) {
   const int64_t blk = B.first_block + blockIdx.x;
-  const int vi = batch_volume_of<&BatchVol::blk_begin>(B.bvol, B.nvol, blk);
   const BatchVol& V = B.bvol[vi];
   P = B.base;
   P.X = V.X;
@@ -622,7 +628,7 @@ __global__ __launch_bounds__(256) void integrate_tsdf_batched_kernel(BatchIntegr
   uint32_t* const s_skip = reinterpret_cast<uint32_t*>(s_frames + 12 * B.base.K);
   IntegrateParams P;
   int64_t vb, ib;
-  batch_setup(B, P, s_pose, &vb, &ib);
+  batch_setup(B, batch_block_volume(B), P, s_pose, &vb, &ib);
   const int64_t n = (int64_t)P.X * P.Y * P.Z;
   frame_skip_flags(s_pose, P.K, P, ib, min(ib + 256, n) - 1, s_skip);
   const int64_t i = ib + threadIdx.x;
@@ -653,15 +659,23 @@ __global__ __launch_bounds__(256) void integrate_tsdf_backward_batched_kernel(
   extern __shared__ float s_frames[];
   float* const s_pose = s_frames;
   uint32_t* const s_skip = reinterpret_cast<uint32_t*>(s_frames + 12 * B.base.K);
+  // most blocks carry no gradient: they leave after reading it, before the
+  // poses and the frame tests
+  const int vi = batch_block_volume(B);
+  float gi = 0.0f;
+  {
+    const BatchVol& V = B.bvol[vi];
+    const int64_t i = (B.first_block + blockIdx.x - V.blk_begin) * 256 + threadIdx.x;
+    if (i < (int64_t)V.X * V.Y * V.Z) gi = g[V.vox_begin + i];
+  }
+  if (!__syncthreads_or(gi != 0.0f)) return;
   IntegrateParams P;
   int64_t vb, ib;
-  const int vi = batch_setup(B, P, s_pose, &vb, &ib);
+  batch_setup(B, vi, P, s_pose, &vb, &ib);
   const int64_t n = (int64_t)P.X * P.Y * P.Z;
   frame_skip_flags(s_pose, P.K, P, ib, min(ib + 256, n) - 1, s_skip);
   const int64_t i = ib + threadIdx.x;
-  if (i >= n) return;
-  const float gi = g[vb + i];
-  if (gi == 0.0f) return;
+  if (i >= n || gi == 0.0f) return;
   float px, py, pz;
   voxel_centre(P, i, &px, &py, &pz);
   backward_voxel(P, s_pose, s_skip, px, py, pz, gi, w0vol ? w0vol[vb + i] : 0,
