@@ -216,17 +216,6 @@ __device__ __forceinline__ uint32_t cube_index(const uint32_t c[8]) {
   return w0 ? 0u : ci;
 }
 
-// c[i] for a runtime i without dynamic register indexing (which would put
-// the array in scratch memory): a tree of selects on the bits of i (a chain
-// of i == k selects is turned back into an indexed load by the compiler).
-__device__ __forceinline__ uint32_t pick8(const uint32_t c[8], int i) {
-  const bool b0 = (i & 1) != 0, b1 = (i & 2) != 0, b2 = (i & 4) != 0;
-  const uint32_t r01 = b0 ? c[1] : c[0], r23 = b0 ? c[3] : c[2];
-  const uint32_t r45 = b0 ? c[5] : c[4], r67 = b0 ? c[7] : c[6];
-  const uint32_t r03 = b1 ? r23 : r01, r47 = b1 ? r67 : r45;
-  return b2 ? r47 : r03;
-}
-
 template <bool BATCH>
 __device__ __forceinline__ const VolDesc& vol_at(const ExtractParams& P, int vi) {
   if constexpr (BATCH) return P.vols[vi];
@@ -466,21 +455,6 @@ __global__ __launch_bounds__(256) void count_kernel(ExtractParams P) {
 // volumes, which take the 32-bit kernel).
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 typedef uint32_t u32x4a4 __attribute__((ext_vector_type(4), aligned(4)));  // 4-byte aligned
-typedef short s16x2 __attribute__((ext_vector_type(2)));
-typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
-
-__device__ __forceinline__ uint32_t min_s16x2(uint32_t a, uint32_t b) {
-  return __builtin_bit_cast(uint32_t, __builtin_elementwise_min(__builtin_bit_cast(s16x2, a),
-                                                                __builtin_bit_cast(s16x2, b)));
-}
-__device__ __forceinline__ uint32_t max_s16x2(uint32_t a, uint32_t b) {
-  return __builtin_bit_cast(uint32_t, __builtin_elementwise_max(__builtin_bit_cast(s16x2, a),
-                                                                __builtin_bit_cast(s16x2, b)));
-}
-__device__ __forceinline__ uint32_t min_u16x2(uint32_t a, uint32_t b) {
-  return __builtin_bit_cast(uint32_t, __builtin_elementwise_min(__builtin_bit_cast(u16x2, a),
-                                                                __builtin_bit_cast(u16x2, b)));
-}
 
 // Workgroup id remapped so that runs of kXcdGroup consecutive workgroups
 // (consecutive columns) execute on one XCD (dispatch is round-robin over the
@@ -1672,10 +1646,10 @@ __global__ __launch_bounds__(256) void gather_kernel(ExtractParams P, const floa
     const int64_t lo = (int64_t)q0 - 1 - (dy ? X : 0);
     base[dy] = (int)(max(lo, (int64_t)0) >> 9);
   }
+  uint64_t m = 0;  // the staged mask word of threads 0..127
   if (tid < 128) {
     const int dz = tid >> 6, dy = (tid >> 5) & 1, slot = (tid >> 3) & 3, word = tid & 7;
     const int cz = z - dz, r = base[dy] + slot;
-    uint64_t m = 0;
     if (cz >= 0 && cz <= Z - 2 && r < V.R)
       m = P.masks[(V.tile_begin + (int64_t)cz * V.R + r) * kCPW + word];
     s_mask[dz][dy][slot][word] = m;
@@ -1686,7 +1660,17 @@ __global__ __launch_bounds__(256) void gather_kernel(ExtractParams P, const floa
     if (cz >= 0 && cz <= Z - 2 && r < V.R) rs = P.tile_rec[V.tile_begin + (int64_t)cz * V.R + r];
     s_rec[dz][dy][slot] = rs;
   }
-  __syncthreads();
+  if (!__syncthreads_or(m != 0)) {
+    // no active cube has a corner in the span: its gradient is zero (the
+    // sums below would all be empty), and it raises no maximum
+    float* const gz0 = gvol + (int64_t)z * XY;
+#pragma unroll
+    for (int j = 0; j < kGatherSpan / 256; ++j) {
+      const uint32_t q = q0 + (uint32_t)(j * 256 + tid);
+      if (q < XY) gz0[q] = 0.0f;
+    }
+    return;
+  }
   if (tid < 16) {
     const int dz = tid >> 3, dy = (tid >> 2) & 1, slot = tid & 3;
     uint32_t pre = 0;
