@@ -80,12 +80,15 @@ struct IntegrateParams {
 // and projecting past one edge of the pixel window: the image of a convex box
 // in front of the camera is the convex hull of its corners' images.  Eight
 // lanes take a frame, one corner each, with the voxels' own fp32 operations
-// (a corner is a voxel centre).  A voxel's fp32 q differs from the exact one
-// by at most d = 4e-7 * (|dx| + |dy| + |dz|) per component (the subtraction
-// and the fma chain, |R| <= 1), so its u differs from the exact projection by
-// at most (fx d / qz_min) (1 + |u - cx|_max / fx), likewise v; the tests use
-// four times that plus one pixel as the margin ("behind": every corner below
-// -max(1e-4 m, 4 d)), so every voxel of a skipped frame would have taken its
+// (a corner is a voxel centre) up to the quotients, which come from the
+// hardware reciprocal (error within 2e-7 |u - cx| plus an ulp).  A voxel's
+// fp32 q differs from the exact one by at most d = 4e-7 * (|dx| + |dy| +
+// |dz|) per component (the subtraction and the fma chain, |R| <= 1), so its u
+// differs from the exact projection by at most (fx d / qz_min) (1 + |u -
+// cx|_max / fx), likewise v; the tests use four times that plus one pixel as
+// the margin ("behind": every corner below -max(1e-4 m, 4 d)); the pixel
+// covers the reciprocal's error wherever a test is close (|u|, |v| < 2^16:
+// under 0.02 pixel), so every voxel of a skipped frame would have taken its
 // q.z or window skip.  Called by every thread of the workgroup (barriers
 // inside).
 __device__ void frame_skip_flags(const float* s_pose, int K, const IntegrateParams& P,
@@ -119,8 +122,9 @@ __device__ void frame_skip_flags(const float* s_pose, int K, const IntegratePara
       const float qx = __fmaf_rn(R[6], dz, __fmaf_rn(R[3], dy, R[0] * dx));
       const float qy = __fmaf_rn(R[7], dz, __fmaf_rn(R[4], dy, R[1] * dx));
       qz = __fmaf_rn(R[8], dz, __fmaf_rn(R[5], dy, R[2] * dx));
-      u = __fmaf_rn(P.fx, qx / qz, P.cx);
-      v = __fmaf_rn(P.fy, qy / qz, P.cy);
+      const float rz = __builtin_amdgcn_rcpf(qz);
+      u = __fmaf_rn(P.fx, qx * rz, P.cx);
+      v = __fmaf_rn(P.fy, qy * rz, P.cy);
     }
     float qzmin = qz, qzmax = qz, umin = u, umax = u, vmin = v, vmax = v, dmax = dist;
 #pragma unroll
@@ -139,10 +143,13 @@ __device__ void frame_skip_flags(const float* s_pose, int K, const IntegratePara
       if (qzmax < -fmaxf(1e-4f, 4.0f * d)) {
         skip = true;
       } else if (qzmin > 0.01f) {
-        const float au = fmaxf(fabsf(umin - P.cx), fabsf(umax - P.cx)) / P.fx;
-        const float av = fmaxf(fabsf(vmin - P.cy), fabsf(vmax - P.cy)) / P.fy;
-        const float tu = 1.0f + 4.0f * (P.fx * d / qzmin) * (1.0f + au);
-        const float tv = 1.0f + 4.0f * (P.fy * d / qzmin) * (1.0f + av);
+        // margins from reciprocals (relative error ~1e-7 in a bound taken
+        // four times over)
+        const float rq = __builtin_amdgcn_rcpf(qzmin);
+        const float au = fmaxf(fabsf(umin - P.cx), fabsf(umax - P.cx)) * __builtin_amdgcn_rcpf(P.fx);
+        const float av = fmaxf(fabsf(vmin - P.cy), fabsf(vmax - P.cy)) * __builtin_amdgcn_rcpf(P.fy);
+        const float tu = 1.0f + 4.0f * (P.fx * d * rq) * (1.0f + au);
+        const float tv = 1.0f + 4.0f * (P.fy * d * rq) * (1.0f + av);
         // a voxel is observed only at rint(u) in [0, W - 2), rint(v) in [0, H - 2)
         skip = umax < -0.5f - tu || umin > (float)P.W - 2.5f + tu || vmax < -0.5f - tv ||
                vmin > (float)P.H - 2.5f + tv;
