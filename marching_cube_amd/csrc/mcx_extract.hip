@@ -2005,7 +2005,8 @@ int prepare_single(ExtractParams& P, const void* d_vol, int X, int Y, int Z, flo
   // and fills its plane pipeline before its first step, costs that slabs of
   // 1-3 planes (what the per-CU target alone gives at 128^3 and 256^3) pay
   // every few planes: at 256^3 the fp32 forward took 0.115 ms with 3-plane
-  // slabs and 0.079 ms with 16; 128^3 0.039 ms with 1 and 0.032 with 4.
+  // slabs and 0.079 ms with 16 (later 0.084 with 16 against 0.087 with 8
+  // and 0.104 with 4, in one process); 128^3 0.039 ms with 1 and 0.032 with 4.
   const int64_t by_size =
       std::max((int64_t)cu_count() * 4, cubes_of(X, Y, Z) / 8192);
   if (by_size < target) target = by_size;
