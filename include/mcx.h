@@ -280,7 +280,8 @@ int mcx_volume_init(void* d_vol, int X, int Y, int Z, void* stream);
  * {R row-major, t}.  d_depth: K x H x W uint16, metres = value * depth_scale,
  * 0 = no measurement.  Frames are fused in order, all K in one sweep of the
  * volume.  Workspace: mcx_integrate_workspace_size(W, H) bytes (per-pixel ray
- * factors, recomputed by every call). */
+ * factors, recomputed by every call).  Frames of W * H * 4 >= 2^31 bytes:
+ * MCX_ERR_UNSUPPORTED. */
 size_t mcx_integrate_workspace_size(int W, int H);
 int mcx_integrate(void* d_vol, int X, int Y, int Z, float h, float trunc, int max_w,
                   const float* intr, const float* d_poses, const uint16_t* d_depth,

@@ -73,6 +73,28 @@ struct IntegrateParams {
   int64_t first_block;   // workgroup index of this launch's blockIdx.x 0
 };
 
+// Depth value and ray factor of pixel pix (>= 0) of frame k, as buffer loads
+// from uniform per-frame bases with 32-bit offsets (no 64-bit address
+// arithmetic per voxel and frame).  Word 3 of a raw 32-bit-element buffer
+// resource on gfx950.
+constexpr int kRawBufferWord3 = 0x00020000;
+template <typename DT>
+__device__ __forceinline__ float frame_depth(const IntegrateParams& P, int k, uint32_t pix) {
+  const int64_t wh = (int64_t)P.W * P.H;
+  const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
+      const_cast<DT*>(static_cast<const DT*>(P.depth) + (int64_t)k * wh), (short)0,
+      (int)(wh * (int64_t)sizeof(DT)), kRawBufferWord3);
+  if constexpr (sizeof(DT) == 2)
+    return (float)(DT)__builtin_amdgcn_raw_buffer_load_b16(rs, pix * 2u, 0, 0);
+  else
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rs, pix * 4u, 0, 0));
+}
+__device__ __forceinline__ float frame_rayfac(const IntegrateParams& P, uint32_t pix) {
+  const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
+      const_cast<float*>(P.rayfac), (short)0, (int)((int64_t)P.W * P.H * 4), kRawBufferWord3);
+  return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rs, pix * 4u, 0, 0));
+}
+
 // Frames that no voxel of a workgroup can observe, as bits of s_skip (one per
 // frame).  The workgroup's voxels, linear indices [i0, i1] of one X x Y x Z
 // volume, lie in a box of voxel centres; frame k is skipped when the whole
@@ -243,7 +265,6 @@ template <typename DT>
 __device__ __forceinline__ void integrate_voxel(const IntegrateParams& P, const float* s_pose,
                                                 const uint32_t* s_skip, float px, float py,
                                                 float pz, int64_t i) {
-  const int64_t frame = (int64_t)P.W * P.H;
   uint32_t cur = 0;
   bool have = false;
   for_active_frames(s_skip, P.K, [&](int k) {
@@ -258,9 +279,8 @@ __device__ __forceinline__ void integrate_voxel(const IntegrateParams& P, const 
     if (!(fabsf(u) < 1e9f) || !(fabsf(v) < 1e9f)) return;
     const int ui = (int)rintf(u), vi = (int)rintf(v);
     if (ui < 0 || vi < 0 || ui >= P.W - 2 || vi >= P.H - 2) return;
-    const int pix = vi * P.W + ui;
-    const DT dm = static_cast<const DT*>(P.depth)[(int64_t)k * frame + pix];
-    const float zval = ((float)dm * P.depth_scale) * P.rayfac[pix];
+    const uint32_t pix = (uint32_t)(vi * P.W + ui);
+    const float zval = (frame_depth<DT>(P, k, pix) * P.depth_scale) * frame_rayfac(P, pix);
     if (!(zval > FLT_MIN)) return;
     const float qn = sqrtf(__fmaf_rn(qz, qz, __fmaf_rn(qy, qy, qx * qx)));
     const float sdf = zval - qn;
@@ -373,8 +393,8 @@ __global__ __launch_bounds__(256) void integrate1_kernel(IntegrateParams P) {
 #pragma unroll
   for (int j = 0; j < NV; ++j) {
     const int q = pix[j] >= 0 ? pix[j] : 0;
-    dm[j] = static_cast<const DT*>(P.depth)[q];
-    rf[j] = P.rayfac[q];
+    dm[j] = static_cast<const DT*>(P.depth)[(uint32_t)q];
+    rf[j] = P.rayfac[(uint32_t)q];
     cur[j] = vb[min(t + 256 * j, nloc - 1)];
   }
 #pragma unroll
@@ -410,9 +430,8 @@ __device__ __forceinline__ bool observe(const IntegrateParams& P, const float* R
   const int ui = (int)rintf(u), vi = (int)rintf(v);
   if (ui < 0 || vi < 0 || ui >= P.W - 2 || vi >= P.H - 2) return false;
   const int pix = vi * P.W + ui;
-  const DT dm = static_cast<const DT*>(P.depth)[(int64_t)k * P.W * P.H + pix];
-  const float rf = P.rayfac[pix];
-  const float d = (float)dm * P.depth_scale;
+  const float rf = frame_rayfac(P, (uint32_t)pix);
+  const float d = frame_depth<DT>(P, k, (uint32_t)pix) * P.depth_scale;
   const float zval = d * rf;
   if (!(zval > FLT_MIN)) return false;
   const float qn = sqrtf(__fmaf_rn(qz, qz, __fmaf_rn(qy, qy, qx * qx)));
@@ -820,6 +839,8 @@ int prepare_integrate(IntegrateParams& P, int X, int Y, int Z, float h, float tr
   if (!intr || X < 1 || Y < 1 || Z < 1 || W < 3 || H < 3 || K < 0) return MCX_ERR_ARG;
   if (!(h > 0.f) || !(trunc > 0.f) || max_w < 1 || max_w > 65535) return MCX_ERR_ARG;
   if ((int64_t)X * Y > 0xffffffffll) return MCX_ERR_UNSUPPORTED;
+  // frames are read through buffer resources of at most 2^31 - 1 bytes
+  if ((int64_t)W * H * 4 >= ((int64_t)1 << 31)) return MCX_ERR_UNSUPPORTED;
   hipLaunchKernelGGL(rayfac_kernel, dim3(blocks_for((int64_t)W * H)), dim3(256), 0, s, rayfac, W,
                      H, intr[0], intr[1], intr[2], intr[3]);
   MCX_HIP(hipGetLastError());
